@@ -1,0 +1,135 @@
+/*
+ * smcm.h - C ABI of the MI355X sparse matrix chain multiplication library (libsmcm.so).
+ *
+ * The library keeps a heterogeneous graph resident in HBM, builds the typed relation matrices
+ * of a meta-path on the device, runs the row-wise sparsity estimator, the chain-order DP and the
+ * row-wise Gustavson SpGEMM (uint64 path counts), and returns the result matrix with the
+ * reference's four timings.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Result semantics (the reference result is Boolean and unsorted; this adds counts):
+ *   rows    = the vertices of the start type in ascending global id (or the caller's source
+ *             rows, in the caller's order);
+ *   columns = global vertex ids of the end type, ascending within a row;
+ *   values  = number of meta-path instances, mod 2^64.  The structure is the Boolean product:
+ *             an entry whose count wraps to 0 mod 2^64 is still stored.
+ *
+ * Status codes: 0 ok; SMCM_EINVAL bad argument / malformed meta-path / type mismatch;
+ * SMCM_ENOMEM device allocation failed; SMCM_EBUDGET predicted result exceeds the budget;
+ * SMCM_EDEVICE any other HIP error.  Nothing throws across the ABI; the message of the last
+ * failure on the calling thread is returned by smcm_last_error().
+ *
+ * Threading: one smcm_ctx per host thread (each owns a HIP stream); handles created from a
+ * context are used with that context only.
+ */
+#ifndef SMCM_H
+#define SMCM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMCM_OK 0
+#define SMCM_EINVAL (-22)
+#define SMCM_ENOMEM (-12)
+#define SMCM_EBUDGET (-28)
+#define SMCM_EDEVICE (-5)
+
+typedef struct smcm_ctx smcm_ctx;
+typedef struct smcm_graph smcm_graph;
+typedef struct smcm_result smcm_result;
+
+/* The reference's globals es_time, dp_time, spgemm_time, smcm_time (efficient/resmcm.cpp:18-21),
+ * in seconds: total covers estimator + DP + all products, as smcm_time (:679-771). */
+typedef struct smcm_times {
+  double total;
+  double spgemm;
+  double dp;
+  double est;
+} smcm_times;
+
+/* Per-call statistics of the last smcm_chain_run* on a context. */
+typedef struct smcm_stats {
+  int64_t products;      /* intermediate products executed (sum over the plan's SpGEMMs) */
+  int64_t nnz;           /* nnz of the result */
+  int64_t rows_tier[4];  /* rows handled by tier: empty, wave, sort, bitmap (summed over SpGEMMs) */
+  int32_t n_spgemm;      /* SpGEMMs executed */
+  int32_t pairs[2 * 15]; /* executed plan (get_optimal_chain_order format) */
+  double c_root;         /* estimated nnz of the whole chain (c_[0][L-1]) */
+} smcm_stats;
+
+/* ---- context ---------------------------------------------------------------------------- */
+/* hbm_budget: bytes the library may hold for results and intermediates (0 = 90% of free HBM). */
+int smcm_ctx_create(int device, uint64_t hbm_budget, smcm_ctx** out);
+int smcm_ctx_destroy(smcm_ctx* ctx);
+/* Tuning / mode keys: "s_max", "m_max", "dense_div", "heavy_ub", "d_threads", "force_tier",
+ * "exact_cost_sum" (1 = sum c_ij in row order on one lane: bit-exact with the reference at one
+ * thread), "lds_bytes". */
+int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);
+int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* value);
+const char* smcm_last_error(void);
+int smcm_last_stats(smcm_ctx* ctx, smcm_stats* out);
+/* Wait for all work queued on the context's stream. */
+int smcm_ctx_sync(smcm_ctx* ctx);
+
+/* ---- graph (replaces HeteroGraph + the per-chain BoolMatrix construction,
+ *      efficient/resmcm.cpp:181-240, :374-396) ------------------------------------------- */
+/* Copies the HIN to the device: vtype[n], adj_ptr[n+1], adj_nbr[m], adj_etype[m] (m = adj_ptr[n];
+ * neighbour order = file order).  Builds the per-type index on the device. */
+int smcm_graph_upload(smcm_ctx* ctx, int32_t n, const int32_t* vtype, const int64_t* adj_ptr,
+                      const int32_t* adj_nbr, const int32_t* adj_etype, smcm_graph** out);
+int smcm_graph_free(smcm_graph* g);
+/* dims[t] for t < min(cap, ntypes); returns ntypes through *ntypes. */
+int smcm_graph_dims(const smcm_graph* g, int32_t* dims, int32_t cap, int32_t* ntypes);
+
+/* ---- chain (replaces build_pair / SSP_dynamic, efficient/resmcm.cpp:653-779) ------------ */
+/* vtypes[L+1], etypes[L]; src_rows: global ids of start-type vertices (NULL = all of them). */
+int smcm_chain_run(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
+                   int32_t L, const int32_t* src_rows, int64_t n_src, smcm_result** out,
+                   smcm_times* times);
+/* Same, executing the given plan (L-1 pairs in get_optimal_chain_order format) instead of the
+ * DP's; the estimator still runs (its densities drive row binning). */
+int smcm_chain_run_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
+                        const int32_t* etypes, int32_t L, const int32_t* src_rows, int64_t n_src,
+                        const int32_t* pairs, smcm_result** out, smcm_times* times);
+/* Estimator + DP only (rows_sparse_optimal_matrix_chain_order, :488-541, and
+ * get_optimal_chain_order, :563-573).  c_table, m_table: L*L doubles; s_table: L*L; pairs:
+ * 2*(L-1).  Any output may be NULL. */
+int smcm_chain_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
+                    int32_t L, const int32_t* src_rows, int64_t n_src, double* c_table,
+                    double* m_table, int32_t* s_table, int32_t* pairs);
+/* Estimated nnz of the whole chain, c_[0][L-1] (accuracy/res.cpp:494-523, 564-600). */
+int smcm_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
+                  int32_t L, const int32_t* src_rows, int64_t n_src, double* est_nnz);
+/* Row densities r_ij of chain position i for j = i..L-1: out[u*(L-i) + (j-i)], u over the rows
+ * of M_i (cap = number of doubles available in out). */
+int smcm_estimator_rows(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
+                        const int32_t* etypes, int32_t L, const int32_t* src_rows, int64_t n_src,
+                        int32_t i, double* out, int64_t cap);
+
+/* ---- result ----------------------------------------------------------------------------- */
+int32_t smcm_result_nrows(const smcm_result* r);
+int64_t smcm_result_nnz(const smcm_result* r);
+/* Host copies; any pointer may be NULL.  rowptr[nrows+1], cols[nnz] (global ids), vals[nnz]. */
+int smcm_result_copy(const smcm_result* r, int64_t* rowptr, int32_t* cols, uint64_t* vals);
+/* Rows [row_begin, row_end): rowptr gets row_end-row_begin+1 offsets relative to row_begin. */
+int smcm_result_copy_rows(const smcm_result* r, int32_t row_begin, int32_t row_end,
+                          int64_t* rowptr, int32_t* cols, uint64_t* vals);
+/* Global ids of the result rows. */
+int smcm_result_row_ids(const smcm_result* r, int32_t* rows);
+/* Per-row digest of the (local column, value) sequence, computed on the device. */
+int smcm_result_row_digest(const smcm_result* r, uint64_t* out);
+int smcm_result_free(smcm_result* r);
+
+/* ---- raw SpGEMM ------------------------------------------------------------------------- */
+/* C = A * B for host CSR operands (A m x k, B k x n; NULL values = all ones; B rows sorted).
+ * The result's columns are the operand's column indices. */
+int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_t* a_ptr,
+                    const int32_t* a_col, const uint64_t* a_val, const int64_t* b_ptr,
+                    const int32_t* b_col, const uint64_t* b_val, smcm_result** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

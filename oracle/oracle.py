@@ -1,0 +1,208 @@
+"""ctypes front-end of the oracle (CPU restatement of the reference path).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the CPU-baseline
+leg of ``bench.py``, and only as the checker.  The product (libsmcm.so) never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+class OrcCSR(C.Structure):
+    _fields_ = [
+        ("nrows", C.c_int32),
+        ("ncols", C.c_int32),
+        ("nnz", C.c_int64),
+        ("ptr", C.POINTER(C.c_int64)),
+        ("col", C.POINTER(C.c_int32)),
+        ("val", C.POINTER(C.c_uint64)),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+        L = C.CDLL(LIB_PATH)
+        PC = C.POINTER(OrcCSR)
+        L.orc_csr_free.argtypes = [PC]
+        L.orc_csr_free.restype = None
+        L.orc_typed_matrix.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                       C.c_int32, C.c_int32, C.c_void_p, C.c_int64, PC]
+        L.orc_estimate.argtypes = [C.c_int32, C.POINTER(PC), C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
+        L.orc_dp.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_dp.restype = C.c_double
+        L.orc_plan.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+        L.orc_plan.restype = None
+        L.orc_spgemm.argtypes = [PC, PC, PC]
+        L.orc_execute_plan.argtypes = [C.c_int32, C.POINTER(PC), C.c_void_p, PC]
+        L.orc_row_digest.argtypes = [PC, C.c_void_p]
+        L.orc_row_digest.restype = None
+        L.orc_products.argtypes = [PC, PC]
+        L.orc_products.restype = C.c_int64
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_set_threads.restype = None
+        _lib = L
+    return _lib
+
+
+class CSR:
+    """Owned copy of an oracle CSR as numpy arrays."""
+
+    def __init__(self, nrows, ncols, ptr, col, val=None):
+        self.nrows, self.ncols = int(nrows), int(ncols)
+        self.ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = None if val is None else np.ascontiguousarray(val, dtype=np.uint64)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.ptr[-1])
+
+    @classmethod
+    def _from_struct(cls, s: OrcCSR) -> "CSR":
+        nnz = int(s.nnz)
+        ptr = np.ctypeslib.as_array(s.ptr, shape=(s.nrows + 1,)).copy()
+        col = np.ctypeslib.as_array(s.col, shape=(nnz,)).copy() if nnz else np.zeros(0, np.int32)
+        val = None
+        if bool(s.val):
+            val = np.ctypeslib.as_array(s.val, shape=(nnz,)).copy() if nnz else np.zeros(0, np.uint64)
+        out = cls(s.nrows, s.ncols, ptr, col, val)
+        lib().orc_csr_free(C.byref(s))
+        return out
+
+    def _as_struct(self) -> OrcCSR:
+        s = OrcCSR()
+        s.nrows, s.ncols, s.nnz = self.nrows, self.ncols, self.nnz
+        s.ptr = self.ptr.ctypes.data_as(C.POINTER(C.c_int64))
+        s.col = self.col.ctypes.data_as(C.POINTER(C.c_int32))
+        s.val = self.val.ctypes.data_as(C.POINTER(C.c_uint64)) if self.val is not None else C.POINTER(C.c_uint64)()
+        s._keep = self  # keep arrays alive
+        return s
+
+    def values(self) -> np.ndarray:
+        return self.val if self.val is not None else np.ones(self.nnz, dtype=np.uint64)
+
+    def to_dense(self) -> np.ndarray:
+        d = np.zeros((self.nrows, self.ncols), dtype=np.uint64)
+        rows = np.repeat(np.arange(self.nrows), np.diff(self.ptr))
+        np.add.at(d, (rows, self.col), self.values())
+        return d
+
+
+def typed_matrix(g, ts: int, e: int, td: int, rows=None) -> CSR:
+    s = OrcCSR()
+    rp = None
+    if rows is not None:
+        rp = np.ascontiguousarray(rows, dtype=np.int32)
+    rc = lib().orc_typed_matrix(g.n, g.vertexTypes.ctypes.data, g.adj_ptr.ctypes.data, g.adj_nbr.ctypes.data,
+                                g.adj_et.ctypes.data, ts, e, td, None if rp is None else rp.ctypes.data,
+                                0 if rp is None else len(rp), C.byref(s))
+    if rc != 0:
+        raise ValueError("bad row selection")
+    return CSR._from_struct(s)
+
+
+def chain_matrices(g, mp, src_rows=None) -> list[CSR]:
+    """Matrices of the meta-path; src_rows (local ids of the start type) restricts M_0."""
+    mats = []
+    for i in range(mp.pathLen):
+        rows = src_rows if i == 0 else None
+        mats.append(typed_matrix(g, mp.vertexTypes[i], mp.edgeTypes[i], mp.vertexTypes[i + 1], rows))
+    return mats
+
+
+def _struct_array(mats):
+    structs = [m._as_struct() for m in mats]
+    arr = (C.POINTER(OrcCSR) * len(mats))(*[C.pointer(s) for s in structs])
+    return arr, structs
+
+
+def estimate(mats: list[CSR], coldims) -> tuple[np.ndarray, list[np.ndarray]]:
+    L = len(mats)
+    arr, keep = _struct_array(mats)
+    cd = np.ascontiguousarray(coldims, dtype=np.int32)
+    c = np.zeros(L * L, dtype=np.float64)
+    R = [np.zeros(m.nrows * (L - i), dtype=np.float64) for i, m in enumerate(mats)]
+    Rp = (C.c_void_p * L)(*[r.ctypes.data for r in R])
+    lib().orc_estimate(L, arr, cd.ctypes.data, c.ctypes.data, Rp)
+    return c.reshape(L, L), [r.reshape(m.nrows, L - i) for i, (r, m) in enumerate(zip(R, mats))]
+
+
+def dp(c: np.ndarray) -> tuple[float, np.ndarray, np.ndarray, np.ndarray]:
+    L = c.shape[0]
+    cc = np.ascontiguousarray(c, dtype=np.float64).ravel()
+    m = np.zeros(L * L, dtype=np.float64)
+    s = np.zeros(L * L, dtype=np.int32)
+    cost = lib().orc_dp(L, cc.ctypes.data, m.ctypes.data, s.ctypes.data)
+    pairs = np.zeros(2 * max(L - 1, 1), dtype=np.int32)
+    lib().orc_plan(L, s.ctypes.data, pairs.ctypes.data)
+    return cost, m.reshape(L, L), s.reshape(L, L), pairs[: 2 * (L - 1)].reshape(-1, 2)
+
+
+def spgemm(A: CSR, B: CSR) -> CSR:
+    out = OrcCSR()
+    a, b = A._as_struct(), B._as_struct()
+    if lib().orc_spgemm(C.byref(a), C.byref(b), C.byref(out)) != 0:
+        raise ValueError("shape mismatch")
+    return CSR._from_struct(out)
+
+
+def execute_plan(mats: list[CSR], pairs) -> CSR:
+    L = len(mats)
+    arr, keep = _struct_array(mats)
+    pp = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).ravel())
+    out = OrcCSR()
+    if lib().orc_execute_plan(L, arr, pp.ctypes.data if pp.size else None, C.byref(out)) != 0:
+        raise ValueError("plan execution failed")
+    return CSR._from_struct(out)
+
+
+def products(A: CSR, B: CSR) -> int:
+    a, b = A._as_struct(), B._as_struct()
+    return int(lib().orc_products(C.byref(a), C.byref(b)))
+
+
+def row_digest(Cm: CSR) -> np.ndarray:
+    out = np.zeros(Cm.nrows, dtype=np.uint64)
+    s = Cm._as_struct()
+    lib().orc_row_digest(C.byref(s), out.ctypes.data)
+    return out
+
+
+def coldims(g, mp) -> np.ndarray:
+    return np.array([g.dimensions[mp.vertexTypes[i + 1]] for i in range(mp.pathLen)], dtype=np.int32)
+
+
+def run_chain(g, mp, src_rows=None, plan=None):
+    """Full reference pipeline: matrices, estimator, DP, plan, product.  Returns
+    (result CSR, c table, R list, pairs)."""
+    mats = chain_matrices(g, mp, src_rows)
+    L = mp.pathLen
+    if L == 1:
+        return mats[0], np.zeros((1, 1)), [], np.zeros((0, 2), np.int32)
+    c, R = estimate(mats, coldims(g, mp))
+    _, _, _, pairs = dp(c)
+    if plan is not None:
+        pairs = np.asarray(plan, dtype=np.int32).reshape(-1, 2)
+    return execute_plan(mats, pairs), c, R, pairs
+
+
+def set_threads(t: int) -> None:
+    lib().orc_set_threads(int(t))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())

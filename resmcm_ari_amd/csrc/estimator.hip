@@ -1,0 +1,209 @@
+// estimator.hip - row-wise sparsity estimator (one pass per chain position i, all j > i fused).
+//
+// Reference: BoolMatrix::getSparsities (efficient/resmcm.cpp:442-452) and
+// DynamicOptimizer::compute_sparsities (:468-485), scheduled by :506-517:
+//   r_ii[u] = min(1, deg_u(M_i) / dims[t_{i+1}])
+//   r_ij[u] = 1 - prod_{v in row_u(M_i)} (1 - r_{i+1,j}[v])      (product in row order)
+//   c_ij    = sum_u r_ij[u] * dims[t_{j+1}]
+// Every r_ij[u] is computed with the reference's operation order in IEEE binary64 (the file is
+// built with -ffp-contract=off), so per-row densities are bit-identical to a one-thread run of
+// the reference.  The product for one row is a dependent chain and cannot be reassociated; rows
+// of degree <= kLightDeg use one lane each, longer rows one wave each (the wave gathers 64
+// factors in parallel and every lane multiplies them in order through readlane broadcasts).
+// Once all running products are exactly 0 they stay 0, so a row stops early - this is exact.
+//
+// c_ij is reduced in a fixed tree order (deterministic run to run); exact_sum instead sums in
+// row order on one lane, reproducing the reference's one-thread sum bit for bit.
+#include "kernels.h"
+
+namespace smcm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLightDeg = 32;
+constexpr int kMaxW = 15;
+
+template <int W>
+__global__ void __launch_bounds__(kThreads) est_light_kernel(const int64_t* __restrict__ ptr,
+                                                             const int32_t* __restrict__ col, int32_t nrows,
+                                                             double leaf_dim, const double* __restrict__ Rn,
+                                                             double* __restrict__ Ri, int32_t* __restrict__ heavy,
+                                                             int32_t* __restrict__ nheavy) {
+  const int32_t u = blockIdx.x * kThreads + threadIdx.x;
+  if (u >= nrows) return;
+  const int64_t p0 = ptr[u], p1 = ptr[u + 1];
+  const int64_t deg = p1 - p0;
+  double s = (double)deg / leaf_dim;
+  double* out = Ri + (size_t)u * (W + 1);
+  out[0] = s > 1 ? 1 : s;
+  if constexpr (W > 0) {
+    if (deg > kLightDeg) {
+      int32_t slot = atomicAdd(nheavy, 1);
+      heavy[slot] = u;
+      return;
+    }
+    double acc[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) acc[j] = 1;
+#pragma unroll 4
+    for (int64_t e = p0; e < p1; ++e) {
+      const double* r = Rn + (size_t)col[e] * W;
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[j] *= (1 - r[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) out[1 + j] = 1 - acc[j];
+  }
+}
+
+// one wave per heavy row
+template <int W>
+__global__ void __launch_bounds__(kThreads) est_heavy_kernel(const int64_t* __restrict__ ptr,
+                                                             const int32_t* __restrict__ col,
+                                                             const double* __restrict__ Rn, double* __restrict__ Ri,
+                                                             const int32_t* __restrict__ heavy,
+                                                             const int32_t* __restrict__ nheavy) {
+  const int lane = lane_id();
+  const int32_t nh = *nheavy;
+  const int32_t wpb = kThreads / kWave;
+  for (int32_t h = blockIdx.x * wpb + threadIdx.x / kWave; h < nh; h += gridDim.x * wpb) {
+    const int32_t u = heavy[h];
+    const int64_t p0 = ptr[u], p1 = ptr[u + 1];
+    double acc[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) acc[j] = 1;
+    double f[W];
+    {
+      const int64_t x = p0 + lane;
+      const double* r = Rn + (size_t)(x < p1 ? col[x] : 0) * W;
+#pragma unroll
+      for (int j = 0; j < W; ++j) f[j] = x < p1 ? (1 - r[j]) : 1.0;
+    }
+    for (int64_t b = p0; b < p1; b += kWave) {
+      // prefetch the next chunk's factors before the dependent multiply chain
+      double fn[W];
+      {
+        const int64_t x = b + kWave + lane;
+        const double* r = Rn + (size_t)(x < p1 ? col[x] : 0) * W;
+#pragma unroll
+        for (int j = 0; j < W; ++j) fn[j] = x < p1 ? (1 - r[j]) : 1.0;
+      }
+      const int cnt = (int)((p1 - b) < kWave ? (p1 - b) : kWave);
+      for (int k = 0; k < cnt; ++k) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) acc[j] *= readlane_f64(f[j], k);
+      }
+      bool all_zero = true;
+#pragma unroll
+      for (int j = 0; j < W; ++j) all_zero &= (acc[j] == 0.0);
+      if (all_zero) break;
+#pragma unroll
+      for (int j = 0; j < W; ++j) f[j] = fn[j];
+    }
+    if (lane == 0) {
+      double* out = Ri + (size_t)u * (W + 1);
+#pragma unroll
+      for (int j = 0; j < W; ++j) out[1 + j] = 1 - acc[j];
+    }
+  }
+}
+
+// per-block partial sums of r_ij[u] * dim_j in a fixed tree order
+template <int W>
+__global__ void __launch_bounds__(kThreads) est_partial_kernel(const double* __restrict__ Ri, int32_t nrows,
+                                                               const double* __restrict__ dimj,
+                                                               double* __restrict__ partial) {
+  __shared__ double sh[kThreads];
+  const int32_t u = blockIdx.x * kThreads + threadIdx.x;
+  for (int j = 0; j < W; ++j) {
+    double x = 0.0;
+    if (u < nrows) {
+      double r = Ri[(size_t)u * (W + 1) + 1 + j];
+      x = r * dimj[j];
+    }
+    sh[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * W + j] = sh[0];
+    __syncthreads();
+  }
+}
+
+// c_j = sum of block partials in block order (one thread per j)
+__global__ void est_final_kernel(const double* __restrict__ partial, int32_t nblocks, int W, double* __restrict__ c) {
+  const int j = threadIdx.x;
+  if (j >= W) return;
+  double s = 0.0;
+  for (int32_t b = 0; b < nblocks; ++b) s = s + partial[(size_t)b * W + j];
+  c[j] = s;
+}
+
+// exact mode: c_j = sequential sum over rows in row order, starting from 0 (one lane per j)
+__global__ void est_exact_kernel(const double* __restrict__ Ri, int32_t nrows, int W, const double* __restrict__ dimj,
+                                 double* __restrict__ c) {
+  const int j = threadIdx.x;
+  if (j >= W) return;
+  const double d = dimj[j];
+  double s = 0.0;
+  for (int32_t u = 0; u < nrows; ++u) {
+    double x = Ri[(size_t)u * (W + 1) + 1 + j] * d;
+    s = s + x;
+  }
+  c[j] = s;
+}
+
+template <int W>
+void run_pass(const EstimatorPass& p, DevAlloc& al) {
+  const Csr& M = *p.M;
+  const int32_t n = M.nrows;
+  hipStream_t st = al.stream;
+  DevBuf<int32_t> heavy(al, n > 0 ? n : 1), nheavy(al, 1);
+  fill_i32(nheavy.p, 1, 0, st);
+  const unsigned blocks = (unsigned)ceil_div(n > 0 ? n : 1, kThreads);
+  if (n > 0) {
+    est_light_kernel<W><<<blocks, kThreads, 0, st>>>(M.ptr, M.col, n, p.leaf_dim, p.Rn, p.Ri, heavy.p, nheavy.p);
+    SMCM_LAUNCH_CHECK();
+  }
+  if constexpr (W > 0) {
+    if (n > 0) {
+      est_heavy_kernel<W><<<2048, kThreads, 0, st>>>(M.ptr, M.col, p.Rn, p.Ri, heavy.p, nheavy.p);
+      SMCM_LAUNCH_CHECK();
+    }
+    DevBuf<double> dimj(al, W);
+    SMCM_HIP(hipMemcpyAsync(dimj.p, p.dimj, sizeof(double) * W, hipMemcpyHostToDevice, st));
+    if (p.exact_sum || n == 0) {
+      est_exact_kernel<<<1, 64, 0, st>>>(p.Ri, n, W, dimj.p, p.c_dev);
+      SMCM_LAUNCH_CHECK();
+    } else {
+      DevBuf<double> partial(al, (int64_t)blocks * W);
+      est_partial_kernel<W><<<blocks, kThreads, 0, st>>>(p.Ri, n, dimj.p, partial.p);
+      SMCM_LAUNCH_CHECK();
+      est_final_kernel<<<1, 64, 0, st>>>(partial.p, (int32_t)blocks, W, p.c_dev);
+      SMCM_LAUNCH_CHECK();
+    }
+    // the host copy of dimj above must complete before dimj is released
+    SMCM_HIP(hipStreamSynchronize(st));
+  }
+}
+
+}  // namespace
+
+void estimator_pass(const EstimatorPass& p, DevAlloc& al) {
+  switch (p.W) {
+#define SMCM_W(w) \
+  case w:         \
+    run_pass<w>(p, al); \
+    break;
+    SMCM_W(0) SMCM_W(1) SMCM_W(2) SMCM_W(3) SMCM_W(4) SMCM_W(5) SMCM_W(6) SMCM_W(7)
+    SMCM_W(8) SMCM_W(9) SMCM_W(10) SMCM_W(11) SMCM_W(12) SMCM_W(13) SMCM_W(14) SMCM_W(15)
+#undef SMCM_W
+    default:
+      throw HipError(-22, "estimator: chains longer than 16 matrices are not supported");
+  }
+}
+
+}  // namespace smcm

@@ -1,0 +1,80 @@
+// kernels.h - host launchers of the gfx950 kernels (all enqueue on DevAlloc::stream).
+#pragma once
+
+#include <vector>
+
+#include "alloc.h"
+#include "common.h"
+
+namespace smcm {
+
+// ---- scan.hip ---------------------------------------------------------------------------
+// out[0] = 0, out[i+1] = out[i] + in[i] for i < n   (out has n+1 entries; in != out)
+void scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n, DevAlloc& al);
+
+// ---- device_graph.hip -------------------------------------------------------------------
+// local[v] = rank of v among vertices of its type (ascending id); bins[t][local] = v.
+// bins[t] must hold dims[t] entries.
+void build_type_index(const int32_t* vtype, int32_t n, int32_t ntypes, int32_t* local,
+                      int32_t* const* bins, DevAlloc& al);
+
+// Typed relation matrix: rows = bins_ts[rows_sel[r]] (rows_sel may be null = all rows 0..nrows-1),
+// entries = neighbours with entry type et and vertex type td, in adjacency order, duplicates
+// kept; columns are local ids.  Sets out.sorted.
+void build_typed_csr(const int32_t* vtype, const int64_t* adj_ptr, const int32_t* adj_nbr,
+                     const int32_t* adj_et, const int32_t* local, const int32_t* bins_ts,
+                     const int32_t* rows_sel, int32_t nrows, int32_t et, int32_t td, int32_t ncols,
+                     Csr& out, DevAlloc& al);
+
+// out = rows rows_sel[0..nsel) of in (values copied when present)
+void csr_gather_rows(const Csr& in, const int32_t* rows_sel, int32_t nsel, Csr& out, DevAlloc& al);
+
+// dst[i] = map[src[i]]
+void map_columns(const int32_t* src, int64_t n, const int32_t* map, int32_t* dst, hipStream_t st);
+
+// per-row digest of the (column, value) sequence; same mixing as the oracle's orc_row_digest
+void row_digest(const Csr& c, uint64_t* out, hipStream_t st);
+
+// fill helpers (kernels, graph-capture safe)
+void fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t st);
+void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+
+// ---- estimator.hip ----------------------------------------------------------------------
+struct EstimatorPass {
+  const Csr* M = nullptr;        // M_i (rows of type t_i, file order)
+  double leaf_dim = 1.0;         // dims[t_{i+1}]
+  const double* Rn = nullptr;    // R_{i+1}: rows_{i+1} x W  (null when W == 0)
+  int W = 0;                     // L-1-i
+  const double* dimj = nullptr;  // host: dims[t_{j+1}] for j = i+1..L-1  (W values)
+  double* Ri = nullptr;          // out: nrows x (W+1); Ri[u][0] = r_ii, Ri[u][1+q] = r_{i,i+1+q}
+  double* c_dev = nullptr;       // out: W values c_{i,i+1+q}
+  bool exact_sum = false;        // sequential sum in row order (bit-exact with one thread)
+};
+void estimator_pass(const EstimatorPass& p, DevAlloc& al);
+
+// ---- spgemm.hip -------------------------------------------------------------------------
+struct SpgemmOpts {
+  int s_max = 64;          // rows with ub <= s_max: one wave per row, register sort
+  int m_max = 1024;        // rows with ub <= m_max (and sparse estimate): workgroup LDS sort
+  int dense_div = 64;      // estimated width >= ncols/dense_div sends the row to the bitmap tier
+  int64_t heavy_ub = 1 << 16;  // bitmap-tier rows with ub above this are scheduled first
+  int d_threads = 512;     // bitmap-tier workgroup size
+  int64_t lds_bytes = 65536;   // LDS available per workgroup
+  int cus = 256;           // compute units (persistent grid size)
+  int force_tier = -1;     // testing: 0 = wave, 1 = LDS sort, 2 = bitmap for every non-empty row
+};
+
+struct SpgemmStats {
+  int64_t products = 0;     // intermediate products (sum over A entries of |B row|)
+  int64_t nnz = 0;
+  int64_t rows[4] = {0, 0, 0, 0};  // empty / wave / sort / bitmap
+  int64_t windows = 1;
+  double ms_analysis = 0, ms_symbolic = 0, ms_numeric = 0;
+};
+
+// C = A * B (uint64 counts, columns sorted).  est_r (nullable, device) gives the estimator's
+// row densities r[u * est_stride]; est_dim = ncols of B as double.  B rows must be sorted.
+void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr& C,
+            const SpgemmOpts& o, SpgemmStats* st, DevAlloc& al, bool timing = false);
+
+}  // namespace smcm

@@ -1,0 +1,619 @@
+// spgemm.hip - row-wise Gustavson SpGEMM with uint64 path counts for gfx950.
+//
+// Reference: SpGEMM / row_wise_SpGEMM_based_on_bitmap (efficient/resmcm.cpp:586-647): for every
+// row u of A, the union over k in row_u(A) of row_k(B).  Here every column also carries the
+// number of paths, sum_k A[u,k] * B[k,v] mod 2^64, and rows are emitted sorted by column.
+//
+// Pipeline (all on one stream):
+//   1. analysis: len[e] = |row_{A.col[e]}(B)| for every entry of A, S = exclusive scan(len);
+//      ub(u) = S[A.ptr[u+1]] - S[A.ptr[u]] is the row's intermediate-product count and an
+//      upper bound of its width.
+//   2. binning: ub == 0 -> empty; ub <= s_max -> wave tier; ub <= m_max with a sparse estimate
+//      -> sort tier; otherwise the bitmap tier (rows with ub > heavy_ub first).  The estimator's
+//      predicted width r(u) * ncols is what sends a short-but-dense row to the bitmap tier.
+//   3. symbolic pass per tier -> exact row widths; scan -> C.ptr
+//   4. numeric pass per tier -> C.col (ascending), C.val
+//
+// Tiers:
+//   wave   : one 64-lane wave per row; products expanded into lanes, bitonic sort across the
+//            wave with xor shuffles, equal keys merged with a segmented scan.
+//   sort   : one workgroup per row; products expanded into LDS, bitonic sort in LDS, merged.
+//   bitmap : one workgroup per row (persistent grid pulling rows from a queue); a column bitmap
+//            of the row lives in LDS (one window of up to NW columns at a time).  Symbolic =
+//            popcount.  Numeric = bitmap, then a prefix count per 64-bit word gives every
+//            column its output slot, so columns come out sorted without a sort; one wave writes
+//            one 64-bit word (lane l <-> bit l) so output stores are coalesced.  Values are
+//            accumulated in LDS when the row fits (VCAP slots), otherwise with global atomics
+//            into the row's (pre-zeroed) output slots.
+// Products of an A entry are visited by the thread owning the entry when its B row is short,
+// or by a whole wave (coalesced B loads) when it is long.
+#include "kernels.h"
+
+#include <algorithm>
+#include <chrono>
+
+namespace smcm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kShortRow = 16;  // B rows up to this length are walked by one thread
+
+// ------------------------------------------------------------------------------------------
+// analysis
+__global__ void entry_len_kernel(const int32_t* __restrict__ acol, int64_t nnz, const int64_t* __restrict__ bptr,
+                                 int64_t* __restrict__ len) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t k = acol[e];
+    len[e] = bptr[k + 1] - bptr[k];
+  }
+}
+
+enum Tier { kEmpty = 0, kWaveTier = 1, kSortTier = 2, kBitmapTier = 3, kHeavyTier = 4, kNumBins = 5 };
+
+__global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __restrict__ aptr, int32_t m,
+                                                            const int64_t* __restrict__ S,
+                                                            const double* __restrict__ est_r, int est_stride,
+                                                            double ncols, int s_max, int m_max, int dense_div,
+                                                            int64_t heavy_ub, int force_tier,
+                                                            int32_t* __restrict__ bins, int32_t* __restrict__ counts) {
+  const int32_t u = blockIdx.x * kThreads + threadIdx.x;
+  int tier = -1;
+  if (u < m) {
+    const int64_t ub = S[aptr[u + 1]] - S[aptr[u]];
+    if (ub == 0) {
+      tier = kEmpty;
+    } else if (force_tier >= 0) {
+      tier = force_tier == 0 ? kWaveTier : force_tier == 1 ? kSortTier : kBitmapTier;
+      if (tier == kWaveTier && ub > 64) tier = kSortTier;
+      if (tier == kSortTier && ub > m_max) tier = kBitmapTier;
+    } else if (ub <= s_max) {
+      tier = kWaveTier;
+    } else {
+      bool dense = false;
+      if (est_r) dense = est_r[(size_t)u * est_stride] * ncols * (double)dense_div >= ncols;
+      tier = (ub <= m_max && !dense) ? kSortTier : kBitmapTier;
+    }
+    if (tier == kBitmapTier && ub > heavy_ub) tier = kHeavyTier;
+  }
+  // wave-aggregated append per bin (empty rows are only counted)
+  const int lane = lane_id();
+  for (int t = 0; t < kNumBins; ++t) {
+    const uint64_t mask = __ballot(tier == t);
+    if (!mask) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(&counts[t], (int32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (tier == t && t != kEmpty) bins[(size_t)t * m + base + lanes_below(mask)] = u;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// shared product traversal for workgroup tiers: f(col, val, pos) for every product of row
+// [rs, re) of A; pos = product index within the row (used by the sort tier).
+template <bool AV, bool BV, bool NEED_POS, class F>
+__device__ __forceinline__ void for_each_product(const int64_t rs, const int64_t re, const int32_t* __restrict__ acol,
+                                                 const uint64_t* __restrict__ aval,
+                                                 const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+                                                 const uint64_t* __restrict__ bval, const int64_t* __restrict__ S,
+                                                 int32_t* q, int* qn, F&& f) {
+  const int nthreads = blockDim.x;
+  const int nwaves = nthreads / kWave;
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  const int64_t S0 = NEED_POS ? S[rs] : 0;
+  for (int64_t base = rs; base < re; base += nthreads) {
+    const int64_t e = base + threadIdx.x;
+    if (e < re) {
+      const int32_t k = acol[e];
+      const int64_t b0 = bptr[k], b1 = bptr[k + 1];
+      if (b1 - b0 > kShortRow) {
+        int slot = atomicAdd(qn, 1);
+        q[slot] = (int32_t)(e - base);
+      } else if (b1 > b0) {
+        const uint64_t a = AV ? aval[e] : 1ull;
+        const int64_t p0 = NEED_POS ? S[e] - S0 : 0;
+        for (int64_t b = b0; b < b1; ++b) f(bcol[b], BV ? a * bval[b] : a, p0 + (b - b0));
+      }
+    }
+    __syncthreads();
+    const int n = *qn;
+    for (int qi = wid; qi < n; qi += nwaves) {
+      const int64_t ee = base + q[qi];
+      const int32_t k = acol[ee];
+      const int64_t b0 = bptr[k], b1 = bptr[k + 1];
+      const uint64_t a = AV ? aval[ee] : 1ull;
+      const int64_t p0 = NEED_POS ? S[ee] - S0 : 0;
+      for (int64_t b = b0 + lane; b < b1; b += kWave) f(bcol[b], BV ? a * bval[b] : a, p0 + (b - b0));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *qn = 0;
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int64_t block_sum_i64(int64_t x, int64_t* red) {
+  x = wave_sum_i64(x);
+  const int wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  __syncthreads();
+  if (lane_id() == 0) red[wid] = x;
+  __syncthreads();
+  int64_t s = 0;
+  for (int w = 0; w < nw; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// block exclusive scan of one int32 per thread
+__device__ __forceinline__ int32_t block_exscan_i32(int32_t x, int32_t* red, int32_t* total) {
+  const int lane = lane_id(), wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  int32_t incl = x;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  __syncthreads();
+  if (lane == kWave - 1) red[wid] = incl;
+  __syncthreads();
+  int32_t off = 0, tot = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (w < wid) off += red[w];
+    tot += red[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - x;
+}
+
+// ------------------------------------------------------------------------------------------
+// wave tier: ub <= 64.  One wave per row, 4 rows per block.
+template <bool NUMERIC, bool AV, bool BV>
+__global__ void __launch_bounds__(kThreads) wave_tier_kernel(
+    const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
+    const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
+    const int64_t* __restrict__ S, const int32_t* __restrict__ rows, int32_t nrows, int64_t* __restrict__ row_nnz,
+    const int64_t* __restrict__ cptr, int32_t* __restrict__ ccol, uint64_t* __restrict__ cval) {
+  __shared__ int32_t skey[kThreads];
+  __shared__ uint64_t sval[NUMERIC ? kThreads : 1];
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int32_t ri = blockIdx.x * (kThreads / kWave) + wid;
+  if (ri >= nrows) return;  // whole wave exits together
+  const int32_t u = rows[ri];
+  const int64_t rs = aptr[u], re = aptr[u + 1];
+  const int64_t S0 = S[rs];
+  const int ub = (int)(S[re] - S0);
+  int32_t* key = skey + wid * kWave;
+  uint64_t* val = sval + (NUMERIC ? wid * kWave : 0);
+  // expand: each lane writes the products of one entry at their row positions
+  for (int64_t cb = rs; cb < re; cb += kWave) {
+    const int64_t e = cb + lane;
+    if (e < re) {
+      const int32_t k = acol[e];
+      const int64_t b0 = bptr[k], b1 = bptr[k + 1];
+      const int off = (int)(S[e] - S0);
+      const uint64_t a = AV ? aval[e] : 1ull;
+      for (int64_t b = b0; b < b1; ++b) {
+        key[off + (b - b0)] = bcol[b];
+        if constexpr (NUMERIC) val[off + (b - b0)] = BV ? a * bval[b] : a;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int32_t k = lane < ub ? key[lane] : INT32_MAX;
+  uint64_t v = 0;
+  if constexpr (NUMERIC) v = lane < ub ? val[lane] : 0ull;
+  // bitonic sort across the wave (ascending by key)
+#pragma unroll
+  for (int size = 2; size <= kWave; size <<= 1) {
+#pragma unroll
+    for (int j = size >> 1; j > 0; j >>= 1) {
+      const int32_t ok = __shfl_xor(k, j);
+      uint64_t ov = 0;
+      if constexpr (NUMERIC) ov = __shfl_xor(v, j);
+      const bool asc = (lane & size) == 0;
+      const bool lower = (lane & j) == 0;
+      const bool take = (lower == asc) ? (ok < k) : (ok > k);
+      if (take) {
+        k = ok;
+        if constexpr (NUMERIC) v = ov;
+      }
+    }
+  }
+  const int32_t kprev = __shfl_up(k, 1);
+  const bool valid = k != INT32_MAX;
+  const bool head = valid && (lane == 0 || kprev != k);
+  const uint64_t hmask = __ballot(head);
+  if constexpr (!NUMERIC) {
+    if (lane == 0) row_nnz[u] = __popcll(hmask);
+  } else {
+    // segmented inclusive scan of values over runs of equal keys
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint64_t y = __shfl_up(v, o);
+      const int32_t ky = __shfl_up(k, o);
+      if (lane >= o && ky == k) v += y;
+    }
+    const int32_t knext = __shfl_down(k, 1);
+    const bool last = valid && (lane == kWave - 1 || knext != k);
+    if (last) {
+      const int64_t pos = cptr[u] + (__popcll(hmask & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull))) - 1);
+      ccol[pos] = k;
+      cval[pos] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// sort tier: one workgroup per row, products sorted in LDS.  Dynamic LDS: key[P] (+ val[P]).
+template <bool NUMERIC, bool AV, bool BV>
+__global__ void __launch_bounds__(kThreads) sort_tier_kernel(
+    const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
+    const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
+    const int64_t* __restrict__ S, const int32_t* __restrict__ rows, int32_t nrows, int cap,
+    int64_t* __restrict__ row_nnz, const int64_t* __restrict__ cptr, int32_t* __restrict__ ccol,
+    uint64_t* __restrict__ cval) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint64_t* val = (uint64_t*)smem;                                  // [cap] (numeric)
+  int32_t* key = (int32_t*)(smem + (NUMERIC ? (size_t)cap * 8 : 0));  // [cap]
+  __shared__ int32_t q[kThreads];
+  __shared__ int qn;
+  __shared__ int64_t red64[kThreads / kWave];
+  __shared__ int32_t red32[kThreads / kWave];
+  if (threadIdx.x == 0) qn = 0;
+  for (int32_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+    const int32_t u = rows[ri];
+    const int64_t rs = aptr[u], re = aptr[u + 1];
+    const int ub = (int)(S[re] - S[rs]);
+    int P = 64;
+    while (P < ub) P <<= 1;
+    for (int i = ub + threadIdx.x; i < P; i += kThreads) key[i] = INT32_MAX;
+    __syncthreads();
+    for_each_product<AV, BV, true>(rs, re, acol, aval, bptr, bcol, bval, S, q, &qn,
+                                   [&](int32_t c, uint64_t v, int64_t pos) {
+                                     key[pos] = c;
+                                     if constexpr (NUMERIC) val[pos] = v;
+                                   });
+    // bitonic sort of P keys (values follow)
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int j = size >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < (P >> 1); i += kThreads) {
+          const int lo = ((i / j) * 2 * j) + (i % j);
+          const int hi = lo + j;
+          const bool asc = (lo & size) == 0;
+          const int32_t a = key[lo], b = key[hi];
+          if ((a > b) == asc) {
+            key[lo] = b;
+            key[hi] = a;
+            if constexpr (NUMERIC) {
+              uint64_t t = val[lo];
+              val[lo] = val[hi];
+              val[hi] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // heads of runs; each thread owns a contiguous range of positions
+    const int per = (P + kThreads - 1) / kThreads;
+    const int p0 = threadIdx.x * per;
+    const int p1 = min(p0 + per, ub);
+    int32_t nh = 0;
+    for (int p = p0; p < p1; ++p) nh += (p == 0 || key[p] != key[p - 1]) ? 1 : 0;
+    if constexpr (!NUMERIC) {
+      int64_t tot = block_sum_i64(nh, red64);
+      if (threadIdx.x == 0) row_nnz[u] = tot;
+    } else {
+      int32_t total;
+      int32_t rank = block_exscan_i32(nh, red32, &total);
+      const int64_t obase = cptr[u];
+      for (int p = p0; p < p1; ++p) {
+        if (p == 0 || key[p] != key[p - 1]) {
+          const int32_t kk = key[p];
+          uint64_t s = val[p];
+          for (int x = p + 1; x < ub && key[x] == kk; ++x) s += val[x];
+          ccol[obase + rank] = kk;
+          cval[obase + rank] = s;
+          ++rank;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// bitmap tier.  Dynamic LDS: bm[NW/32] u32 | rank[NW/64] u32 | val[VCAP] u64 (numeric)
+struct BitmapLayout {
+  int64_t nw = 0;    // window width in columns (multiple of 128)
+  int64_t vcap = 0;  // LDS value slots
+  size_t bytes_sym() const { return (size_t)nw / 8; }
+  size_t bytes_num() const { return (size_t)nw / 8 + (size_t)nw / 16 + (size_t)vcap * 8; }
+};
+// offsets inside the dynamic LDS block (nw multiple of 128 keeps every section 8-byte aligned)
+__host__ __device__ inline int64_t bm_rank_off(int64_t nw) { return nw / 8; }
+__host__ __device__ inline int64_t bm_val_off(int64_t nw) { return nw / 8 + nw / 16; }
+
+template <bool NUMERIC, bool AV, bool BV>
+__global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol,
+                                   const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
+                                   const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
+                                   const int32_t* __restrict__ rows0, int32_t n0, const int32_t* __restrict__ rows1,
+                                   int32_t n1, int32_t ncols, int64_t nw, int64_t vcap, int32_t* __restrict__ queue,
+                                   int64_t* __restrict__ row_nnz, const int64_t* __restrict__ cptr,
+                                   int32_t* __restrict__ ccol, uint64_t* __restrict__ cval) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* bm = (uint32_t*)smem;
+  const uint64_t* bm64 = (const uint64_t*)smem;
+  uint32_t* rank = (uint32_t*)(smem + bm_rank_off(nw));
+  uint64_t* sval = (uint64_t*)(smem + bm_val_off(nw));
+  __shared__ int32_t q[1024];
+  __shared__ int qn;
+  __shared__ int32_t cur;
+  __shared__ int64_t red64[1024 / kWave];
+  __shared__ int32_t red32[1024 / kWave];
+  const int nthreads = blockDim.x;
+  const int nwaves = nthreads / kWave;
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  if (threadIdx.x == 0) qn = 0;
+  const int32_t total_rows = n0 + n1;
+  while (true) {
+    if (threadIdx.x == 0) cur = atomicAdd(queue, 1);
+    __syncthreads();
+    const int32_t ri = cur;
+    __syncthreads();
+    if (ri >= total_rows) break;
+    const int32_t u = ri < n0 ? rows0[ri] : rows1[ri - n0];
+    const int64_t rs = aptr[u], re = aptr[u + 1];
+    int64_t width = 0;
+    int64_t rowbase = 0;
+    bool lds_vals = false;
+    if constexpr (NUMERIC) {
+      rowbase = cptr[u];
+      width = cptr[u + 1] - rowbase;
+      lds_vals = width <= vcap;
+    }
+    int64_t done = 0;  // output slots produced by earlier windows
+    int64_t cnt = 0;
+    for (int64_t w0 = 0; w0 < ncols; w0 += nw) {
+      const int64_t wlen = min((int64_t)ncols - w0, nw);
+      const int64_t nwords32 = (wlen + 31) / 32;
+      const int64_t nwords64 = (wlen + 63) / 64;
+      for (int64_t i = threadIdx.x; i < nwords64 * 2; i += nthreads) bm[i] = 0u;
+      __syncthreads();
+      for_each_product<false, false, false>(rs, re, acol, nullptr, bptr, bcol, nullptr, nullptr, q, &qn,
+                                            [&](int32_t c, uint64_t, int64_t) {
+                                              const int64_t x = (int64_t)c - w0;
+                                              if (x >= 0 && x < wlen) atomicOr(&bm[x >> 5], 1u << (x & 31));
+                                            });
+      if constexpr (!NUMERIC) {
+        int64_t local = 0;
+        for (int64_t i = threadIdx.x; i < nwords32; i += nthreads) local += __popc(bm[i]);
+        cnt += block_sum_i64(local, red64);
+      } else {
+        // prefix count per 64-bit word: each thread scans a contiguous range of words
+        const int64_t per = (nwords64 + nthreads - 1) / nthreads;
+        const int64_t wa = threadIdx.x * per, wb = min(wa + per, nwords64);
+        int32_t s = 0;
+        for (int64_t i = wa; i < wb; ++i) s += __popcll(bm64[i]);
+        int32_t wtot;
+        int32_t off = block_exscan_i32(s, red32, &wtot);
+        for (int64_t i = wa; i < wb; ++i) {
+          rank[i] = (uint32_t)off;
+          off += __popcll(bm64[i]);
+        }
+        __syncthreads();
+        // columns (and zeroed values when accumulating globally): wave per word, lane = bit
+        for (int64_t i = wid; i < nwords64; i += nwaves) {
+          const uint64_t word = bm64[i];
+          if (!word) continue;
+          if ((word >> lane) & 1ull) {
+            const int64_t pos = rowbase + done + rank[i] + __popcll(word & ((1ull << lane) - 1ull));
+            ccol[pos] = (int32_t)(w0 + i * 64 + lane);
+            if (!lds_vals) cval[pos] = 0ull;
+          }
+        }
+        if (lds_vals) {
+          for (int64_t i = threadIdx.x; i < wtot; i += nthreads) sval[i] = 0ull;
+        } else {
+          __threadfence();
+        }
+        __syncthreads();
+        for_each_product<AV, BV, false>(rs, re, acol, aval, bptr, bcol, bval, nullptr, q, &qn,
+                                        [&](int32_t c, uint64_t v, int64_t) {
+                                          const int64_t x = (int64_t)c - w0;
+                                          if (x >= 0 && x < wlen) {
+                                            const int64_t wd = x >> 6;
+                                            const int32_t slot =
+                                                (int32_t)rank[wd] + __popcll(bm64[wd] & ((1ull << (x & 63)) - 1ull));
+                                            if (lds_vals)
+                                              atomicAdd((unsigned long long*)&sval[slot], (unsigned long long)v);
+                                            else
+                                              atomicAdd((unsigned long long*)&cval[rowbase + done + slot],
+                                                        (unsigned long long)v);
+                                          }
+                                        });
+        if (lds_vals) {
+          for (int64_t i = threadIdx.x; i < wtot; i += nthreads) cval[rowbase + done + i] = sval[i];
+        }
+        done += wtot;
+        __syncthreads();
+      }
+    }
+    if constexpr (!NUMERIC) {
+      if (threadIdx.x == 0) row_nnz[u] = cnt;
+    }
+    __syncthreads();
+  }
+}
+
+struct Launch {
+  hipStream_t st;
+};
+
+template <bool NUMERIC, bool AV, bool BV>
+void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const int32_t* counts_h,
+                  int64_t* row_nnz, Csr* C, const SpgemmOpts& o, const BitmapLayout& lay, int32_t* queue,
+                  hipStream_t st) {
+  const int32_t m = A.nrows;
+  const int32_t nW = counts_h[kWaveTier], nS = counts_h[kSortTier], nB = counts_h[kBitmapTier],
+                nH = counts_h[kHeavyTier];
+  const int64_t* cptr = C ? C->ptr : nullptr;
+  int32_t* ccol = C ? C->col : nullptr;
+  uint64_t* cval = C ? C->val : nullptr;
+  if (nW > 0) {
+    const unsigned g = (unsigned)ceil_div(nW, kThreads / kWave);
+    wave_tier_kernel<NUMERIC, AV, BV><<<g, kThreads, 0, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, S,
+                                                             bins + (size_t)kWaveTier * m, nW, row_nnz, cptr, ccol,
+                                                             cval);
+    SMCM_LAUNCH_CHECK();
+  }
+  if (nS > 0) {
+    int cap = 64;
+    while (cap < o.m_max) cap <<= 1;
+    const size_t lds = (size_t)cap * (NUMERIC ? 12 : 4);
+    const unsigned g = (unsigned)std::min<int64_t>(nS, (int64_t)o.cus * 16);
+    sort_tier_kernel<NUMERIC, AV, BV><<<g, kThreads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, S,
+                                                               bins + (size_t)kSortTier * m, nS, cap, row_nnz, cptr,
+                                                               ccol, cval);
+    SMCM_LAUNCH_CHECK();
+  }
+  if (nB + nH > 0) {
+    fill_i32(queue, 1, 0, st);
+    const size_t lds = NUMERIC ? lay.bytes_num() : lay.bytes_sym();
+    const int per_cu = std::max<int64_t>(1, std::min<int64_t>(4, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
+    const unsigned g = (unsigned)std::min<int64_t>(nB + nH, (int64_t)o.cus * per_cu);
+    bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(
+        A.ptr, A.col, A.val, B.ptr, B.col, B.val, bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m,
+        nB, B.ncols, lay.nw, lay.vcap, queue, row_nnz, cptr, ccol, cval);
+    SMCM_LAUNCH_CHECK();
+  }
+}
+
+template <bool NUMERIC>
+void dispatch(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const int32_t* counts_h,
+              int64_t* row_nnz, Csr* C, const SpgemmOpts& o, const BitmapLayout& lay, int32_t* queue,
+              hipStream_t st) {
+  const bool av = A.val != nullptr, bv = B.val != nullptr;
+  if (av && bv) launch_tiers<NUMERIC, true, true>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
+  else if (av) launch_tiers<NUMERIC, true, false>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
+  else if (bv) launch_tiers<NUMERIC, false, true>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
+  else launch_tiers<NUMERIC, false, false>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
+}
+
+template <bool NUMERIC, bool AV, bool BV>
+void set_lds_attr(size_t bytes) {
+  (void)hipFuncSetAttribute((const void*)bitmap_tier_kernel<NUMERIC, AV, BV>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+void set_all_lds_attrs(size_t bytes) {
+  static size_t done = 0;
+  if (bytes <= done) return;
+  set_lds_attr<false, false, false>(bytes);
+  set_lds_attr<false, true, false>(bytes);
+  set_lds_attr<false, false, true>(bytes);
+  set_lds_attr<false, true, true>(bytes);
+  set_lds_attr<true, false, false>(bytes);
+  set_lds_attr<true, true, false>(bytes);
+  set_lds_attr<true, false, true>(bytes);
+  set_lds_attr<true, true, true>(bytes);
+  (void)hipGetLastError();
+  done = bytes;
+}
+
+BitmapLayout choose_layout(int64_t ncols, const SpgemmOpts& o) {
+  // LDS budget for the dynamic part; static arrays and scan scratch take ~5 KB
+  const int64_t budget = o.lds_bytes - 8 * 1024;
+  BitmapLayout l;
+  const int64_t want = std::max<int64_t>(128, (ncols + 127) / 128 * 128);
+  // numeric: nw/8 + nw/16 + 8*vcap <= budget with vcap >= 2048
+  int64_t max_nw = (budget - 8 * 2048) * 16 / 3;
+  max_nw = std::max<int64_t>(128, max_nw / 128 * 128);
+  l.nw = std::min(want, max_nw);
+  l.vcap = (budget - (int64_t)l.bytes_num()) / 8;
+  if (l.vcap > (1 << 16)) l.vcap = 1 << 16;
+  if (l.vcap < 0) l.vcap = 0;
+  return l;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr& C, const SpgemmOpts& o,
+            SpgemmStats* stats, DevAlloc& al, bool timing) {
+  if (A.ncols != B.nrows) throw HipError(-22, "spgemm: inner dimensions differ");
+  if (!B.sorted) throw HipError(-22, "spgemm: right operand rows must be sorted");
+  hipStream_t st = al.stream;
+  const int32_t m = A.nrows;
+  auto t0 = std::chrono::steady_clock::now();
+  C.nrows = m;
+  C.ncols = B.ncols;
+  C.sorted = true;
+  C.ptr = al.alloc_n<int64_t>((int64_t)m + 1);
+  // 1. analysis
+  DevBuf<int64_t> len(al, A.nnz), S(al, A.nnz + 1);
+  if (A.nnz > 0) {
+    entry_len_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nnz, kThreads), 65536), kThreads, 0, st>>>(
+        A.col, A.nnz, B.ptr, len.p);
+    SMCM_LAUNCH_CHECK();
+  }
+  scan_exclusive_i64(len.p, S.p, A.nnz, al);
+  len.reset();
+  // 2. binning
+  DevBuf<int32_t> bins(al, (int64_t)kNumBins * (m > 0 ? m : 1)), counts(al, kNumBins + 1);
+  fill_i32(counts.p, kNumBins + 1, 0, st);
+  if (m > 0) {
+    bin_rows_kernel<<<(unsigned)ceil_div(m, kThreads), kThreads, 0, st>>>(
+        A.ptr, m, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub, o.force_tier,
+        bins.p, counts.p);
+    SMCM_LAUNCH_CHECK();
+  }
+  int32_t counts_h[kNumBins + 1];
+  int64_t products = 0;
+  SMCM_HIP(hipMemcpyAsync(counts_h, counts.p, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, st));
+  SMCM_HIP(hipMemcpyAsync(&products, S.p + A.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SMCM_HIP(hipStreamSynchronize(st));
+  if (stats) {
+    stats->products = products;
+    stats->rows[0] = counts_h[kEmpty];
+    stats->rows[1] = counts_h[kWaveTier];
+    stats->rows[2] = counts_h[kSortTier];
+    stats->rows[3] = counts_h[kBitmapTier] + counts_h[kHeavyTier];
+    stats->ms_analysis = ms_since(t0);
+  }
+  const BitmapLayout lay = choose_layout(B.ncols, o);
+  if (stats) stats->windows = ceil_div(std::max<int64_t>(B.ncols, 1), lay.nw);
+  set_all_lds_attrs(std::max(lay.bytes_num(), lay.bytes_sym()));
+  // 3. symbolic
+  auto t1 = std::chrono::steady_clock::now();
+  DevBuf<int64_t> row_nnz(al, m > 0 ? m : 1);
+  DevBuf<int32_t> queue(al, 1);
+  fill_i64(row_nnz.p, m, 0, st);
+  dispatch<false>(A, B, S.p, bins.p, counts_h, row_nnz.p, nullptr, o, lay, queue.p, st);
+  scan_exclusive_i64(row_nnz.p, C.ptr, m, al);
+  int64_t nnz = 0;
+  SMCM_HIP(hipMemcpyAsync(&nnz, C.ptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SMCM_HIP(hipStreamSynchronize(st));
+  if (stats) stats->ms_symbolic = ms_since(t1);
+  row_nnz.reset();
+  // 4. numeric
+  auto t2 = std::chrono::steady_clock::now();
+  C.nnz = nnz;
+  C.col = al.alloc_n<int32_t>(nnz);
+  C.val = al.alloc_n<uint64_t>(nnz);
+  dispatch<true>(A, B, S.p, bins.p, counts_h, nullptr, &C, o, lay, queue.p, st);
+  if (timing || stats) SMCM_HIP(hipStreamSynchronize(st));
+  if (stats) {
+    stats->ms_numeric = ms_since(t2);
+    stats->nnz = nnz;
+  }
+}
+
+}  // namespace smcm

@@ -1,0 +1,300 @@
+"""ctypes front-end of ``lib/libsmcm.so`` (the C ABI in include/smcm.h).
+
+``build_pair`` keeps the reference entry's meaning (efficient/resmcm.cpp:774-779): a meta-path
+over a resident graph in, the result matrix and the times ``es_time / dp_time / spgemm_time /
+smcm_time`` out.  There is no CPU fallback: loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _build
+from .hin import HeteroGraph, MetaPath
+
+SMCM_OK = 0
+SMCM_EINVAL = -22
+SMCM_ENOMEM = -12
+SMCM_EBUDGET = -28
+SMCM_EDEVICE = -5
+
+LIB_PATH = os.path.join(_build.LIBDIR, "libsmcm.so")
+
+# every entry point declared in include/smcm.h
+EXPORTS = [
+    "smcm_ctx_create", "smcm_ctx_destroy", "smcm_set_option", "smcm_get_option", "smcm_last_error",
+    "smcm_last_stats", "smcm_ctx_sync", "smcm_graph_upload", "smcm_graph_free", "smcm_graph_dims",
+    "smcm_chain_run", "smcm_chain_run_plan", "smcm_chain_plan", "smcm_estimate", "smcm_estimator_rows",
+    "smcm_result_nrows", "smcm_result_nnz", "smcm_result_copy", "smcm_result_copy_rows", "smcm_result_row_ids",
+    "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr",
+]
+
+
+class Times(C.Structure):
+    _fields_ = [("total", C.c_double), ("spgemm", C.c_double), ("dp", C.c_double), ("est", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("products", C.c_int64), ("nnz", C.c_int64), ("rows_tier", C.c_int64 * 4),
+                ("n_spgemm", C.c_int32), ("pairs", C.c_int32 * 30), ("c_root", C.c_double)]
+
+
+class SmcmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libsmcm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, u64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+        sig = {
+            "smcm_ctx_create": [C.c_int, u64, vp],
+            "smcm_ctx_destroy": [vp],
+            "smcm_set_option": [vp, C.c_char_p, i64],
+            "smcm_get_option": [vp, C.c_char_p, vp],
+            "smcm_last_stats": [vp, vp],
+            "smcm_ctx_sync": [vp],
+            "smcm_graph_upload": [vp, i32, vp, vp, vp, vp, vp],
+            "smcm_graph_free": [vp],
+            "smcm_graph_dims": [vp, vp, i32, vp],
+            "smcm_chain_run": [vp, vp, vp, vp, i32, vp, i64, vp, vp],
+            "smcm_chain_run_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp],
+            "smcm_chain_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp, vp],
+            "smcm_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
+            "smcm_estimator_rows": [vp, vp, vp, vp, i32, vp, i64, i32, vp, i64],
+            "smcm_result_nrows": [vp],
+            "smcm_result_nnz": [vp],
+            "smcm_result_copy": [vp, vp, vp, vp],
+            "smcm_result_copy_rows": [vp, i32, i32, vp, vp, vp],
+            "smcm_result_row_ids": [vp, vp],
+            "smcm_result_row_digest": [vp, vp],
+            "smcm_result_free": [vp],
+            "smcm_spgemm_csr": [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.smcm_result_nrows.restype = C.c_int32
+        L.smcm_result_nnz.restype = C.c_int64
+        L.smcm_last_error.restype = C.c_char_p
+        L.smcm_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != SMCM_OK:
+        raise SmcmError(rc, load().smcm_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One device context (own HIP stream, stream-ordered memory pool)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._h = C.c_void_p()
+        _check(load().smcm_ctx_create(device, hbm_budget, C.byref(self._h)))
+
+    def set_option(self, key: str, value: int) -> None:
+        _check(load().smcm_set_option(self._h, key.encode(), int(value)))
+
+    def get_option(self, key: str) -> int:
+        v = C.c_int64()
+        _check(load().smcm_get_option(self._h, key.encode(), C.byref(v)))
+        return int(v.value)
+
+    def last_stats(self) -> dict:
+        s = Stats()
+        _check(load().smcm_last_stats(self._h, C.byref(s)))
+        n = max(s.n_spgemm, 0)
+        return {"products": int(s.products), "nnz": int(s.nnz), "rows_tier": list(s.rows_tier),
+                "n_spgemm": int(s.n_spgemm), "pairs": [int(x) for x in s.pairs[: 2 * n]], "c_root": float(s.c_root)}
+
+    def sync(self) -> None:
+        _check(load().smcm_ctx_sync(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            load().smcm_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def spgemm_csr(self, m, k, n, a_ptr, a_col, a_val, b_ptr, b_col, b_val) -> "Result":
+        ap = np.ascontiguousarray(a_ptr, np.int64)
+        ac = np.ascontiguousarray(a_col, np.int32)
+        av = None if a_val is None else np.ascontiguousarray(a_val, np.uint64)
+        bp = np.ascontiguousarray(b_ptr, np.int64)
+        bc = np.ascontiguousarray(b_col, np.int32)
+        bv = None if b_val is None else np.ascontiguousarray(b_val, np.uint64)
+        h = C.c_void_p()
+        _check(load().smcm_spgemm_csr(self._h, m, k, n, _ptr(ap), _ptr(ac), _ptr(av), _ptr(bp), _ptr(bc), _ptr(bv),
+                                      C.byref(h)))
+        return Result(h, self)
+
+
+class Graph:
+    """A HeteroGraph resident in HBM (typed relation matrices are built on the device and
+    cached per meta-path edge)."""
+
+    def __init__(self, ctx: Context, hg: HeteroGraph):
+        self.ctx = ctx
+        self.hg = hg
+        self._h = C.c_void_p()
+        _check(load().smcm_graph_upload(ctx._h, hg.n, _ptr(hg.vertexTypes), _ptr(hg.adj_ptr), _ptr(hg.adj_nbr),
+                                        _ptr(hg.adj_et), C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            load().smcm_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _path(mp: MetaPath):
+        vt = np.ascontiguousarray(mp.vertexTypes, np.int32)
+        et = np.ascontiguousarray(mp.edgeTypes if mp.pathLen else [0], np.int32)
+        return vt, et
+
+    @staticmethod
+    def _src(src_rows):
+        if src_rows is None:
+            return None, 0
+        s = np.ascontiguousarray(src_rows, np.int32)
+        return s, len(s)
+
+    def run(self, mp: MetaPath, src_rows=None, plan=None) -> tuple["Result", dict]:
+        vt, et = self._path(mp)
+        s, ns = self._src(src_rows)
+        h = C.c_void_p()
+        t = Times()
+        if plan is None:
+            _check(load().smcm_chain_run(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen, _ptr(s), ns,
+                                         C.byref(h), C.byref(t)))
+        else:
+            pp = np.ascontiguousarray(np.asarray(plan, np.int32).ravel())
+            _check(load().smcm_chain_run_plan(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen, _ptr(s), ns,
+                                              _ptr(pp), C.byref(h), C.byref(t)))
+        times = {"total": t.total, "spgemm": t.spgemm, "dp": t.dp, "est": t.est}
+        return Result(h, self.ctx), times
+
+    def plan(self, mp: MetaPath, src_rows=None):
+        L = mp.pathLen
+        vt, et = self._path(mp)
+        s, ns = self._src(src_rows)
+        c = np.zeros(L * L, np.float64)
+        m = np.zeros(L * L, np.float64)
+        st = np.zeros(L * L, np.int32)
+        pairs = np.zeros(max(2 * (L - 1), 1), np.int32)
+        _check(load().smcm_chain_plan(self.ctx._h, self._h, _ptr(vt), _ptr(et), L, _ptr(s), ns, _ptr(c), _ptr(m),
+                                      _ptr(st), _ptr(pairs)))
+        return c.reshape(L, L), m.reshape(L, L), st.reshape(L, L), pairs[: 2 * (L - 1)].reshape(-1, 2)
+
+    def estimate(self, mp: MetaPath, src_rows=None) -> float:
+        vt, et = self._path(mp)
+        s, ns = self._src(src_rows)
+        x = C.c_double()
+        _check(load().smcm_estimate(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen, _ptr(s), ns, C.byref(x)))
+        return float(x.value)
+
+    def estimator_rows(self, mp: MetaPath, i: int, src_rows=None) -> np.ndarray:
+        L = mp.pathLen
+        vt, et = self._path(mp)
+        s, ns = self._src(src_rows)
+        nrows = ns if (i == 0 and s is not None) else int(self.hg.dimensions[mp.vertexTypes[i]])
+        out = np.zeros(max(nrows * (L - i), 1), np.float64)
+        _check(load().smcm_estimator_rows(self.ctx._h, self._h, _ptr(vt), _ptr(et), L, _ptr(s), ns, i, _ptr(out),
+                                          out.size))
+        return out[: nrows * (L - i)].reshape(nrows, L - i)
+
+
+class Result:
+    def __init__(self, h, ctx: Context):
+        self._h = h
+        self.ctx = ctx
+
+    @property
+    def nrows(self) -> int:
+        return int(load().smcm_result_nrows(self._h))
+
+    @property
+    def nnz(self) -> int:
+        return int(load().smcm_result_nnz(self._h))
+
+    def to_host(self):
+        n, nnz = self.nrows, self.nnz
+        ptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1), np.uint64)
+        _check(load().smcm_result_copy(self._h, _ptr(ptr), _ptr(col), _ptr(val)))
+        return ptr, col[:nnz], val[:nnz]
+
+    def rows(self, b: int, e: int):
+        ptr = np.zeros(e - b + 1, np.int64)
+        _check(load().smcm_result_copy_rows(self._h, b, e, _ptr(ptr), None, None))
+        cnt = int(ptr[-1])
+        col = np.zeros(max(cnt, 1), np.int32)
+        val = np.zeros(max(cnt, 1), np.uint64)
+        _check(load().smcm_result_copy_rows(self._h, b, e, _ptr(ptr), _ptr(col), _ptr(val)))
+        return ptr, col[:cnt], val[:cnt]
+
+    def row_ids(self) -> np.ndarray:
+        out = np.zeros(max(self.nrows, 1), np.int32)
+        _check(load().smcm_result_row_ids(self._h, _ptr(out)))
+        return out[: self.nrows]
+
+    def row_digest(self) -> np.ndarray:
+        out = np.zeros(max(self.nrows, 1), np.uint64)
+        _check(load().smcm_result_row_digest(self._h, _ptr(out)))
+        return out[: self.nrows]
+
+    def free(self) -> None:
+        if self._h:
+            load().smcm_result_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class PairTimes:
+    es_time: float
+    dp_time: float
+    spgemm_time: float
+    smcm_time: float
+
+
+def build_pair(graph: Graph, path: MetaPath, src_rows=None) -> tuple[Result, PairTimes]:
+    """The reference's build_pair/SSP_dynamic (efficient/resmcm.cpp:653-779): returns the chain
+    product and the four timings."""
+    r, t = graph.run(path, src_rows)
+    return r, PairTimes(es_time=t["est"], dp_time=t["dp"], spgemm_time=t["spgemm"], smcm_time=t["total"])

@@ -1,0 +1,169 @@
+"""Whole chains on the GPU vs the oracle: typed matrices, estimator (bit-exact per row), DP plan
+and the product (bit-exact rowptr / sorted global columns / uint64 counts)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import assert_same_csr
+from resmcm_ari_amd import configs, smcm
+
+pytestmark = pytest.mark.gpu
+
+C_RTOL = 1e-12  # default tree-ordered c_ij sums vs the reference's one-thread row-order sum
+
+
+def all_paths(schema, L):
+    """Every schema-valid meta-path with L matrices (letters joined by '-')."""
+    sch = configs.BIB_SCHEMA if schema == "bib" else configs.LOC_SCHEMA
+    names = {v: k for k, v in (configs.BIB_NAMES if schema == "bib" else configs.LOC_NAMES).items()}
+    out = []
+    stack = [[s] for s in sch]
+    while stack:
+        p = stack.pop()
+        if len(p) == L + 1:
+            out.append("-".join(names[t] for t in p))
+            continue
+        for t in sch[p[-1]]:
+            stack.append(p + [t])
+    return sorted(out)
+
+
+@pytest.fixture(scope="module")
+def tiny_graphs(ctx):
+    gb = configs.tiny_bib().build()
+    gl = configs.tiny_loc().build()
+    return {"bib": (gb, smcm.Graph(ctx, gb)), "loc": (gl, smcm.Graph(ctx, gl))}
+
+
+def check_chain(hg, G, mp, src=None, exact_cost=False):
+    local = hg.local_ids()
+    src_local = None if src is None else local[np.asarray(src)]
+    ref, c_ref, R_ref, pairs_ref = O.run_chain(hg, mp, src_rows=src_local)
+    r, times = G.run(mp, src_rows=src)
+    ptr, col, val = r.to_host()
+    bins = hg.bins
+    assert_same_csr(ptr, col, val, ref, col_map=bins[mp.vertexTypes[-1]], what=mp.toString())
+    rows = r.row_ids()
+    expect_rows = bins[mp.vertexTypes[0]] if src is None else np.asarray(src)
+    np.testing.assert_array_equal(rows, expect_rows)
+    r.free()
+    if mp.pathLen > 1:
+        c, m, s, pairs = G.plan(mp, src_rows=src)
+        if exact_cost:
+            np.testing.assert_array_equal(c, c_ref)
+        else:
+            np.testing.assert_allclose(c, c_ref, rtol=C_RTOL, atol=0)
+            np.testing.assert_array_equal(pairs, pairs_ref)
+        for i in range(mp.pathLen):
+            Ri = G.estimator_rows(mp, i, src_rows=src)
+            np.testing.assert_array_equal(Ri, R_ref[i], err_msg=f"{mp} r_{i},* differs")
+    return times
+
+
+@pytest.mark.parametrize("schema,L", [("bib", 1), ("bib", 2), ("bib", 3), ("bib", 4), ("loc", 2), ("loc", 4)])
+def test_all_paths_tiny(tiny_graphs, schema, L):
+    hg, G = tiny_graphs[schema]
+    for p in all_paths(schema, L):
+        check_chain(hg, G, configs.metapath_from_letters(p, schema))
+
+
+def test_long_chain_tiny(tiny_graphs):
+    hg, G = tiny_graphs["loc"]
+    for p in ["U-L-C-L-Y-L-U-L-S-L-U", "U-L-C-L-U-L-Y-L-S-L-U-L-U", "L-Y-L-C-L-S-L-U-L"]:
+        check_chain(hg, G, configs.metapath_from_letters(p, "loc"))
+
+
+def test_exact_cost_sum(ctx, tiny_graphs):
+    """exact_cost_sum reproduces the reference's one-thread c_ij bit for bit."""
+    hg, G = tiny_graphs["bib"]
+    ctx.set_option("exact_cost_sum", 1)
+    try:
+        for p in ["A-P-V-P-A", "P-A-P-T-P", "T-P-A-P-V"]:
+            check_chain(hg, G, configs.metapath_from_letters(p, "bib"), exact_cost=True)
+    finally:
+        ctx.set_option("exact_cost_sum", 0)
+
+
+def test_source_subset_is_slice(tiny_graphs):
+    hg, G = tiny_graphs["bib"]
+    mp = configs.metapath_from_letters("A-P-V-P-A", "bib")
+    full, _ = G.run(mp)
+    fptr, fcol, fval = full.to_host()
+    rng = np.random.default_rng(5)
+    authors = hg.bins[configs.A]
+    src = rng.choice(authors, size=17, replace=False)
+    check_chain(hg, G, mp, src=src)
+    sub, _ = G.run(mp, src_rows=src)
+    sptr, scol, sval = sub.to_host()
+    local = hg.local_ids()
+    for q, v in enumerate(src):
+        u = local[v]
+        np.testing.assert_array_equal(scol[sptr[q]:sptr[q + 1]], fcol[fptr[u]:fptr[u + 1]])
+        np.testing.assert_array_equal(sval[sptr[q]:sptr[q + 1]], fval[fptr[u]:fptr[u + 1]])
+
+
+def test_every_plan_same_result(tiny_graphs):
+    """Order independence: all parenthesisations give identical bits."""
+    hg, G = tiny_graphs["bib"]
+    mp = configs.metapath_from_letters("A-P-T-P-V-P", "bib")
+    base, _ = G.run(mp)
+    b = base.to_host()
+    L = mp.pathLen
+    for plan in enumerate_plans(L):
+        r, _ = G.run(mp, plan=plan)
+        o = r.to_host()
+        for x, y in zip(b, o):
+            np.testing.assert_array_equal(x, y)
+
+
+def enumerate_plans(L):
+    """All post-order plans (get_optimal_chain_order format) for L matrices."""
+
+    def trees(i, j):
+        if i == j:
+            yield ("leaf", i)
+            return
+        for k in range(i, j):
+            for lt in trees(i, k):
+                for rt in trees(k + 1, j):
+                    yield ("node", lt, rt)
+
+    out = []
+    for t in trees(0, L - 1):
+        pairs = []
+
+        def emit(n):
+            if n[0] == "leaf":
+                return n[1]
+            a = emit(n[1])
+            b = emit(n[2])
+            pairs.append((a, b))
+            return -1
+
+        emit(t)
+        out.append(pairs)
+    return out
+
+
+def test_bad_inputs(ctx, tiny_graphs):
+    hg, G = tiny_graphs["bib"]
+    from resmcm_ari_amd.hin import MetaPath
+    with pytest.raises(smcm.SmcmError) as e:
+        G.run(MetaPath(vertexTypes=[1, 9], edgeTypes=[3]))
+    assert e.value.code == smcm.SMCM_EINVAL
+    with pytest.raises(smcm.SmcmError):
+        G.run(configs.metapath_from_letters("A-P-A", "bib"), src_rows=[0])  # vertex 0 is a paper
+    with pytest.raises(smcm.SmcmError):
+        G.run(configs.metapath_from_letters("A-P-A", "bib"), plan=[[0, 0]])
+
+
+@pytest.fixture(scope="module")
+def cfg1(ctx):
+    hg = configs.CFG1.build()
+    return hg, smcm.Graph(ctx, hg)
+
+
+@pytest.mark.parametrize("p", ["A-P-V-P-A", "P-T-P-V", "P-V-P-A", "V-P-A-P-V"])
+def test_config1(cfg1, p):
+    hg, G = cfg1
+    check_chain(hg, G, configs.metapath_from_letters(p, "bib"))

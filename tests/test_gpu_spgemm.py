@@ -1,0 +1,84 @@
+"""SpGEMM tiers on the GPU vs the oracle (bit-exact rowptr, sorted columns, uint64 values)."""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import assert_same_csr, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ctx, A, B):
+    r = ctx.spgemm_csr(A.nrows, A.ncols, B.ncols, A.ptr, A.col, A.val, B.ptr, B.col, B.val)
+    out = r.to_host()
+    r.free()
+    return out
+
+
+CASES = [
+    # (m, k, n, A row length, B row length)
+    ("tiny", 5, 4, 3, lambda r: r.integers(0, 3), lambda r: r.integers(0, 3)),
+    ("short", 300, 200, 150, lambda r: r.integers(0, 6), lambda r: r.integers(0, 8)),
+    ("medium", 200, 300, 2000, lambda r: r.integers(10, 60), lambda r: r.integers(0, 40)),
+    ("wide", 64, 400, 30000, lambda r: r.integers(0, 200), lambda r: r.integers(50, 400)),
+    ("hub", 8, 3000, 5000, lambda r: r.integers(1000, 3000), lambda r: r.integers(0, 5)),
+    ("skew", 400, 500, 7000, lambda r: int(r.zipf(1.5)) % 500, lambda r: int(r.zipf(1.3)) % 3000),
+]
+
+
+@pytest.mark.parametrize("tier", [-1, 0, 1, 2])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("vals", ["none", "A", "B", "AB"])
+def test_spgemm_tiers(ctx, case, tier, vals):
+    name, m, k, n, la, lb = case
+    rng = np.random.default_rng(zlib.crc32(f"{name}/{tier}/{vals}".encode()))
+    A = random_csr(rng, m, k, row_len=la, with_vals=vals in ("A", "AB"))
+    B = random_csr(rng, k, n, row_len=lb, with_vals=vals in ("B", "AB"))
+    ref = O.spgemm(A, B)
+    ctx.set_option("force_tier", tier)
+    try:
+        ptr, col, val = _run(ctx, A, B)
+    finally:
+        ctx.set_option("force_tier", -1)
+    assert_same_csr(ptr, col, val, ref, what=f"{name} tier={tier} vals={vals}")
+
+
+def test_spgemm_duplicates_in_A(ctx):
+    """Duplicate entries in a row of A count every path (multiplicities add)."""
+    rng = np.random.default_rng(3)
+    A = random_csr(rng, 100, 50, row_len=lambda r: r.integers(0, 30), dup=True, sorted_rows=False)
+    B = random_csr(rng, 50, 80, row_len=lambda r: r.integers(0, 10))
+    ref = O.spgemm(A, B)
+    for tier in (-1, 0, 1, 2):
+        ctx.set_option("force_tier", tier)
+        try:
+            ptr, col, val = _run(ctx, A, B)
+        finally:
+            ctx.set_option("force_tier", -1)
+        assert_same_csr(ptr, col, val, ref, what=f"dup tier={tier}")
+
+
+def test_spgemm_empty(ctx):
+    A = O.CSR(3, 4, np.zeros(4, np.int64), np.zeros(0, np.int32))
+    B = O.CSR(4, 5, np.zeros(5, np.int64), np.zeros(0, np.int32))
+    ptr, col, val = _run(ctx, A, B)
+    assert list(ptr) == [0, 0, 0, 0] and len(col) == 0
+
+
+def test_spgemm_wraparound(ctx):
+    """Counts are exact mod 2^64; an entry whose sum wraps to 0 stays structural."""
+    big = np.uint64(2**63)
+    A = O.CSR(1, 2, np.array([0, 2], np.int64), np.array([0, 1], np.int32), np.array([big, big], np.uint64))
+    B = O.CSR(2, 3, np.array([0, 2, 3], np.int64), np.array([0, 2, 2], np.int32), np.array([1, 1, 1], np.uint64))
+    ref = O.spgemm(A, B)
+    for tier in (-1, 0, 1, 2):
+        ctx.set_option("force_tier", tier)
+        try:
+            ptr, col, val = _run(ctx, A, B)
+        finally:
+            ctx.set_option("force_tier", -1)
+        assert list(col) == [0, 2]
+        assert list(val) == [int(big), 0]
+        assert_same_csr(ptr, col, val, ref, what=f"wrap tier={tier}")
