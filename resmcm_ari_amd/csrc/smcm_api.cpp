@@ -83,6 +83,7 @@ struct smcm_ctx {
   smcm::SpgemmOpts opts;
   bool exact_cost_sum = false;
   smcm_stats last{};
+  int64_t live = 0;  // graphs and results created from this context and not yet freed
 };
 
 struct smcm_graph {
@@ -396,6 +397,7 @@ smcm_result* make_result(Chain& ch, Csr C) {
   auto* r = new smcm_result;
   r->ctx = ctx;
   r->C = C;
+  ctx->live++;
   const int32_t tcol = ch.vts[ch.L];
   const int32_t dcol = ch.g->dims[tcol];
   r->col_map = ctx->al.alloc_n<int32_t>(dcol);
@@ -514,6 +516,8 @@ int smcm_ctx_create(int device, uint64_t hbm_budget, smcm_ctx** out) {
 int smcm_ctx_destroy(smcm_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
+    if (ctx->live != 0)
+      fail(SMCM_EINVAL, std::to_string(ctx->live) + " graph/result handle(s) still alive; free them first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
@@ -635,6 +639,7 @@ int smcm_graph_upload(smcm_ctx* ctx, int32_t n, const int32_t* vtype, const int6
     for (int32_t t = 0; t < g->ntypes; ++t) g->bins[t] = al.alloc_n<int32_t>(g->dims[t]);
     smcm::build_type_index(g->vtype, n, g->ntypes, g->local, g->bins.data(), al);
     SMCM_HIP(hipStreamSynchronize(al.stream));
+    ctx->live++;
     *out = g.release();
   });
 }
@@ -652,6 +657,7 @@ int smcm_graph_free(smcm_graph* g) {
     al.free(g->adj_et);
     al.free(g->local);
     (void)hipStreamSynchronize(al.stream);
+    g->ctx->live--;
     delete g;
   });
 }
@@ -789,6 +795,7 @@ int smcm_result_free(smcm_result* r) {
     (void)hipSetDevice(r->ctx->device);
     free_csr(r->ctx->al, r->C);
     r->ctx->al.free(r->col_map);
+    r->ctx->live--;
     delete r;
   });
 }
@@ -847,6 +854,7 @@ int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_
     auto* r = new smcm_result;
     r->ctx = ctx;
     r->C = C;
+    ctx->live++;
     r->row_ids.resize(m);
     for (int32_t i = 0; i < m; ++i) r->row_ids[i] = i;
     SMCM_HIP(hipStreamSynchronize(al.stream));

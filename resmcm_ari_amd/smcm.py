@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -109,6 +110,7 @@ class Context:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         self._h = C.c_void_p()
+        self._children = weakref.WeakSet()  # graphs and results; closed before the context
         _check(load().smcm_ctx_create(device, hbm_budget, C.byref(self._h)))
 
     def set_option(self, key: str, value: int) -> None:
@@ -131,7 +133,9 @@ class Context:
 
     def close(self) -> None:
         if self._h:
-            load().smcm_ctx_destroy(self._h)
+            for ch in list(self._children):
+                ch.free() if isinstance(ch, Result) else ch.close()
+            _check(load().smcm_ctx_destroy(self._h))
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -161,13 +165,14 @@ class Graph:
         self.ctx = ctx
         self.hg = hg
         self._h = C.c_void_p()
+        ctx._children.add(self)
         _check(load().smcm_graph_upload(ctx._h, hg.n, _ptr(hg.vertexTypes), _ptr(hg.adj_ptr), _ptr(hg.adj_nbr),
                                         _ptr(hg.adj_et), C.byref(self._h)))
 
     def close(self) -> None:
-        if self._h:
+        if self._h and self.ctx._h:
             load().smcm_graph_free(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -237,6 +242,7 @@ class Result:
     def __init__(self, h, ctx: Context):
         self._h = h
         self.ctx = ctx
+        ctx._children.add(self)
 
     @property
     def nrows(self) -> int:
@@ -274,9 +280,9 @@ class Result:
         return out[: self.nrows]
 
     def free(self) -> None:
-        if self._h:
+        if self._h and self.ctx._h:
             load().smcm_result_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
