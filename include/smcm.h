@@ -65,13 +65,20 @@ int smcm_ctx_create(int device, uint64_t hbm_budget, smcm_ctx** out);
 int smcm_ctx_destroy(smcm_ctx* ctx);
 /* Tuning / mode keys: "s_max", "m_max", "dense_div", "heavy_ub", "d_threads", "force_tier",
  * "exact_cost_sum" (1 = sum c_ij in row order on one lane: bit-exact with the reference at one
- * thread), "lds_bytes". */
+ * thread), "kernel_timing" (1 = time every kernel with HIP events on the context's stream),
+ * "lds_bytes". */
 int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);
 int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* value);
 const char* smcm_last_error(void);
 int smcm_last_stats(smcm_ctx* ctx, smcm_stats* out);
 /* Wait for all work queued on the context's stream. */
 int smcm_ctx_sync(smcm_ctx* ctx);
+/* Per-kernel totals collected while "kernel_timing" is on: entry idx (0-based; returns
+ * SMCM_EINVAL past the end) gives the kernel name, summed event time, launch count and
+ * algorithmic bytes of those launches. */
+int smcm_kernel_stats(smcm_ctx* ctx, int32_t idx, char* name, int32_t cap, double* total_ms,
+                      int64_t* count, double* bytes);
+int smcm_kernel_stats_reset(smcm_ctx* ctx);
 
 /* ---- graph (replaces HeteroGraph + the per-chain BoolMatrix construction,
  *      efficient/resmcm.cpp:181-240, :374-396) ------------------------------------------- */
@@ -88,6 +95,15 @@ int smcm_graph_dims(const smcm_graph* g, int32_t* dims, int32_t cap, int32_t* nt
 int smcm_chain_run(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
                    int32_t L, const int32_t* src_rows, int64_t n_src, smcm_result** out,
                    smcm_times* times);
+/* Row-blocked run for results larger than device memory: the root product is materialised in
+ * HBM in row blocks of at most block_bytes of output, each block is reduced to its row count
+ * and (when row_digest != NULL) per-row digests (as smcm_result_row_digest), then its memory
+ * is reused.  *nnz receives the result's nnz, row_digest[nrows] the digests.  The reference's
+ * driver likewise only keeps the nnz of a result (efficient/resmcm.cpp:859-873). */
+int smcm_chain_run_blocked(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
+                           const int32_t* etypes, int32_t L, const int32_t* src_rows,
+                           int64_t n_src, uint64_t block_bytes, int64_t* nnz, uint64_t* row_digest,
+                           smcm_times* times);
 /* Same, executing the given plan (L-1 pairs in get_optimal_chain_order format) instead of the
  * DP's; the estimator still runs (its densities drive row binning). */
 int smcm_chain_run_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
@@ -118,7 +134,8 @@ int smcm_result_copy_rows(const smcm_result* r, int32_t row_begin, int32_t row_e
                           int64_t* rowptr, int32_t* cols, uint64_t* vals);
 /* Global ids of the result rows. */
 int smcm_result_row_ids(const smcm_result* r, int32_t* rows);
-/* Per-row digest of the (local column, value) sequence, computed on the device. */
+/* Per-row digest, computed on the device: mix(width) + sum over the row's entries of
+ * mix(mix(local column) ^ value), mod 2^64, with mix the splitmix64 finaliser. */
 int smcm_result_row_digest(const smcm_result* r, uint64_t* out);
 int smcm_result_free(smcm_result* r);
 

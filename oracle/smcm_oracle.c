@@ -329,15 +329,15 @@ static uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+/* digest = mix(width) + sum over entries of mix(mix(col) ^ val), mod 2^64 (rows are sorted, so
+ * the order-free sum still identifies the row) */
 void orc_row_digest(const orc_csr* C, uint64_t* out) {
 #pragma omp parallel for schedule(static)
   for (int32_t u = 0; u < C->nrows; ++u) {
-    uint64_t h = 0x243F6A8885A308D3ull ^ (uint64_t)(C->ptr[u + 1] - C->ptr[u]);
-    for (int64_t x = C->ptr[u]; x < C->ptr[u + 1]; ++x) {
-      h = mix64(h ^ (uint64_t)(uint32_t)C->col[x]);
-      h = mix64(h ^ (C->val ? C->val[x] : 1u));
-    }
-    out[u] = h;
+    uint64_t h = 0;
+    for (int64_t x = C->ptr[u]; x < C->ptr[u + 1]; ++x)
+      h += mix64(mix64((uint64_t)(uint32_t)C->col[x] ^ 0x9E3779B97F4A7C15ull) ^ (C->val ? C->val[x] : 1u));
+    out[u] = h + mix64((uint64_t)(C->ptr[u + 1] - C->ptr[u]) ^ 0x243F6A8885A308D3ull);
   }
 }
 

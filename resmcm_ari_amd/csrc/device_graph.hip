@@ -139,17 +139,22 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-__global__ void row_digest_kernel(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                                  const uint64_t* __restrict__ val, int32_t nrows, uint64_t* __restrict__ out) {
-  const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= nrows) return;
-  const int64_t p0 = ptr[u], p1 = ptr[u + 1];
-  uint64_t h = 0x243F6A8885A308D3ull ^ (uint64_t)(p1 - p0);
-  for (int64_t x = p0; x < p1; ++x) {
-    h = mix64(h ^ (uint64_t)(uint32_t)col[x]);
-    h = mix64(h ^ (val ? val[x] : 1ull));
+// row digest = mix(width) + sum over entries of mix(mix(col) ^ val), mod 2^64: one wave per row
+__global__ void __launch_bounds__(kThreads) row_digest_kernel(const int64_t* __restrict__ ptr,
+                                                              const int32_t* __restrict__ col,
+                                                              const uint64_t* __restrict__ val, int32_t nrows,
+                                                              uint64_t* __restrict__ out) {
+  const int lane = lane_id();
+  const int64_t wpb = kThreads / kWave;
+  for (int64_t u = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; u < nrows; u += (int64_t)gridDim.x * wpb) {
+    const int64_t p0 = ptr[u], p1 = ptr[u + 1];
+    uint64_t h = 0;
+    for (int64_t x = p0 + lane; x < p1; x += kWave)
+      h += mix64(mix64((uint64_t)(uint32_t)col[x] ^ 0x9E3779B97F4A7C15ull) ^ (val ? val[x] : 1ull));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+    if (lane == 0) out[u] = h + mix64((uint64_t)(p1 - p0) ^ 0x243F6A8885A308D3ull);
   }
-  out[u] = h;
 }
 
 unsigned grid_for(int64_t items, int per_block, int64_t cap = 1 << 20) {
@@ -251,7 +256,8 @@ void csr_gather_rows(const Csr& in, const int32_t* rows_sel, int32_t nsel, Csr& 
 
 void row_digest(const Csr& c, uint64_t* out, hipStream_t st) {
   if (c.nrows <= 0) return;
-  row_digest_kernel<<<grid_for(c.nrows, kThreads), kThreads, 0, st>>>(c.ptr, c.col, c.val, c.nrows, out);
+  row_digest_kernel<<<grid_for(c.nrows, kThreads / kWave, 65536), kThreads, 0, st>>>(c.ptr, c.col, c.val, c.nrows,
+                                                                                     out);
   SMCM_LAUNCH_CHECK();
 }
 

@@ -164,6 +164,10 @@ void run_pass(const EstimatorPass& p, DevAlloc& al) {
   DevBuf<int32_t> heavy(al, n > 0 ? n : 1), nheavy(al, 1);
   fill_i32(nheavy.p, 1, 0, st);
   const unsigned blocks = (unsigned)ceil_div(n > 0 ? n : 1, kThreads);
+  // algorithmic bytes of the pass: row pointers, column ids, W gathered densities per entry,
+  // W+1 densities written per row
+  const double bytes = 16.0 * (n + 1) + 4.0 * (double)M.nnz + 8.0 * W * (double)M.nnz + 8.0 * (W + 1) * n;
+  const int tok = p.tm ? p.tm->start("estimator.pass", st) : -1;
   if (n > 0) {
     est_light_kernel<W><<<blocks, kThreads, 0, st>>>(M.ptr, M.col, n, p.leaf_dim, p.Rn, p.Ri, heavy.p, nheavy.p);
     SMCM_LAUNCH_CHECK();
@@ -173,6 +177,7 @@ void run_pass(const EstimatorPass& p, DevAlloc& al) {
       est_heavy_kernel<W><<<2048, kThreads, 0, st>>>(M.ptr, M.col, p.Rn, p.Ri, heavy.p, nheavy.p);
       SMCM_LAUNCH_CHECK();
     }
+    if (p.tm) p.tm->stop(tok, st, bytes);
     DevBuf<double> dimj(al, W);
     SMCM_HIP(hipMemcpyAsync(dimj.p, p.dimj, sizeof(double) * W, hipMemcpyHostToDevice, st));
     if (p.exact_sum || n == 0) {
@@ -187,6 +192,12 @@ void run_pass(const EstimatorPass& p, DevAlloc& al) {
     }
     // the host copy of dimj above must complete before dimj is released
     SMCM_HIP(hipStreamSynchronize(st));
+  } else {
+    if (p.tm) p.tm->stop(tok, st, bytes);
+  }
+  if (p.tm) {
+    SMCM_HIP(hipStreamSynchronize(st));
+    p.tm->flush();
   }
 }
 

@@ -1,10 +1,12 @@
 // kernels.h - host launchers of the gfx950 kernels (all enqueue on DevAlloc::stream).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "alloc.h"
 #include "common.h"
+#include "ktimer.h"
 
 namespace smcm {
 
@@ -49,6 +51,7 @@ struct EstimatorPass {
   double* Ri = nullptr;          // out: nrows x (W+1); Ri[u][0] = r_ii, Ri[u][1+q] = r_{i,i+1+q}
   double* c_dev = nullptr;       // out: W values c_{i,i+1+q}
   bool exact_sum = false;        // sequential sum in row order (bit-exact with one thread)
+  KernelTimer* tm = nullptr;
 };
 void estimator_pass(const EstimatorPass& p, DevAlloc& al);
 
@@ -73,8 +76,18 @@ struct SpgemmStats {
 };
 
 // C = A * B (uint64 counts, columns sorted).  est_r (nullable, device) gives the estimator's
-// row densities r[u * est_stride]; est_dim = ncols of B as double.  B rows must be sorted.
+// row densities r[u * est_stride]; the predicted width is r * ncols(B).  B rows must be sorted.
 void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr& C,
-            const SpgemmOpts& o, SpgemmStats* st, DevAlloc& al, bool timing = false);
+            const SpgemmOpts& o, SpgemmStats* st, DevAlloc& al, bool timing = false,
+            KernelTimer* tm = nullptr);
+
+// Row-blocked C = A * B for outputs larger than memory: the symbolic pass runs once, then
+// rows are cut into blocks of at most block_bytes of output (a single row is always a block);
+// each block is materialised in HBM and handed to consume(block, row_begin, row_end) before
+// its memory is reused.  Block CSRs use block-relative row pointers.
+using BlockConsumer = std::function<void(const Csr&, int32_t, int32_t)>;
+void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_stride,
+                    int64_t block_bytes, const BlockConsumer& consume, const SpgemmOpts& o,
+                    SpgemmStats* st, DevAlloc& al, KernelTimer* tm = nullptr);
 
 }  // namespace smcm

@@ -84,6 +84,7 @@ struct smcm_ctx {
   bool exact_cost_sum = false;
   smcm_stats last{};
   int64_t live = 0;  // graphs and results created from this context and not yet freed
+  smcm::KernelTimer timer;
 };
 
 struct smcm_graph {
@@ -232,6 +233,7 @@ struct Chain {
       p.Ri = R[i].p;
       p.c_dev = cdev.p + (size_t)i * L + i + 1;
       p.exact_sum = ctx->exact_cost_sum;
+      p.tm = ctx->timer.enabled ? &ctx->timer : nullptr;
       smcm::estimator_pass(p, al);
     }
     SMCM_HIP(hipMemcpyAsync(c.data(), cdev.p, sizeof(double) * c.size(), hipMemcpyDeviceToHost, al.stream));
@@ -317,8 +319,11 @@ struct Chain {
     bool owned;
   };
 
-  // resmcm.cpp:709-760 with device products; returns the product (owned by the caller)
-  Csr execute(const std::vector<int32_t>& plan, smcm_stats* stats) {
+  // resmcm.cpp:709-760 with device products; returns the product (owned by the caller).
+  // With a consumer, the root product is materialised in row blocks of at most block_bytes
+  // and handed to the consumer (the returned Csr is then empty).
+  Csr execute(const std::vector<int32_t>& plan, smcm_stats* stats, const smcm::BlockConsumer* consumer = nullptr,
+              int64_t block_bytes = 0) {
     DevAlloc& al = ctx->al;
     const double t0 = now_s();
     auto base = [&](int32_t x, bool right) -> Opd {
@@ -330,12 +335,19 @@ struct Chain {
       return Opd{mats[x], x, x, false};
     };
     std::vector<Opd> st;
+    smcm::KernelTimer* tm = ctx->timer.enabled ? &ctx->timer : nullptr;
+    const size_t nprod = plan.size() / 2;
+    size_t done_products = 0;
     auto product = [&](Opd a, Opd b) -> Opd {
       Csr C;
       smcm::SpgemmStats ss;
       const int32_t i = a.i, j = b.j;
       const double* est = R[i].p + (j - i);
-      smcm::spgemm(a.m, b.m, est, L - i, C, ctx->opts, &ss, al, true);
+      const bool root = ++done_products == nprod;
+      if (root && consumer)
+        smcm::spgemm_blocked(a.m, b.m, est, L - i, block_bytes, *consumer, ctx->opts, &ss, al, tm);
+      else
+        smcm::spgemm(a.m, b.m, est, L - i, C, ctx->opts, &ss, al, true, tm);
       if (stats) {
         stats->products += ss.products;
         for (int t = 0; t < 4; ++t) stats->rows_tier[t] += ss.rows[t];
@@ -421,6 +433,60 @@ smcm_result* make_result(Chain& ch, Csr C) {
   return r;
 }
 
+// Runs a chain: estimator, DP (or the forced plan) and the products.  Without a consumer the
+// product is returned in *C; with one, the root product is streamed in row blocks.
+void run_core(Chain& ch, const int32_t* forced_pairs, Csr* C, const smcm::BlockConsumer* consumer,
+              int64_t block_bytes, smcm_times* times) {
+  smcm_ctx* ctx = ch.ctx;
+  const int32_t L = ch.L;
+  smcm_stats st{};
+  smcm::KernelTimer* tm = ctx->timer.enabled ? &ctx->timer : nullptr;
+  Csr out;
+  if (L == 1) {
+    // one matrix: merge duplicates and sort by multiplying with the identity
+    Csr I = identity_csr(ctx->al, ch.mats[0].ncols);
+    smcm::SpgemmStats ss;
+    const double t0 = now_s();
+    try {
+      if (consumer)
+        smcm::spgemm_blocked(ch.mats[0], I, nullptr, 0, block_bytes, *consumer, ctx->opts, &ss, ctx->al, tm);
+      else
+        smcm::spgemm(ch.mats[0], I, nullptr, 0, out, ctx->opts, &ss, ctx->al, true, tm);
+    } catch (...) {
+      free_csr(ctx->al, I);
+      throw;
+    }
+    free_csr(ctx->al, I);
+    ch.spgemm_time = now_s() - t0;
+    st.products = ss.products;
+    st.nnz = ss.nnz;
+    st.n_spgemm = 0;
+  } else {
+    const double t0 = now_s();
+    ch.estimate();
+    ch.dp();
+    std::vector<int32_t> plan = ch.pairs;
+    if (forced_pairs) {
+      ch.check_plan(forced_pairs);
+      plan.assign(forced_pairs, forced_pairs + 2 * (L - 1));
+    }
+    out = ch.execute(plan, &st, consumer, block_bytes);
+    ch.total = now_s() - t0;
+    for (size_t q = 0; q < plan.size() && q < 30; ++q) st.pairs[q] = plan[q];
+    st.c_root = ch.c[L - 1];
+    if (!consumer) st.nnz = out.nnz;
+  }
+  if (!consumer) st.nnz = out.nnz;
+  ctx->last = st;
+  if (times) {
+    times->total = ch.total;
+    times->spgemm = ch.spgemm_time;
+    times->dp = ch.dp_time;
+    times->est = ch.es_time;
+  }
+  if (C) *C = out;
+}
+
 int run_chain(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes, int32_t L,
               const int32_t* src_rows, int64_t n_src, const int32_t* forced_pairs, smcm_result** out,
               smcm_times* times) {
@@ -430,43 +496,8 @@ int run_chain(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t
     *out = nullptr;
     Chain ch{ctx, g};
     ch.prepare(vtypes, etypes, L, src_rows, n_src);
-    smcm_stats st{};
     Csr C;
-    if (L == 1) {
-      // one matrix: merge duplicates and sort by multiplying with the identity
-      Csr I = identity_csr(ctx->al, ch.mats[0].ncols);
-      smcm::SpgemmStats ss;
-      try {
-        smcm::spgemm(ch.mats[0], I, nullptr, 0, C, ctx->opts, &ss, ctx->al, true);
-      } catch (...) {
-        free_csr(ctx->al, I);
-        throw;
-      }
-      free_csr(ctx->al, I);
-      st.products = ss.products;
-      st.n_spgemm = 0;
-    } else {
-      const double t0 = now_s();
-      ch.estimate();
-      ch.dp();
-      std::vector<int32_t> plan = ch.pairs;
-      if (forced_pairs) {
-        ch.check_plan(forced_pairs);
-        plan.assign(forced_pairs, forced_pairs + 2 * (L - 1));
-      }
-      C = ch.execute(plan, &st);
-      ch.total = now_s() - t0;
-      for (size_t q = 0; q < plan.size() && q < 30; ++q) st.pairs[q] = plan[q];
-      st.c_root = ch.c[L - 1];
-    }
-    st.nnz = C.nnz;
-    ctx->last = st;
-    if (times) {
-      times->total = ch.total;
-      times->spgemm = ch.spgemm_time;
-      times->dp = ch.dp_time;
-      times->est = ch.es_time;
-    }
+    run_core(ch, forced_pairs, &C, nullptr, 0, times);
     try {
       *out = make_result(ch, C);
     } catch (...) {
@@ -556,6 +587,8 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
       o.force_tier = (int)v;
     } else if (k == "exact_cost_sum") {
       ctx->exact_cost_sum = v != 0;
+    } else if (k == "kernel_timing") {
+      ctx->timer.enabled = v != 0;
     } else if (k == "lds_bytes") {
       if (v < 16384) fail(SMCM_EINVAL, "lds_bytes too small");
       int dev_max = 65536;
@@ -580,6 +613,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "d_threads") *v = o.d_threads;
     else if (k == "force_tier") *v = o.force_tier;
     else if (k == "exact_cost_sum") *v = ctx->exact_cost_sum;
+    else if (k == "kernel_timing") *v = ctx->timer.enabled;
     else if (k == "lds_bytes") *v = o.lds_bytes;
     else if (k == "cus") *v = o.cus;
     else if (k == "budget") *v = (int64_t)ctx->budget;
@@ -680,6 +714,55 @@ int smcm_chain_run_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, con
                         smcm_times* times) {
   if (!pairs && L > 1) return set_err(SMCM_EINVAL, "null plan");
   return run_chain(ctx, g, vtypes, etypes, L, src_rows, n_src, pairs, out, times);
+}
+
+int smcm_chain_run_blocked(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
+                           int32_t L, const int32_t* src_rows, int64_t n_src, uint64_t block_bytes, int64_t* nnz,
+                           uint64_t* row_digest, smcm_times* times) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (block_bytes < 12) fail(SMCM_EINVAL, "block_bytes must hold at least one entry");
+    Chain ch{ctx, g};
+    ch.prepare(vtypes, etypes, L, src_rows, n_src);
+    DevAlloc& al = ctx->al;
+    const int32_t nrows = ch.mats[0].nrows;
+    DevBuf<uint64_t> dig(al, row_digest ? nrows : 1);
+    int64_t total = 0;
+    smcm::BlockConsumer consume = [&](const Csr& blk, int32_t b, int32_t e) {
+      total += blk.nnz;
+      if (row_digest) smcm::row_digest(blk, dig.p + b, al.stream);
+    };
+    run_core(ch, nullptr, nullptr, &consume, (int64_t)block_bytes, times);
+    ctx->last.nnz = total;
+    if (nnz) *nnz = total;
+    if (row_digest && nrows)
+      SMCM_HIP(hipMemcpyAsync(row_digest, dig.p, sizeof(uint64_t) * nrows, hipMemcpyDeviceToHost, al.stream));
+    SMCM_HIP(hipStreamSynchronize(al.stream));
+  });
+}
+
+int smcm_kernel_stats(smcm_ctx* ctx, int32_t idx, char* name, int32_t cap, double* total_ms, int64_t* count,
+                      double* bytes) {
+  return guarded([&] {
+    if (!ctx) fail(SMCM_EINVAL, "null context");
+    if (idx < 0 || idx >= (int32_t)ctx->timer.agg.size()) fail(SMCM_EINVAL, "index out of range");
+    auto it = ctx->timer.agg.begin();
+    std::advance(it, idx);
+    if (name && cap > 0) {
+      std::strncpy(name, it->first.c_str(), (size_t)cap - 1);
+      name[cap - 1] = 0;
+    }
+    if (total_ms) *total_ms = it->second.ms;
+    if (count) *count = it->second.count;
+    if (bytes) *bytes = it->second.bytes;
+  });
+}
+
+int smcm_kernel_stats_reset(smcm_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) fail(SMCM_EINVAL, "null context");
+    ctx->timer.reset();
+  });
 }
 
 int smcm_chain_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes, int32_t L,

@@ -31,6 +31,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
 
 namespace smcm {
 
@@ -51,16 +54,19 @@ __global__ void entry_len_kernel(const int32_t* __restrict__ acol, int64_t nnz, 
 
 enum Tier { kEmpty = 0, kWaveTier = 1, kSortTier = 2, kBitmapTier = 3, kHeavyTier = 4, kNumBins = 5 };
 
-__global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __restrict__ aptr, int32_t m,
-                                                            const int64_t* __restrict__ S,
+__global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __restrict__ aptr, int32_t r0, int32_t r1,
+                                                            int32_t m, const int64_t* __restrict__ S,
                                                             const double* __restrict__ est_r, int est_stride,
                                                             double ncols, int s_max, int m_max, int dense_div,
                                                             int64_t heavy_ub, int force_tier,
-                                                            int32_t* __restrict__ bins, int32_t* __restrict__ counts) {
-  const int32_t u = blockIdx.x * kThreads + threadIdx.x;
+                                                            int32_t* __restrict__ bins, int32_t* __restrict__ counts,
+                                                            unsigned long long* __restrict__ sums) {
+  const int32_t u = r0 + (int32_t)(blockIdx.x * kThreads + threadIdx.x);
   int tier = -1;
-  if (u < m) {
-    const int64_t ub = S[aptr[u + 1]] - S[aptr[u]];
+  int64_t ub = 0, d = 0;
+  if (u < r1) {
+    ub = S[aptr[u + 1]] - S[aptr[u]];
+    d = aptr[u + 1] - aptr[u];
     if (ub == 0) {
       tier = kEmpty;
     } else if (force_tier >= 0) {
@@ -76,17 +82,44 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
     }
     if (tier == kBitmapTier && ub > heavy_ub) tier = kHeavyTier;
   }
-  // wave-aggregated append per bin (empty rows are only counted)
+  // wave-aggregated append per bin (empty rows are only counted); per-tier sums of A entries
+  // and products feed the algorithmic-byte accounting
   const int lane = lane_id();
   for (int t = 0; t < kNumBins; ++t) {
     const uint64_t mask = __ballot(tier == t);
     if (!mask) continue;
     const int leader = __ffsll((long long)mask) - 1;
     int32_t base = 0;
-    if (lane == leader) base = atomicAdd(&counts[t], (int32_t)__popcll(mask));
+    const int64_t sd = wave_sum_i64(tier == t ? d : 0);
+    const int64_t su = wave_sum_i64(tier == t ? ub : 0);
+    if (lane == leader) {
+      base = atomicAdd(&counts[t], (int32_t)__popcll(mask));
+      atomicAdd(&sums[2 * t], (unsigned long long)sd);
+      atomicAdd(&sums[2 * t + 1], (unsigned long long)su);
+    }
     base = __shfl(base, leader);
     if (tier == t && t != kEmpty) bins[(size_t)t * m + base + lanes_below(mask)] = u;
   }
+}
+
+// sum of symbolic row widths per tier
+__global__ void __launch_bounds__(kThreads) tier_nnz_kernel(const int32_t* __restrict__ bins, int32_t m,
+                                                            const int32_t* __restrict__ counts,
+                                                            const int64_t* __restrict__ row_nnz,
+                                                            unsigned long long* __restrict__ out) {
+  const int t = blockIdx.y;
+  const int32_t n = counts[t];
+  int64_t s = 0;
+  for (int32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    s += row_nnz[bins[(size_t)t * m + i]];
+  s = wave_sum_i64(s);
+  if (lane_id() == 0 && s) atomicAdd(&out[t], (unsigned long long)s);
+}
+
+// rowptr of a block relative to its first row
+__global__ void rebase_ptr_kernel(const int64_t* __restrict__ ptr, int32_t n, int64_t* __restrict__ out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) out[i] = ptr[i] - ptr[0];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -450,58 +483,81 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
   }
 }
 
-struct Launch {
-  hipStream_t st;
+// per-tier accounting of one pass: rows, A entries, products, output entries
+struct TierCounts {
+  int32_t rows[kNumBins] = {0, 0, 0, 0, 0};
+  int64_t a_nnz[kNumBins] = {0, 0, 0, 0, 0};
+  int64_t products[kNumBins] = {0, 0, 0, 0, 0};
+  int64_t c_nnz[kNumBins] = {0, 0, 0, 0, 0};
 };
 
+// algorithmic bytes of a pass over one tier: A and C row pointers, A columns (+values) with
+// the B row-pointer pair per A entry, every product's B column (+value), every output entry
+// (column + value); the symbolic pass writes one int64 width per row instead of C
+double tier_bytes(bool numeric, const TierCounts& tc, int t, bool av, bool bv) {
+  const double r = tc.rows[t], d = (double)tc.a_nnz[t], p = (double)tc.products[t], w = (double)tc.c_nnz[t];
+  if (!numeric) return 16.0 * (r + 1) + 20.0 * d + 4.0 * p;
+  return 16.0 * (r + 1) + (20.0 + (av ? 8 : 0)) * d + (4.0 + (bv ? 8 : 0)) * p + 12.0 * w;
+}
+
 template <bool NUMERIC, bool AV, bool BV>
-void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const int32_t* counts_h,
-                  int64_t* row_nnz, Csr* C, const SpgemmOpts& o, const BitmapLayout& lay, int32_t* queue,
-                  hipStream_t st) {
+void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const TierCounts& tc,
+                  int64_t* row_nnz, const int64_t* cptr, int32_t* ccol, uint64_t* cval, const SpgemmOpts& o,
+                  const BitmapLayout& lay, int32_t* queue, hipStream_t st, KernelTimer* tm) {
   const int32_t m = A.nrows;
-  const int32_t nW = counts_h[kWaveTier], nS = counts_h[kSortTier], nB = counts_h[kBitmapTier],
-                nH = counts_h[kHeavyTier];
-  const int64_t* cptr = C ? C->ptr : nullptr;
-  int32_t* ccol = C ? C->col : nullptr;
-  uint64_t* cval = C ? C->val : nullptr;
+  const int32_t nW = tc.rows[kWaveTier], nS = tc.rows[kSortTier], nB = tc.rows[kBitmapTier],
+                nH = tc.rows[kHeavyTier];
+  KernelTimer dummy;
+  KernelTimer& T = tm ? *tm : dummy;
   if (nW > 0) {
     const unsigned g = (unsigned)ceil_div(nW, kThreads / kWave);
+    const int tok = T.start(NUMERIC ? "spgemm.numeric.wave" : "spgemm.symbolic.wave", st);
     wave_tier_kernel<NUMERIC, AV, BV><<<g, kThreads, 0, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, S,
                                                              bins + (size_t)kWaveTier * m, nW, row_nnz, cptr, ccol,
                                                              cval);
     SMCM_LAUNCH_CHECK();
+    T.stop(tok, st, tier_bytes(NUMERIC, tc, kWaveTier, AV, BV));
   }
   if (nS > 0) {
     int cap = 64;
     while (cap < o.m_max) cap <<= 1;
     const size_t lds = (size_t)cap * (NUMERIC ? 12 : 4);
     const unsigned g = (unsigned)std::min<int64_t>(nS, (int64_t)o.cus * 16);
+    const int tok = T.start(NUMERIC ? "spgemm.numeric.sort" : "spgemm.symbolic.sort", st);
     sort_tier_kernel<NUMERIC, AV, BV><<<g, kThreads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, S,
                                                                bins + (size_t)kSortTier * m, nS, cap, row_nnz, cptr,
                                                                ccol, cval);
     SMCM_LAUNCH_CHECK();
+    T.stop(tok, st, tier_bytes(NUMERIC, tc, kSortTier, AV, BV));
   }
   if (nB + nH > 0) {
     fill_i32(queue, 1, 0, st);
     const size_t lds = NUMERIC ? lay.bytes_num() : lay.bytes_sym();
     const int per_cu = std::max<int64_t>(1, std::min<int64_t>(4, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
     const unsigned g = (unsigned)std::min<int64_t>(nB + nH, (int64_t)o.cus * per_cu);
+    const int tok = T.start(NUMERIC ? "spgemm.numeric.bitmap" : "spgemm.symbolic.bitmap", st);
     bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(
         A.ptr, A.col, A.val, B.ptr, B.col, B.val, bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m,
         nB, B.ncols, lay.nw, lay.vcap, queue, row_nnz, cptr, ccol, cval);
     SMCM_LAUNCH_CHECK();
+    T.stop(tok, st, tier_bytes(NUMERIC, tc, kBitmapTier, AV, BV) + tier_bytes(NUMERIC, tc, kHeavyTier, AV, BV) -
+                        (NUMERIC ? 16.0 : 16.0));
   }
 }
 
 template <bool NUMERIC>
-void dispatch(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const int32_t* counts_h,
-              int64_t* row_nnz, Csr* C, const SpgemmOpts& o, const BitmapLayout& lay, int32_t* queue,
-              hipStream_t st) {
+void dispatch(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const TierCounts& tc,
+              int64_t* row_nnz, const int64_t* cptr, int32_t* ccol, uint64_t* cval, const SpgemmOpts& o,
+              const BitmapLayout& lay, int32_t* queue, hipStream_t st, KernelTimer* tm) {
   const bool av = A.val != nullptr, bv = B.val != nullptr;
-  if (av && bv) launch_tiers<NUMERIC, true, true>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
-  else if (av) launch_tiers<NUMERIC, true, false>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
-  else if (bv) launch_tiers<NUMERIC, false, true>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
-  else launch_tiers<NUMERIC, false, false>(A, B, S, bins, counts_h, row_nnz, C, o, lay, queue, st);
+  if (av && bv)
+    launch_tiers<NUMERIC, true, true>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
+  else if (av)
+    launch_tiers<NUMERIC, true, false>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
+  else if (bv)
+    launch_tiers<NUMERIC, false, true>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
+  else
+    launch_tiers<NUMERIC, false, false>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
 }
 
 template <bool NUMERIC, bool AV, bool BV>
@@ -544,10 +600,100 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Analysis and binning of one product, reusable for row blocks of the same product.
+struct SpgemmCore {
+  const Csr& A;
+  const Csr& B;
+  const double* est_r;
+  int est_stride;
+  const SpgemmOpts& o;
+  DevAlloc& al;
+  KernelTimer* tm;
+  DevBuf<int64_t> S;
+  DevBuf<int32_t> bins, counts, queue;
+  DevBuf<unsigned long long> sums;
+  TierCounts tc;
+  BitmapLayout lay;
+  int64_t products = 0;
+
+  SpgemmCore(const Csr& a, const Csr& b, const double* er, int es, const SpgemmOpts& opts, DevAlloc& alloc,
+             KernelTimer* timer)
+      : A(a), B(b), est_r(er), est_stride(es), o(opts), al(alloc), tm(timer) {
+    hipStream_t st = al.stream;
+    DevBuf<int64_t> len(al, A.nnz);
+    S = DevBuf<int64_t>(al, A.nnz + 1);
+    if (A.nnz > 0) {
+      const int tok = tm ? tm->start("spgemm.analysis", st) : -1;
+      entry_len_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nnz, kThreads), 65536), kThreads, 0, st>>>(
+          A.col, A.nnz, B.ptr, len.p);
+      SMCM_LAUNCH_CHECK();
+      if (tm) tm->stop(tok, st, 16.0 * (double)A.nnz + 4.0 * (double)A.nnz);
+    }
+    scan_exclusive_i64(len.p, S.p, A.nnz, al);
+    bins = DevBuf<int32_t>(al, (int64_t)kNumBins * (A.nrows > 0 ? A.nrows : 1));
+    counts = DevBuf<int32_t>(al, kNumBins + 1);
+    queue = DevBuf<int32_t>(al, 1);
+    sums = DevBuf<unsigned long long>(al, 3 * kNumBins);
+    lay = choose_layout(B.ncols, o);
+    set_all_lds_attrs(std::max(lay.bytes_num(), lay.bytes_sym()));
+    SMCM_HIP(hipMemcpyAsync(&products, S.p + A.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  }
+
+  // bin rows [r0, r1) (bins hold absolute row ids); results in tc (synchronises)
+  void bin(int32_t r0, int32_t r1) {
+    hipStream_t st = al.stream;
+    fill_i32(counts.p, kNumBins + 1, 0, st);
+    fill_i64((int64_t*)sums.p, 3 * kNumBins, 0, st);
+    if (r1 > r0) {
+      bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, kThreads), kThreads, 0, st>>>(
+          A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
+          o.force_tier, bins.p, counts.p, sums.p);
+      SMCM_LAUNCH_CHECK();
+    }
+    unsigned long long sh[2 * kNumBins];
+    SMCM_HIP(hipMemcpyAsync(tc.rows, counts.p, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipMemcpyAsync(sh, sums.p, sizeof(sh), hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipStreamSynchronize(st));
+    for (int t = 0; t < kNumBins; ++t) {
+      tc.a_nnz[t] = (int64_t)sh[2 * t];
+      tc.products[t] = (int64_t)sh[2 * t + 1];
+      tc.c_nnz[t] = 0;
+    }
+  }
+
+  // widths of the binned rows into row_nnz (other entries untouched); fills tc.c_nnz
+  void symbolic(int64_t* row_nnz) {
+    hipStream_t st = al.stream;
+    dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, queue.p, st, tm);
+    unsigned long long* tn = sums.p + 2 * kNumBins;
+    fill_i64((int64_t*)tn, kNumBins, 0, st);
+    tier_nnz_kernel<<<dim3(64, kNumBins), kThreads, 0, st>>>(bins.p, A.nrows, counts.p, row_nnz, tn);
+    SMCM_LAUNCH_CHECK();
+    unsigned long long h[kNumBins];
+    SMCM_HIP(hipMemcpyAsync(h, tn, sizeof(h), hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipStreamSynchronize(st));
+    for (int t = 0; t < kNumBins; ++t) tc.c_nnz[t] = (int64_t)h[t];
+  }
+
+  // numeric pass of the binned rows; cptr holds absolute offsets, ccol/cval are shifted so
+  // that ccol[cptr[u]] is the first output of row u
+  void numeric(const int64_t* cptr, int32_t* ccol, uint64_t* cval) {
+    dispatch<true>(A, B, S.p, bins.p, tc, nullptr, cptr, ccol, cval, o, lay, queue.p, al.stream, tm);
+  }
+
+  void add_stats(SpgemmStats* st) const {
+    if (!st) return;
+    st->rows[0] += tc.rows[kEmpty];
+    st->rows[1] += tc.rows[kWaveTier];
+    st->rows[2] += tc.rows[kSortTier];
+    st->rows[3] += tc.rows[kBitmapTier] + tc.rows[kHeavyTier];
+  }
+};
+
 }  // namespace
 
 void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr& C, const SpgemmOpts& o,
-            SpgemmStats* stats, DevAlloc& al, bool timing) {
+            SpgemmStats* stats, DevAlloc& al, bool timing, KernelTimer* tm) {
   if (A.ncols != B.nrows) throw HipError(-22, "spgemm: inner dimensions differ");
   if (!B.sorted) throw HipError(-22, "spgemm: right operand rows must be sorted");
   hipStream_t st = al.stream;
@@ -557,63 +703,110 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
   C.ncols = B.ncols;
   C.sorted = true;
   C.ptr = al.alloc_n<int64_t>((int64_t)m + 1);
-  // 1. analysis
-  DevBuf<int64_t> len(al, A.nnz), S(al, A.nnz + 1);
-  if (A.nnz > 0) {
-    entry_len_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nnz, kThreads), 65536), kThreads, 0, st>>>(
-        A.col, A.nnz, B.ptr, len.p);
-    SMCM_LAUNCH_CHECK();
-  }
-  scan_exclusive_i64(len.p, S.p, A.nnz, al);
-  len.reset();
-  // 2. binning
-  DevBuf<int32_t> bins(al, (int64_t)kNumBins * (m > 0 ? m : 1)), counts(al, kNumBins + 1);
-  fill_i32(counts.p, kNumBins + 1, 0, st);
-  if (m > 0) {
-    bin_rows_kernel<<<(unsigned)ceil_div(m, kThreads), kThreads, 0, st>>>(
-        A.ptr, m, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub, o.force_tier,
-        bins.p, counts.p);
-    SMCM_LAUNCH_CHECK();
-  }
-  int32_t counts_h[kNumBins + 1];
-  int64_t products = 0;
-  SMCM_HIP(hipMemcpyAsync(counts_h, counts.p, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, st));
-  SMCM_HIP(hipMemcpyAsync(&products, S.p + A.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  SMCM_HIP(hipStreamSynchronize(st));
+  SpgemmCore core(A, B, est_r, est_stride, o, al, tm);
+  core.bin(0, m);
   if (stats) {
-    stats->products = products;
-    stats->rows[0] = counts_h[kEmpty];
-    stats->rows[1] = counts_h[kWaveTier];
-    stats->rows[2] = counts_h[kSortTier];
-    stats->rows[3] = counts_h[kBitmapTier] + counts_h[kHeavyTier];
+    *stats = SpgemmStats{};
+    stats->products = core.products;
+    core.add_stats(stats);
+    stats->windows = ceil_div(std::max<int64_t>(B.ncols, 1), core.lay.nw);
     stats->ms_analysis = ms_since(t0);
   }
-  const BitmapLayout lay = choose_layout(B.ncols, o);
-  if (stats) stats->windows = ceil_div(std::max<int64_t>(B.ncols, 1), lay.nw);
-  set_all_lds_attrs(std::max(lay.bytes_num(), lay.bytes_sym()));
-  // 3. symbolic
   auto t1 = std::chrono::steady_clock::now();
-  DevBuf<int64_t> row_nnz(al, m > 0 ? m : 1);
-  DevBuf<int32_t> queue(al, 1);
-  fill_i64(row_nnz.p, m, 0, st);
-  dispatch<false>(A, B, S.p, bins.p, counts_h, row_nnz.p, nullptr, o, lay, queue.p, st);
-  scan_exclusive_i64(row_nnz.p, C.ptr, m, al);
+  {
+    DevBuf<int64_t> row_nnz(al, m > 0 ? m : 1);
+    fill_i64(row_nnz.p, m, 0, st);
+    core.symbolic(row_nnz.p);
+    scan_exclusive_i64(row_nnz.p, C.ptr, m, al);
+  }
   int64_t nnz = 0;
   SMCM_HIP(hipMemcpyAsync(&nnz, C.ptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SMCM_HIP(hipStreamSynchronize(st));
   if (stats) stats->ms_symbolic = ms_since(t1);
-  row_nnz.reset();
-  // 4. numeric
   auto t2 = std::chrono::steady_clock::now();
   C.nnz = nnz;
   C.col = al.alloc_n<int32_t>(nnz);
   C.val = al.alloc_n<uint64_t>(nnz);
-  dispatch<true>(A, B, S.p, bins.p, counts_h, nullptr, &C, o, lay, queue.p, st);
-  if (timing || stats) SMCM_HIP(hipStreamSynchronize(st));
+  core.numeric(C.ptr, C.col, C.val);
+  if (timing || stats || tm) SMCM_HIP(hipStreamSynchronize(st));
+  if (tm) tm->flush();
   if (stats) {
     stats->ms_numeric = ms_since(t2);
     stats->nnz = nnz;
   }
+}
+
+void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_stride, int64_t block_bytes,
+                    const BlockConsumer& consume, const SpgemmOpts& o, SpgemmStats* stats, DevAlloc& al,
+                    KernelTimer* tm) {
+  if (A.ncols != B.nrows) throw HipError(-22, "spgemm: inner dimensions differ");
+  if (!B.sorted) throw HipError(-22, "spgemm: right operand rows must be sorted");
+  hipStream_t st = al.stream;
+  const int32_t m = A.nrows;
+  SpgemmCore core(A, B, est_r, est_stride, o, al, tm);
+  core.bin(0, m);
+  if (stats) {
+    *stats = SpgemmStats{};
+    stats->products = core.products;
+    core.add_stats(stats);
+  }
+  DevBuf<int64_t> ptr(al, (int64_t)m + 1);
+  {
+    DevBuf<int64_t> row_nnz(al, m > 0 ? m : 1);
+    fill_i64(row_nnz.p, m, 0, st);
+    core.symbolic(row_nnz.p);
+    scan_exclusive_i64(row_nnz.p, ptr.p, m, al);
+  }
+  std::vector<int64_t> hptr((size_t)m + 1);
+  SMCM_HIP(hipMemcpyAsync(hptr.data(), ptr.p, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost, st));
+  SMCM_HIP(hipStreamSynchronize(st));
+  if (tm) tm->flush();
+  const int64_t max_entries = std::max<int64_t>(1, block_bytes / 12);
+  // cut rows into blocks: widest [b, e) whose output fits; a single row always forms a block
+  std::vector<std::pair<int32_t, int32_t>> blocks;
+  int64_t widest = 1;
+  for (int32_t b = 0; b < m;) {
+    int32_t lo = b + 1, hi = m;
+    while (lo < hi) {
+      const int32_t mid = lo + (hi - lo + 1) / 2;
+      if (hptr[mid] - hptr[b] <= max_entries) lo = mid;
+      else hi = mid - 1;
+    }
+    const int32_t e = std::max(lo, b + 1);
+    blocks.push_back({b, e});
+    widest = std::max<int64_t>(widest, hptr[e] - hptr[b]);
+    b = e;
+  }
+  const bool trace = std::getenv("SMCM_TRACE") != nullptr;
+  // block buffers are allocated once and reused by every block
+  DevBuf<int32_t> bcol(al, widest);
+  DevBuf<uint64_t> bval(al, widest);
+  DevBuf<int64_t> bptr(al, (int64_t)m + 1);
+  for (auto [b, e] : blocks) {
+    auto tb = std::chrono::steady_clock::now();
+    const int64_t p0 = hptr[b], cnt = hptr[e] - hptr[b];
+    Csr blk;
+    blk.nrows = e - b;
+    blk.ncols = B.ncols;
+    blk.nnz = cnt;
+    blk.sorted = true;
+    core.bin(b, e);
+    auto t1 = std::chrono::steady_clock::now();
+    core.numeric(ptr.p, bcol.p - p0, bval.p - p0);
+    rebase_ptr_kernel<<<(unsigned)ceil_div((int64_t)(e - b) + 1, kThreads), kThreads, 0, st>>>(ptr.p + b, e - b,
+                                                                                                bptr.p);
+    SMCM_LAUNCH_CHECK();
+    blk.ptr = bptr.p;
+    blk.col = bcol.p;
+    blk.val = bval.p;
+    consume(blk, b, e);
+    SMCM_HIP(hipStreamSynchronize(st));
+    if (trace)
+      fprintf(stderr, "[smcm] block rows [%d, %d) nnz %lld bin %.3f ms numeric+consume %.3f ms\n", b, e,
+              (long long)cnt, std::chrono::duration<double, std::milli>(t1 - tb).count(), ms_since(t1));
+    if (tm) tm->flush();
+  }
+  if (stats) stats->nnz = hptr[m];
 }
 
 }  // namespace smcm

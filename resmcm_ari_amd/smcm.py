@@ -30,7 +30,8 @@ EXPORTS = [
     "smcm_last_stats", "smcm_ctx_sync", "smcm_graph_upload", "smcm_graph_free", "smcm_graph_dims",
     "smcm_chain_run", "smcm_chain_run_plan", "smcm_chain_plan", "smcm_estimate", "smcm_estimator_rows",
     "smcm_result_nrows", "smcm_result_nnz", "smcm_result_copy", "smcm_result_copy_rows", "smcm_result_row_ids",
-    "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr",
+    "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
+    "smcm_kernel_stats", "smcm_kernel_stats_reset",
 ]
 
 
@@ -83,6 +84,9 @@ def load() -> C.CDLL:
             "smcm_result_row_digest": [vp, vp],
             "smcm_result_free": [vp],
             "smcm_spgemm_csr": [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp],
+            "smcm_chain_run_blocked": [vp, vp, vp, vp, i32, vp, i64, u64, vp, vp, vp],
+            "smcm_kernel_stats": [vp, i32, C.c_char_p, i32, vp, vp, vp],
+            "smcm_kernel_stats_reset": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -130,6 +134,20 @@ class Context:
 
     def sync(self) -> None:
         _check(load().smcm_ctx_sync(self._h))
+
+    def kernel_stats(self) -> dict:
+        """{kernel name: (total ms, launches, algorithmic bytes)} since the last reset."""
+        out = {}
+        i = 0
+        name = C.create_string_buffer(128)
+        ms, cnt, by = C.c_double(), C.c_int64(), C.c_double()
+        while load().smcm_kernel_stats(self._h, i, name, 128, C.byref(ms), C.byref(cnt), C.byref(by)) == SMCM_OK:
+            out[name.value.decode()] = (ms.value, int(cnt.value), by.value)
+            i += 1
+        return out
+
+    def kernel_stats_reset(self) -> None:
+        _check(load().smcm_kernel_stats_reset(self._h))
 
     def close(self) -> None:
         if self._h:
@@ -207,6 +225,19 @@ class Graph:
                                               _ptr(pp), C.byref(h), C.byref(t)))
         times = {"total": t.total, "spgemm": t.spgemm, "dp": t.dp, "est": t.est}
         return Result(h, self.ctx), times
+
+    def run_blocked(self, mp: MetaPath, block_bytes: int, src_rows=None, digest: bool = False):
+        """Row-blocked run: returns (nnz, per-row digests or None, times)."""
+        vt, et = self._path(mp)
+        s, ns = self._src(src_rows)
+        nrows = ns if s is not None else int(self.hg.dimensions[mp.vertexTypes[0]])
+        dig = np.zeros(max(nrows, 1), np.uint64) if digest else None
+        nnz = C.c_int64()
+        t = Times()
+        _check(load().smcm_chain_run_blocked(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen, _ptr(s), ns,
+                                             int(block_bytes), C.byref(nnz), _ptr(dig), C.byref(t)))
+        times = {"total": t.total, "spgemm": t.spgemm, "dp": t.dp, "est": t.est}
+        return int(nnz.value), (dig[:nrows] if digest else None), times
 
     def plan(self, mp: MetaPath, src_rows=None):
         L = mp.pathLen

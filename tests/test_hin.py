@@ -1,0 +1,97 @@
+"""Host logic: meta-path parsing (reference rules), text IO round trips, loader semantics,
+generator determinism and the workload definitions."""
+import os
+
+import numpy as np
+import pytest
+
+from resmcm_ari_amd import configs
+from resmcm_ari_amd.hin import HeteroGraph, MetaPath, read_metapathesN
+
+
+def test_metapath_parse_and_string():
+    mp = MetaPath("1-3-0-1-2-4-0-0-1")
+    assert mp.pathLen == 4
+    assert mp.vertexTypes == [1, 0, 2, 0, 1]
+    assert mp.edgeTypes == [3, 1, 4, 0]
+    assert mp.toString() == "1-3-0-1-2-4-0-0-1"
+    # any non-digit separates numbers (resmcm.cpp:273-285)
+    assert MetaPath("1 3 0").vertexTypes == [1, 0]
+    assert MetaPath("12,7;3").edgeTypes == [7]
+
+
+@pytest.mark.parametrize("bad", ["1-3", "1-3-0-1", "", "1-3-"])
+def test_metapath_malformed(bad):
+    with pytest.raises(ValueError):
+        MetaPath(bad)
+
+
+def test_metapath_zero_length():
+    # "-" is one vertex type and no edge type for the reference parser: a zero-length path
+    mp = MetaPath("-")
+    assert mp.pathLen == 0 and mp.vertexTypes == [0]
+
+
+def test_read_metapathes_stops_at_malformed(tmp_path):
+    f = tmp_path / "mp.txt"
+    f.write_text("1-3-0\n0-0-1-3-0 extra\n1-3\n0-1-2\n")
+    ps = read_metapathesN(str(f))
+    assert [p.toString() for p in ps] == ["1-3-0", "0-0-1-3-0"]
+
+
+def test_text_roundtrip(tmp_path):
+    hg = configs.tiny_loc(seed=4).build()
+    files = [str(tmp_path / x) for x in ("g.txt", "v.txt", "e.txt")]
+    hg.write_text(*files)
+    h2 = HeteroGraph(*files)
+    assert h2.n == hg.n and h2.m == hg.m
+    for a in ("vertexTypes", "adj_ptr", "adj_nbr", "adj_et", "dimensions"):
+        np.testing.assert_array_equal(getattr(h2, a), getattr(hg, a))
+
+
+def test_loader_semantics(tmp_path):
+    """n = lines of vertex.txt; entries resolve their type through edge.txt; a vertex without a
+    graph.txt line has no neighbours; file order is kept."""
+    (tmp_path / "v.txt").write_text("0 0\n1 1\n2 1\n3 0\n")
+    (tmp_path / "e.txt").write_text("0 5\n1 5\n2 7\n")
+    (tmp_path / "g.txt").write_text("0 2 1 1 0\n1 0 2\n")
+    g = HeteroGraph(str(tmp_path / "g.txt"), str(tmp_path / "v.txt"), str(tmp_path / "e.txt"))
+    assert g.n == 4 and g.m == 3
+    np.testing.assert_array_equal(g.adj_ptr, [0, 2, 3, 3, 3])
+    np.testing.assert_array_equal(g.adj_nbr, [2, 1, 0])
+    np.testing.assert_array_equal(g.adj_et, [5, 5, 7])
+    np.testing.assert_array_equal(g.dimensions, [2, 2])
+    assert [b.tolist() for b in g.bins] == [[0, 3], [1, 2]]
+
+
+def test_generators_deterministic():
+    a = configs.tiny_bib(seed=3).build()
+    b = configs.tiny_bib(seed=3).build()
+    c = configs.tiny_bib(seed=4).build()
+    np.testing.assert_array_equal(a.adj_nbr, b.adj_nbr)
+    assert not (len(a.adj_nbr) == len(c.adj_nbr) and np.array_equal(a.adj_nbr, c.adj_nbr))
+
+
+def test_config1_shape():
+    hg = configs.CFG1.build()
+    np.testing.assert_array_equal(hg.dimensions, [30_000, 20_000, 200, 20_000])
+    # every paper: 1+ distinct authors, exactly one venue, 8 distinct terms; ascending lists
+    vt = hg.vertexTypes
+    for p in range(0, 30_000, 997):
+        nb = hg.adj_nbr[hg.adj_ptr[p]:hg.adj_ptr[p + 1]]
+        types = vt[nb]
+        assert (types == configs.V).sum() == 1
+        assert (types == configs.T).sum() == 8
+        assert (types == configs.A).sum() >= 1
+        assert np.all(np.diff(nb) > 0)
+
+
+def test_random_walk_paths_are_schema_valid():
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        L = int(rng.integers(2, 11))
+        mp = configs.random_walk_path(rng, L, "bib")
+        assert mp.pathLen == L
+        for i in range(L):
+            assert (mp.vertexTypes[i], mp.vertexTypes[i + 1]) in configs.BIB_EDGE
+            assert configs.BIB_EDGE[(mp.vertexTypes[i], mp.vertexTypes[i + 1])] == mp.edgeTypes[i]
