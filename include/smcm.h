@@ -57,6 +57,9 @@ typedef struct smcm_stats {
   int32_t n_spgemm;      /* SpGEMMs executed */
   int32_t pairs[2 * 15]; /* executed plan (get_optimal_chain_order format) */
   double c_root;         /* estimated nnz of the whole chain (c_[0][L-1]) */
+  int64_t ref_nnz;       /* nnz as the reference counts it: equals nnz for L >= 2; for L == 1 the
+                            reference returns matrix 0 itself, duplicate entries included
+                            (efficient/resmcm.cpp:670-677) */
 } smcm_stats;
 
 /* ---- context ---------------------------------------------------------------------------- */

@@ -95,7 +95,9 @@ int main(int argc, char** argv) {
       spg += t.spgemm / iterations;
       dp += t.dp / iterations;
       es += t.est / iterations;
-      nnz = smcm_result_nnz(r);
+      smcm_stats st;
+      smcm_last_stats(ctx, &st);
+      nnz = st.ref_nnz;  // nnz as the reference counts it (:859-868)
       smcm_result_free(r);
     }
     char mp[1 << 14];

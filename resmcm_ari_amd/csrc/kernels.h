@@ -62,6 +62,10 @@ struct SpgemmOpts {
   int dense_div = 64;      // estimated width >= ncols/dense_div sends the row to the bitmap tier
   int64_t heavy_ub = 1 << 16;  // bitmap-tier rows with ub above this are scheduled first
   int d_threads = 512;     // bitmap-tier workgroup size
+  int w_threads = 1024;    // window-tier workgroup size
+  int64_t window_cols = 0; // window width cap (0 = largest that fits in LDS)
+  int64_t bitmap_vcap = -1; // cap of the bitmap tier's LDS value slots (-1 = as many as fit);
+                            // rows wider than this use the window kernel
   int64_t lds_bytes = 65536;   // LDS available per workgroup
   int cus = 256;           // compute units (persistent grid size)
   int force_tier = -1;     // testing: 0 = wave, 1 = LDS sort, 2 = bitmap for every non-empty row

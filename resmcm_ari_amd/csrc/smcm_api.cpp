@@ -460,6 +460,7 @@ void run_core(Chain& ch, const int32_t* forced_pairs, Csr* C, const smcm::BlockC
     ch.spgemm_time = now_s() - t0;
     st.products = ss.products;
     st.nnz = ss.nnz;
+    st.ref_nnz = ch.mats[0].nnz;
     st.n_spgemm = 0;
   } else {
     const double t0 = now_s();
@@ -477,6 +478,7 @@ void run_core(Chain& ch, const int32_t* forced_pairs, Csr* C, const smcm::BlockC
     if (!consumer) st.nnz = out.nnz;
   }
   if (!consumer) st.nnz = out.nnz;
+  if (L > 1) st.ref_nnz = st.nnz;
   ctx->last = st;
   if (times) {
     times->total = ch.total;
@@ -582,6 +584,15 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "d_threads") {
       if (v != 256 && v != 512 && v != 1024) fail(SMCM_EINVAL, "d_threads must be 256, 512 or 1024");
       o.d_threads = (int)v;
+    } else if (k == "w_threads") {
+      if (v != 256 && v != 512 && v != 1024) fail(SMCM_EINVAL, "w_threads must be 256, 512 or 1024");
+      o.w_threads = (int)v;
+    } else if (k == "window_cols") {
+      if (v != 0 && (v < 64 || (v & (v - 1)) != 0)) fail(SMCM_EINVAL, "window_cols must be 0 or a power of two >= 64");
+      o.window_cols = v;
+    } else if (k == "bitmap_vcap") {
+      if (v < -1) fail(SMCM_EINVAL, "bitmap_vcap must be -1 or >= 0");
+      o.bitmap_vcap = v;
     } else if (k == "force_tier") {
       if (v < -1 || v > 2) fail(SMCM_EINVAL, "force_tier must be -1..2");
       o.force_tier = (int)v;
@@ -611,6 +622,9 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "dense_div") *v = o.dense_div;
     else if (k == "heavy_ub") *v = o.heavy_ub;
     else if (k == "d_threads") *v = o.d_threads;
+    else if (k == "w_threads") *v = o.w_threads;
+    else if (k == "window_cols") *v = o.window_cols;
+    else if (k == "bitmap_vcap") *v = o.bitmap_vcap;
     else if (k == "force_tier") *v = o.force_tier;
     else if (k == "exact_cost_sum") *v = ctx->exact_cost_sum;
     else if (k == "kernel_timing") *v = ctx->timer.enabled;
@@ -734,6 +748,7 @@ int smcm_chain_run_blocked(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, 
     };
     run_core(ch, nullptr, nullptr, &consume, (int64_t)block_bytes, times);
     ctx->last.nnz = total;
+    if (L > 1) ctx->last.ref_nnz = total;
     if (nnz) *nnz = total;
     if (row_digest && nrows)
       SMCM_HIP(hipMemcpyAsync(row_digest, dig.p, sizeof(uint64_t) * nrows, hipMemcpyDeviceToHost, al.stream));

@@ -102,12 +102,13 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
   }
 }
 
-// sum of symbolic row widths per tier
+// sum of symbolic row widths per tier; the empty tier has no row list (its rows are only
+// counted), so blockIdx.y covers the tiers after it
 __global__ void __launch_bounds__(kThreads) tier_nnz_kernel(const int32_t* __restrict__ bins, int32_t m,
                                                             const int32_t* __restrict__ counts,
                                                             const int64_t* __restrict__ row_nnz,
                                                             unsigned long long* __restrict__ out) {
-  const int t = blockIdx.y;
+  const int t = kEmpty + 1 + (int)blockIdx.y;
   const int32_t n = counts[t];
   int64_t s = 0;
   for (int32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
@@ -157,7 +158,20 @@ __device__ __forceinline__ void for_each_product(const int64_t rs, const int64_t
       const int64_t b0 = bptr[k], b1 = bptr[k + 1];
       const uint64_t a = AV ? aval[ee] : 1ull;
       const int64_t p0 = NEED_POS ? S[ee] - S0 : 0;
-      for (int64_t b = b0 + lane; b < b1; b += kWave) f(bcol[b], BV ? a * bval[b] : a, p0 + (b - b0));
+      // four loads in flight per lane before their products are applied
+      int64_t b = b0 + lane;
+      for (; b + 3 * kWave < b1; b += 4 * kWave) {
+        int32_t c[4];
+        uint64_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          c[u] = bcol[b + u * kWave];
+          v[u] = BV ? bval[b + u * kWave] : 1ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) f(c[u], BV ? a * v[u] : a, p0 + (b + u * kWave - b0));
+      }
+      for (; b < b1; b += kWave) f(bcol[b], BV ? a * bval[b] : a, p0 + (b - b0));
     }
     __syncthreads();
     if (threadIdx.x == 0) *qn = 0;
@@ -483,12 +497,217 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// window tier (numeric pass of rows wider than the bitmap tier's LDS value slots).
+// A dense accumulator over a window of W columns lives in LDS (W uint64 values + presence
+// bits).  Every A entry keeps a cursor into its (sorted) B row; a window consumes, for every
+// entry, the run of elements below the window end, so each product is visited exactly once.
+// Finished windows are compacted in column order (wave per 64-bit presence word, lane = bit)
+// and the next window starts at the smallest column still pending.  Cursors live in LDS for
+// rows with at most kCurLds entries, otherwise in a global scratch array aligned with A.
+constexpr int kCurLds = 2048;
+
+struct WindowLayout {
+  int64_t w = 0;  // window width in columns (power of two >= 64)
+  size_t bytes() const { return (size_t)w * 8 + (size_t)w / 8 + (size_t)w / 16 + (size_t)kCurLds * 4; }
+};
+
+template <bool AV, bool BV>
+__global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol,
+                                   const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
+                                   const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
+                                   const int32_t* __restrict__ rows, int32_t nrows, int32_t ncols, int32_t W,
+                                   int32_t* __restrict__ queue, int32_t* __restrict__ gcur,
+                                   const int64_t* __restrict__ cptr, int32_t* __restrict__ ccol,
+                                   uint64_t* __restrict__ cval) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint64_t* vals = (uint64_t*)smem;
+  uint32_t* pres = (uint32_t*)(smem + (size_t)W * 8);
+  const uint64_t* pres64 = (const uint64_t*)pres;
+  uint32_t* woff = pres + W / 32;
+  int32_t* lcur = (int32_t*)(woff + W / 64);
+  __shared__ int32_t q[1024];
+  __shared__ int qn;
+  __shared__ int32_t cur;
+  __shared__ int32_t next_min;
+  __shared__ int32_t red32[1024 / kWave];
+  const int nthreads = blockDim.x, nwaves = nthreads / kWave;
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  for (int i = threadIdx.x; i < W; i += nthreads) vals[i] = 0ull;
+  for (int i = threadIdx.x; i < W / 32; i += nthreads) pres[i] = 0u;
+  if (threadIdx.x == 0) qn = 0;
+  __syncthreads();
+  while (true) {
+    if (threadIdx.x == 0) cur = atomicAdd(queue, 1);
+    __syncthreads();
+    const int32_t ri = cur;
+    __syncthreads();
+    if (ri >= nrows) break;
+    const int32_t u = rows[ri];
+    const int64_t rs = aptr[u], re = aptr[u + 1];
+    const int64_t d = re - rs;
+    int32_t* cu = d <= kCurLds ? lcur : gcur + rs;
+    for (int64_t i = threadIdx.x; i < d; i += nthreads) cu[i] = 0;
+    const int64_t rowbase = cptr[u];
+    int64_t done = 0;
+    int64_t w0 = 0;
+    __syncthreads();
+    while (w0 < ncols) {
+      const int64_t wend = min(w0 + (int64_t)W, (int64_t)ncols);
+      if (threadIdx.x == 0) next_min = INT32_MAX;
+      __syncthreads();
+      for (int64_t base = rs; base < re; base += nthreads) {
+        const int64_t e = base + threadIdx.x;
+        if (e < re) {
+          const int32_t k = acol[e];
+          const int64_t b0 = bptr[k], len = bptr[k + 1] - b0;
+          int64_t off = cu[e - rs];
+          if (off < len) {
+            if (len - off <= kShortRow) {
+              const uint64_t a = AV ? aval[e] : 1ull;
+              for (; off < len; ++off) {
+                const int32_t c = bcol[b0 + off];
+                if (c >= wend) {
+                  atomicMin(&next_min, c);
+                  break;
+                }
+                const int32_t x = (int32_t)(c - w0);
+                atomicAdd((unsigned long long*)&vals[x], (unsigned long long)(BV ? a * bval[b0 + off] : a));
+                atomicOr(&pres[x >> 5], 1u << (x & 31));
+              }
+              cu[e - rs] = (int32_t)off;
+            } else {
+              q[atomicAdd(&qn, 1)] = (int32_t)(e - base);
+            }
+          }
+        }
+        __syncthreads();
+        const int n = qn;
+        for (int qi = wid; qi < n; qi += nwaves) {
+          const int64_t ee = base + q[qi];
+          const int32_t k = acol[ee];
+          const int64_t b0 = bptr[k], len = bptr[k + 1] - b0;
+          const uint64_t a = AV ? aval[ee] : 1ull;
+          int64_t off = cu[ee - rs];
+          while (true) {
+            // B rows are sorted: the elements below the window end form a prefix of the lanes
+            const int64_t x = off + lane;
+            int32_t c = INT32_MAX;
+            uint64_t bv = 1ull;
+            if (x < len) {
+              c = bcol[b0 + x];
+              if (BV) bv = bval[b0 + x];
+            }
+            const bool in = x < len && c < wend;
+            if (in) {
+              const int32_t xx = (int32_t)(c - w0);
+              atomicAdd((unsigned long long*)&vals[xx], (unsigned long long)(BV ? a * bv : a));
+              atomicOr(&pres[xx >> 5], 1u << (xx & 31));
+            }
+            const int cnt = __popcll(__ballot(in));
+            off += cnt;
+            if (cnt < kWave) {
+              if (lane == cnt && off < len) atomicMin(&next_min, c);
+              break;
+            }
+          }
+          if (lane == 0) cu[ee - rs] = (int32_t)off;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) qn = 0;
+        __syncthreads();
+      }
+      // compact the window in column order
+      const int64_t nwords = (wend - w0 + 63) / 64;
+      const int64_t per = (nwords + nthreads - 1) / nthreads;
+      const int64_t wa = threadIdx.x * per, wb = min(wa + per, nwords);
+      int32_t s = 0;
+      for (int64_t i = wa; i < wb; ++i) s += __popcll(pres64[i]);
+      int32_t wtot;
+      int32_t off = block_exscan_i32(s, red32, &wtot);
+      for (int64_t i = wa; i < wb; ++i) {
+        woff[i] = (uint32_t)off;
+        off += __popcll(pres64[i]);
+      }
+      __syncthreads();
+      for (int64_t i = wid; i < nwords; i += nwaves) {
+        const uint64_t word = pres64[i];
+        if (!word) continue;
+        if ((word >> lane) & 1ull) {
+          const int64_t pos = rowbase + done + woff[i] + __popcll(word & ((1ull << lane) - 1ull));
+          const int64_t x = i * 64 + lane;
+          ccol[pos] = (int32_t)(w0 + x);
+          cval[pos] = vals[x];
+          vals[x] = 0ull;
+        }
+      }
+      __syncthreads();
+      for (int64_t i = threadIdx.x; i < nwords * 2; i += nthreads) pres[i] = 0u;
+      done += wtot;
+      const int32_t nm = next_min;
+      __syncthreads();
+      w0 = nm == INT32_MAX ? (int64_t)ncols : (int64_t)(nm & ~63);
+    }
+  }
+}
+
+// split the bitmap-tier rows (heavy first) by exact width: <= vcap stay in the bitmap kernel,
+// wider rows go to the window kernel; per-class sums of A entries, products and widths
+__global__ void __launch_bounds__(kThreads) split_wide_kernel(const int32_t* __restrict__ rows0, int32_t n0,
+                                                              const int32_t* __restrict__ rows1, int32_t n1,
+                                                              const int64_t* __restrict__ aptr,
+                                                              const int64_t* __restrict__ S,
+                                                              const int64_t* __restrict__ cptr, int64_t vcap,
+                                                              int32_t* __restrict__ narrow, int32_t* __restrict__ wide,
+                                                              int32_t* __restrict__ counts,
+                                                              unsigned long long* __restrict__ sums) {
+  const int32_t i = blockIdx.x * kThreads + threadIdx.x;
+  int cls = -1;
+  int32_t u = 0;
+  int64_t d = 0, ub = 0, w = 0;
+  if (i < n0 + n1) {
+    u = i < n0 ? rows0[i] : rows1[i - n0];
+    w = cptr[u + 1] - cptr[u];
+    d = aptr[u + 1] - aptr[u];
+    ub = S[aptr[u + 1]] - S[aptr[u]];
+    cls = w > vcap ? 1 : 0;
+  }
+  const int lane = lane_id();
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t mask = __ballot(cls == c);
+    if (!mask) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    const int64_t sd = wave_sum_i64(cls == c ? d : 0);
+    const int64_t su = wave_sum_i64(cls == c ? ub : 0);
+    const int64_t sw = wave_sum_i64(cls == c ? w : 0);
+    int32_t base = 0;
+    if (lane == leader) {
+      base = atomicAdd(&counts[c], (int32_t)__popcll(mask));
+      atomicAdd(&sums[3 * c], (unsigned long long)sd);
+      atomicAdd(&sums[3 * c + 1], (unsigned long long)su);
+      atomicAdd(&sums[3 * c + 2], (unsigned long long)sw);
+    }
+    base = __shfl(base, leader);
+    if (cls == c) (c == 0 ? narrow : wide)[base + lanes_below(mask)] = u;
+  }
+}
+
 // per-tier accounting of one pass: rows, A entries, products, output entries
 struct TierCounts {
   int32_t rows[kNumBins] = {0, 0, 0, 0, 0};
   int64_t a_nnz[kNumBins] = {0, 0, 0, 0, 0};
   int64_t products[kNumBins] = {0, 0, 0, 0, 0};
   int64_t c_nnz[kNumBins] = {0, 0, 0, 0, 0};
+};
+
+// scratch of the numeric bitmap/window split (device)
+struct SplitScratch {
+  int32_t* narrow = nullptr;
+  int32_t* wide = nullptr;
+  int32_t* counts = nullptr;          // 2
+  unsigned long long* sums = nullptr;  // 6
+  int32_t* gcur = nullptr;            // cursor per A entry
+  WindowLayout wl;
 };
 
 // algorithmic bytes of a pass over one tier: A and C row pointers, A columns (+values) with
@@ -503,7 +722,7 @@ double tier_bytes(bool numeric, const TierCounts& tc, int t, bool av, bool bv) {
 template <bool NUMERIC, bool AV, bool BV>
 void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const TierCounts& tc,
                   int64_t* row_nnz, const int64_t* cptr, int32_t* ccol, uint64_t* cval, const SpgemmOpts& o,
-                  const BitmapLayout& lay, int32_t* queue, hipStream_t st, KernelTimer* tm) {
+                  const BitmapLayout& lay, SplitScratch* split, int32_t* queue, hipStream_t st, KernelTimer* tm) {
   const int32_t m = A.nrows;
   const int32_t nW = tc.rows[kWaveTier], nS = tc.rows[kSortTier], nB = tc.rows[kBitmapTier],
                 nH = tc.rows[kHeavyTier];
@@ -530,40 +749,92 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     SMCM_LAUNCH_CHECK();
     T.stop(tok, st, tier_bytes(NUMERIC, tc, kSortTier, AV, BV));
   }
-  if (nB + nH > 0) {
+  if (nB + nH > 0 && !NUMERIC) {
     fill_i32(queue, 1, 0, st);
-    const size_t lds = NUMERIC ? lay.bytes_num() : lay.bytes_sym();
+    const size_t lds = lay.bytes_sym();
     const int per_cu = std::max<int64_t>(1, std::min<int64_t>(4, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
     const unsigned g = (unsigned)std::min<int64_t>(nB + nH, (int64_t)o.cus * per_cu);
-    const int tok = T.start(NUMERIC ? "spgemm.numeric.bitmap" : "spgemm.symbolic.bitmap", st);
+    TierCounts both;
+    both.rows[0] = nB + nH;
+    both.a_nnz[0] = tc.a_nnz[kBitmapTier] + tc.a_nnz[kHeavyTier];
+    both.products[0] = tc.products[kBitmapTier] + tc.products[kHeavyTier];
+    const int tok = T.start("spgemm.symbolic.bitmap", st);
     bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(
         A.ptr, A.col, A.val, B.ptr, B.col, B.val, bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m,
         nB, B.ncols, lay.nw, lay.vcap, queue, row_nnz, cptr, ccol, cval);
     SMCM_LAUNCH_CHECK();
-    T.stop(tok, st, tier_bytes(NUMERIC, tc, kBitmapTier, AV, BV) + tier_bytes(NUMERIC, tc, kHeavyTier, AV, BV) -
-                        (NUMERIC ? 16.0 : 16.0));
+    T.stop(tok, st, tier_bytes(false, both, 0, AV, BV));
+  }
+  if (nB + nH > 0 && NUMERIC) {
+    // split by exact width: narrow rows keep the LDS bitmap + rank kernel, wide rows use windows
+    fill_i32(split->counts, 2, 0, st);
+    fill_i64((int64_t*)split->sums, 6, 0, st);
+    split_wide_kernel<<<(unsigned)ceil_div(nB + nH, kThreads), kThreads, 0, st>>>(
+        bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m, nB, A.ptr, S, cptr, lay.vcap,
+        split->narrow, split->wide, split->counts, split->sums);
+    SMCM_LAUNCH_CHECK();
+    int32_t cnt[2];
+    unsigned long long sums[6];
+    SMCM_HIP(hipMemcpyAsync(cnt, split->counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipMemcpyAsync(sums, split->sums, sizeof(sums), hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipStreamSynchronize(st));
+    TierCounts cls;
+    for (int c = 0; c < 2; ++c) {
+      cls.rows[c] = cnt[c];
+      cls.a_nnz[c] = (int64_t)sums[3 * c];
+      cls.products[c] = (int64_t)sums[3 * c + 1];
+      cls.c_nnz[c] = (int64_t)sums[3 * c + 2];
+    }
+    if (cnt[0] > 0) {
+      fill_i32(queue, 1, 0, st);
+      const size_t lds = lay.bytes_num();
+      const int per_cu = std::max<int64_t>(1, std::min<int64_t>(4, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
+      const unsigned g = (unsigned)std::min<int64_t>(cnt[0], (int64_t)o.cus * per_cu);
+      const int tok = T.start("spgemm.numeric.bitmap", st);
+      bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val,
+                                                                       nullptr, 0, split->narrow, cnt[0], B.ncols,
+                                                                       lay.nw, lay.vcap, queue, row_nnz, cptr, ccol,
+                                                                       cval);
+      SMCM_LAUNCH_CHECK();
+      T.stop(tok, st, tier_bytes(true, cls, 0, AV, BV));
+    }
+    if (cnt[1] > 0) {
+      fill_i32(queue, 1, 0, st);
+      const size_t lds = split->wl.bytes();
+      const unsigned g = (unsigned)std::min<int64_t>(cnt[1], (int64_t)o.cus);
+      const int tok = T.start("spgemm.numeric.window", st);
+      window_tier_kernel<AV, BV><<<g, o.w_threads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, split->wide,
+                                                              cnt[1], B.ncols, (int32_t)split->wl.w, queue,
+                                                              split->gcur, cptr, ccol, cval);
+      SMCM_LAUNCH_CHECK();
+      T.stop(tok, st, tier_bytes(true, cls, 1, AV, BV));
+    }
   }
 }
 
 template <bool NUMERIC>
 void dispatch(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const TierCounts& tc,
               int64_t* row_nnz, const int64_t* cptr, int32_t* ccol, uint64_t* cval, const SpgemmOpts& o,
-              const BitmapLayout& lay, int32_t* queue, hipStream_t st, KernelTimer* tm) {
+              const BitmapLayout& lay, SplitScratch* split, int32_t* queue, hipStream_t st, KernelTimer* tm) {
   const bool av = A.val != nullptr, bv = B.val != nullptr;
   if (av && bv)
-    launch_tiers<NUMERIC, true, true>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
+    launch_tiers<NUMERIC, true, true>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, split, queue, st, tm);
   else if (av)
-    launch_tiers<NUMERIC, true, false>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
+    launch_tiers<NUMERIC, true, false>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, split, queue, st, tm);
   else if (bv)
-    launch_tiers<NUMERIC, false, true>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
+    launch_tiers<NUMERIC, false, true>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, split, queue, st, tm);
   else
-    launch_tiers<NUMERIC, false, false>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, queue, st, tm);
+    launch_tiers<NUMERIC, false, false>(A, B, S, bins, tc, row_nnz, cptr, ccol, cval, o, lay, split, queue, st,
+                                        tm);
 }
 
 template <bool NUMERIC, bool AV, bool BV>
 void set_lds_attr(size_t bytes) {
   (void)hipFuncSetAttribute((const void*)bitmap_tier_kernel<NUMERIC, AV, BV>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if constexpr (NUMERIC)
+    (void)hipFuncSetAttribute((const void*)window_tier_kernel<AV, BV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
 }
 
 void set_all_lds_attrs(size_t bytes) {
@@ -593,6 +864,21 @@ BitmapLayout choose_layout(int64_t ncols, const SpgemmOpts& o) {
   l.vcap = (budget - (int64_t)l.bytes_num()) / 8;
   if (l.vcap > (1 << 16)) l.vcap = 1 << 16;
   if (l.vcap < 0) l.vcap = 0;
+  if (o.bitmap_vcap >= 0) l.vcap = std::min(l.vcap, o.bitmap_vcap);
+  return l;
+}
+
+WindowLayout choose_window(int64_t ncols, const SpgemmOpts& o) {
+  const int64_t budget = o.lds_bytes - 8 * 1024;
+  WindowLayout l;
+  l.w = 64;
+  while (l.w < (1 << 20)) {
+    WindowLayout n2;
+    n2.w = l.w * 2;
+    if ((int64_t)n2.bytes() > budget || n2.w > std::max<int64_t>(64, 2 * ncols)) break;
+    l = n2;
+  }
+  if (o.window_cols > 0) l.w = std::min<int64_t>(l.w, o.window_cols);
   return l;
 }
 
@@ -612,6 +898,9 @@ struct SpgemmCore {
   DevBuf<int64_t> S;
   DevBuf<int32_t> bins, counts, queue;
   DevBuf<unsigned long long> sums;
+  DevBuf<int32_t> split_rows, split_counts, gcur;
+  DevBuf<unsigned long long> split_sums;
+  SplitScratch split;
   TierCounts tc;
   BitmapLayout lay;
   int64_t products = 0;
@@ -635,7 +924,17 @@ struct SpgemmCore {
     queue = DevBuf<int32_t>(al, 1);
     sums = DevBuf<unsigned long long>(al, 3 * kNumBins);
     lay = choose_layout(B.ncols, o);
-    set_all_lds_attrs(std::max(lay.bytes_num(), lay.bytes_sym()));
+    split.wl = choose_window(B.ncols, o);
+    set_all_lds_attrs(std::max({lay.bytes_num(), lay.bytes_sym(), split.wl.bytes()}));
+    split_rows = DevBuf<int32_t>(al, 2 * (int64_t)(A.nrows > 0 ? A.nrows : 1));
+    split_counts = DevBuf<int32_t>(al, 2);
+    split_sums = DevBuf<unsigned long long>(al, 6);
+    gcur = DevBuf<int32_t>(al, A.nnz > 0 ? A.nnz : 1);
+    split.narrow = split_rows.p;
+    split.wide = split_rows.p + (A.nrows > 0 ? A.nrows : 1);
+    split.counts = split_counts.p;
+    split.sums = split_sums.p;
+    split.gcur = gcur.p;
     SMCM_HIP(hipMemcpyAsync(&products, S.p + A.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   }
 
@@ -664,10 +963,10 @@ struct SpgemmCore {
   // widths of the binned rows into row_nnz (other entries untouched); fills tc.c_nnz
   void symbolic(int64_t* row_nnz) {
     hipStream_t st = al.stream;
-    dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, queue.p, st, tm);
+    dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, &split, queue.p, st, tm);
     unsigned long long* tn = sums.p + 2 * kNumBins;
     fill_i64((int64_t*)tn, kNumBins, 0, st);
-    tier_nnz_kernel<<<dim3(64, kNumBins), kThreads, 0, st>>>(bins.p, A.nrows, counts.p, row_nnz, tn);
+    tier_nnz_kernel<<<dim3(64, kNumBins - 1 - kEmpty), kThreads, 0, st>>>(bins.p, A.nrows, counts.p, row_nnz, tn);
     SMCM_LAUNCH_CHECK();
     unsigned long long h[kNumBins];
     SMCM_HIP(hipMemcpyAsync(h, tn, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -678,7 +977,7 @@ struct SpgemmCore {
   // numeric pass of the binned rows; cptr holds absolute offsets, ccol/cval are shifted so
   // that ccol[cptr[u]] is the first output of row u
   void numeric(const int64_t* cptr, int32_t* ccol, uint64_t* cval) {
-    dispatch<true>(A, B, S.p, bins.p, tc, nullptr, cptr, ccol, cval, o, lay, queue.p, al.stream, tm);
+    dispatch<true>(A, B, S.p, bins.p, tc, nullptr, cptr, ccol, cval, o, lay, &split, queue.p, al.stream, tm);
   }
 
   void add_stats(SpgemmStats* st) const {

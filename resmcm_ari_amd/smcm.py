@@ -41,7 +41,8 @@ class Times(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("products", C.c_int64), ("nnz", C.c_int64), ("rows_tier", C.c_int64 * 4),
-                ("n_spgemm", C.c_int32), ("pairs", C.c_int32 * 30), ("c_root", C.c_double)]
+                ("n_spgemm", C.c_int32), ("pairs", C.c_int32 * 30), ("c_root", C.c_double),
+                ("ref_nnz", C.c_int64)]
 
 
 class SmcmError(RuntimeError):
@@ -130,7 +131,8 @@ class Context:
         _check(load().smcm_last_stats(self._h, C.byref(s)))
         n = max(s.n_spgemm, 0)
         return {"products": int(s.products), "nnz": int(s.nnz), "rows_tier": list(s.rows_tier),
-                "n_spgemm": int(s.n_spgemm), "pairs": [int(x) for x in s.pairs[: 2 * n]], "c_root": float(s.c_root)}
+                "n_spgemm": int(s.n_spgemm), "pairs": [int(x) for x in s.pairs[: 2 * n]], "c_root": float(s.c_root),
+                "ref_nnz": int(s.ref_nnz)}
 
     def sync(self) -> None:
         _check(load().smcm_ctx_sync(self._h))

@@ -167,3 +167,29 @@ def cfg1(ctx):
 def test_config1(cfg1, p):
     hg, G = cfg1
     check_chain(hg, G, configs.metapath_from_letters(p, "bib"))
+
+
+def test_cli_matches_reference_lines(tmp_path):
+    """The drop-in program writes the reference's line format; its nnz column equals the
+    reference program's on the same inputs (fixtures from the reference build)."""
+    import os
+    import subprocess
+
+    from resmcm_ari_amd import _build
+    import test_oracle_golden as TG
+
+    for name in ["edge", "bib_tiny", "cfg1"]:
+        z, hg = TG.load(name)
+        files = [str(tmp_path / f"{name}_{x}.txt") for x in ("graph", "vertex", "edge")]
+        hg.write_text(*files)
+        mpf = tmp_path / f"{name}_mp.txt"
+        mpf.write_text("".join(str(p) + "\n" for p in z["paths"]))
+        out = tmp_path / f"{name}_out.txt"
+        subprocess.run([os.path.join(_build.LIBDIR, "smcm_resmcm"), "8", "1", *files, str(mpf), str(out)],
+                       check=True, stdout=subprocess.DEVNULL)
+        got = [line.split() for line in out.read_text().splitlines()]
+        want = TG.ref_lines(name, "resmcm")
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            assert len(g) == 6
+            assert g[0] == w[0] and g[1] == w[1], (name, g, w)

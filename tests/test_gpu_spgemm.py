@@ -82,3 +82,45 @@ def test_spgemm_wraparound(ctx):
         assert list(col) == [0, 2]
         assert list(val) == [int(big), 0]
         assert_same_csr(ptr, col, val, ref, what=f"wrap tier={tier}")
+
+
+@pytest.mark.parametrize("window", [64, 256, 0])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("vals", ["none", "AB"])
+def test_spgemm_window_tier(ctx, case, window, vals):
+    """Every non-empty row through the bitmap tier with no LDS value slots: the numeric pass
+    runs in the window kernel (cursor-driven dense LDS windows)."""
+    name, m, k, n, la, lb = case
+    rng = np.random.default_rng(zlib.crc32(f"win/{name}/{window}/{vals}".encode()))
+    A = random_csr(rng, m, k, row_len=la, with_vals=vals == "AB")
+    B = random_csr(rng, k, n, row_len=lb, with_vals=vals == "AB")
+    ref = O.spgemm(A, B)
+    ctx.set_option("force_tier", 2)
+    ctx.set_option("bitmap_vcap", 0)
+    ctx.set_option("window_cols", window)
+    try:
+        ptr, col, val = _run(ctx, A, B)
+    finally:
+        ctx.set_option("force_tier", -1)
+        ctx.set_option("bitmap_vcap", -1)
+        ctx.set_option("window_cols", 0)
+    assert_same_csr(ptr, col, val, ref, what=f"{name} window={window} vals={vals}")
+
+
+def test_spgemm_window_many_entries(ctx):
+    """Rows with more A entries than the LDS cursor slots use the global cursor array."""
+    rng = np.random.default_rng(77)
+    A = random_csr(rng, 6, 5000, row_len=lambda r: r.integers(2500, 4000))
+    B = random_csr(rng, 5000, 3000, row_len=lambda r: r.integers(0, 40), with_vals=True)
+    ref = O.spgemm(A, B)
+    for window in (64, 1024, 0):
+        ctx.set_option("force_tier", 2)
+        ctx.set_option("bitmap_vcap", 0)
+        ctx.set_option("window_cols", window)
+        try:
+            ptr, col, val = _run(ctx, A, B)
+        finally:
+            ctx.set_option("force_tier", -1)
+            ctx.set_option("bitmap_vcap", -1)
+            ctx.set_option("window_cols", 0)
+        assert_same_csr(ptr, col, val, ref, what=f"many entries window={window}")
