@@ -116,8 +116,9 @@ def main() -> int:
     t0 = time.perf_counter()
     prods_step, nnz_step = 0, 0
     for k in range(args.steps):
+        ts = time.perf_counter()
         prods_step, nnz_step = step()
-        print(f"[bench] step {k} done", file=sys.stderr, flush=True)
+        print(f"[bench] step {k} done in {time.perf_counter() - ts:.3f} s", file=sys.stderr, flush=True)
     ctx.sync()
     dt = time.perf_counter() - t0
     ks = ctx.kernel_stats()

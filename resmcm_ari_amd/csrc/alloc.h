@@ -1,26 +1,58 @@
-// alloc.h - stream-ordered device memory (hipMallocAsync on the context's stream, with the
-// device's default pool configured to keep freed blocks for reuse).
+// alloc.h - stream-ordered device memory for one context.
+//
+// Small blocks go through hipMallocAsync / hipFreeAsync on the context's stream.  Large blocks
+// (>= kCacheMin) are kept mapped after they are freed and handed out again (best fit, at most
+// 1.5x the request): remapping tens of GB of HBM for every call costs far more than the
+// products themselves.  All work of a context runs on its one stream, so a block freed here can
+// be reused by any later allocation in stream order.  The cache is released when an allocation
+// fails and when the context is destroyed.
 #pragma once
+
+#include <map>
+#include <unordered_map>
 
 #include "common.h"
 
 namespace smcm {
 
 struct DevAlloc {
+  static constexpr size_t kCacheMin = (size_t)64 << 20;
   hipStream_t stream = nullptr;
   size_t live_bytes = 0;
   size_t peak_bytes = 0;
+  size_t cached_bytes = 0;
+  std::unordered_map<void*, size_t> sizes;
+  std::multimap<size_t, void*> cache;
 
   void* alloc(size_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
     if (bytes == 0) bytes = 256;
     void* p = nullptr;
-    hipError_t e = hipMallocAsync(&p, bytes, stream);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw HipError((int)e, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
-                                 hipGetErrorString(e));
+    size_t got = bytes;
+    if (bytes >= kCacheMin) {
+      auto it = cache.lower_bound(bytes);
+      if (it != cache.end() && it->first <= bytes + bytes / 2) {
+        p = it->second;
+        got = it->first;
+        cached_bytes -= got;
+        cache.erase(it);
+      }
     }
-    live_bytes += bytes;
+    if (!p) {
+      hipError_t e = hipMallocAsync(&p, bytes, stream);
+      if (e == hipErrorOutOfMemory && cached_bytes > 0) {
+        (void)hipGetLastError();
+        release_cache();
+        e = hipMallocAsync(&p, bytes, stream);
+      }
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw HipError((int)e, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                   hipGetErrorString(e));
+      }
+    }
+    sizes[p] = got;
+    live_bytes += got;
     if (live_bytes > peak_bytes) peak_bytes = live_bytes;
     return p;
   }
@@ -28,10 +60,23 @@ struct DevAlloc {
   T* alloc_n(int64_t n) {
     return (T*)alloc(sizeof(T) * (size_t)(n > 0 ? n : 1));
   }
-  void free(void* p, size_t bytes = 0) {
+  void free(void* p) {
     if (!p) return;
+    auto it = sizes.find(p);
+    const size_t sz = it != sizes.end() ? it->second : 0;
+    if (it != sizes.end()) sizes.erase(it);
+    live_bytes = live_bytes > sz ? live_bytes - sz : 0;
+    if (sz >= kCacheMin) {
+      cache.emplace(sz, p);
+      cached_bytes += sz;
+      return;
+    }
     SMCM_HIP(hipFreeAsync(p, stream));
-    live_bytes = live_bytes > bytes ? live_bytes - bytes : 0;
+  }
+  void release_cache() {
+    for (auto& kv : cache) (void)hipFreeAsync(kv.second, stream);
+    cache.clear();
+    cached_bytes = 0;
   }
 };
 
@@ -56,7 +101,7 @@ struct DevBuf {
   }
   ~DevBuf() { reset(); }
   void reset() {
-    if (p && a) a->free(p, sizeof(T) * (size_t)(n > 0 ? n : 1));
+    if (p && a) a->free(p);
     p = nullptr;
   }
   T* release() {

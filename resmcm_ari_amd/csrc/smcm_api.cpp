@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -473,6 +475,9 @@ void run_core(Chain& ch, const int32_t* forced_pairs, Csr* C, const smcm::BlockC
     }
     out = ch.execute(plan, &st, consumer, block_bytes);
     ch.total = now_s() - t0;
+    if (std::getenv("SMCM_TRACE"))
+      fprintf(stderr, "[smcm] chain L=%d: est %.3f dp %.3f spgemm %.3f total %.3f ms\n", L, ch.es_time * 1e3,
+              ch.dp_time * 1e3, ch.spgemm_time * 1e3, ch.total * 1e3);
     for (size_t q = 0; q < plan.size() && q < 30; ++q) st.pairs[q] = plan[q];
     st.c_root = ch.c[L - 1];
     if (!consumer) st.nnz = out.nnz;
@@ -552,6 +557,7 @@ int smcm_ctx_destroy(smcm_ctx* ctx) {
     if (ctx->live != 0)
       fail(SMCM_EINVAL, std::to_string(ctx->live) + " graph/result handle(s) still alive; free them first");
     (void)hipSetDevice(ctx->device);
+    ctx->al.release_cache();
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

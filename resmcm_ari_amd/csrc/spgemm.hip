@@ -993,6 +993,7 @@ struct SpgemmCore {
 
 void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr& C, const SpgemmOpts& o,
             SpgemmStats* stats, DevAlloc& al, bool timing, KernelTimer* tm) {
+  const bool trace = std::getenv("SMCM_TRACE") != nullptr;
   if (A.ncols != B.nrows) throw HipError(-22, "spgemm: inner dimensions differ");
   if (!B.sorted) throw HipError(-22, "spgemm: right operand rows must be sorted");
   hipStream_t st = al.stream;
@@ -1033,6 +1034,10 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
     stats->ms_numeric = ms_since(t2);
     stats->nnz = nnz;
   }
+  if (trace)
+    fprintf(stderr, "[smcm] spgemm %dx%d * %dx%d: analysis %.3f symbolic %.3f numeric(+alloc) %.3f ms, nnz %lld\n",
+            A.nrows, A.ncols, B.nrows, B.ncols, std::chrono::duration<double, std::milli>(t1 - t0).count(),
+            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), (long long)nnz);
 }
 
 void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_stride, int64_t block_bytes,
@@ -1042,6 +1047,8 @@ void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_str
   if (!B.sorted) throw HipError(-22, "spgemm: right operand rows must be sorted");
   hipStream_t st = al.stream;
   const int32_t m = A.nrows;
+  const bool trace = std::getenv("SMCM_TRACE") != nullptr;
+  auto t0 = std::chrono::steady_clock::now();
   SpgemmCore core(A, B, est_r, est_stride, o, al, tm);
   core.bin(0, m);
   if (stats) {
@@ -1049,6 +1056,7 @@ void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_str
     stats->products = core.products;
     core.add_stats(stats);
   }
+  auto t1 = std::chrono::steady_clock::now();
   DevBuf<int64_t> ptr(al, (int64_t)m + 1);
   {
     DevBuf<int64_t> row_nnz(al, m > 0 ? m : 1);
@@ -1076,11 +1084,17 @@ void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_str
     widest = std::max<int64_t>(widest, hptr[e] - hptr[b]);
     b = e;
   }
-  const bool trace = std::getenv("SMCM_TRACE") != nullptr;
+  auto t2 = std::chrono::steady_clock::now();
   // block buffers are allocated once and reused by every block
   DevBuf<int32_t> bcol(al, widest);
   DevBuf<uint64_t> bval(al, widest);
   DevBuf<int64_t> bptr(al, (int64_t)m + 1);
+  if (trace) {
+    SMCM_HIP(hipStreamSynchronize(st));
+    fprintf(stderr, "[smcm] blocked %dx%d * %dx%d: analysis %.3f symbolic %.3f alloc %.3f ms, %zu blocks\n", A.nrows,
+            A.ncols, B.nrows, B.ncols, std::chrono::duration<double, std::milli>(t1 - t0).count(),
+            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), blocks.size());
+  }
   for (auto [b, e] : blocks) {
     auto tb = std::chrono::steady_clock::now();
     const int64_t p0 = hptr[b], cnt = hptr[e] - hptr[b];
