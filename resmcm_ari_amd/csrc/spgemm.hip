@@ -213,6 +213,127 @@ __device__ __forceinline__ int32_t block_exscan_i32(int32_t x, int32_t* red, int
   return off + incl - x;
 }
 
+__device__ __forceinline__ int64_t block_exscan_i64(int64_t x, int64_t* red, int64_t* total) {
+  const int lane = lane_id(), wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  int64_t incl = x;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int64_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  __syncthreads();
+  if (lane == kWave - 1) red[wid] = incl;
+  __syncthreads();
+  int64_t off = 0, tot = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (w < wid) off += red[w];
+    tot += red[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - x;
+}
+
+// ------------------------------------------------------------------------------------------
+// Balanced product traversal for the workgroup-per-row tiers.  For one row of A, every entry e
+// (B row k) contributes a segment [off, off + cnt) of row k of B, given by seg().  Segments of at
+// most kShortRow products are walked by the thread that owns the entry.  Longer segments are
+// laid out back to back (prefix sums in LDS) and the concatenation is split evenly over the
+// waves: lanes take consecutive products (coalesced loads, four in flight per lane) and find
+// their segment by a binary search followed by forward steps.  A single very long B row (a hub)
+// is therefore shared by the whole workgroup instead of being walked by one wave.
+constexpr int kChunk = 512;  // A entries per traversal round
+constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
+
+struct TravLds {
+  int64_t start[kChunk];  // first B index of the segment
+  uint64_t av[kChunk];    // A value of the entry
+  int64_t P[kChunk + 1];  // exclusive prefix of long-segment lengths
+  int64_t red[1024 / kWave];
+};
+
+template <bool AV, bool BV, class SegF, class F>
+__device__ __forceinline__ void traverse(const int64_t rs, const int64_t re, const int32_t* __restrict__ acol,
+                                         const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
+                                         const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
+                                         TravLds& L, SegF&& seg, F&& f) {
+  const int nwaves = blockDim.x / kWave;
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  const int chunk = min(kChunk, (int)blockDim.x);  // entries handled per round (one per thread)
+  for (int64_t base = rs; base < re; base += chunk) {
+    int64_t mylen = 0;
+    if ((int)threadIdx.x < chunk) {
+      const int64_t e = base + threadIdx.x;
+      int64_t st = 0;
+      uint64_t a = 1ull;
+      if (e < re) {
+        const int32_t k = acol[e];
+        const int64_t b0 = bptr[k];
+        const int64_t len = bptr[k + 1] - b0;
+        int64_t off = 0, cnt = 0;
+        seg(e, k, b0, len, off, cnt);
+        a = AV ? aval[e] : 1ull;
+        st = b0 + off;
+        if (cnt <= kShortRow) {
+          for (int64_t x = 0; x < cnt; ++x) f(bcol[st + x], BV ? a * bval[st + x] : a);
+        } else {
+          mylen = cnt;
+        }
+      }
+      L.start[threadIdx.x] = st;
+      L.av[threadIdx.x] = a;
+    }
+    int64_t total;
+    const int64_t ex = block_exscan_i64(mylen, L.red, &total);
+    if ((int)threadIdx.x < chunk) L.P[threadIdx.x] = ex;
+    if (threadIdx.x == 0) L.P[chunk] = total;
+    __syncthreads();
+    if (total > 0) {
+      const int64_t lo = total * wid / nwaves, hi = total * (wid + 1) / nwaves;
+      int64_t p = lo + lane;
+      // first segment with P[t] <= p < P[t+1]
+      int t = 0;
+      {
+        int a = 0, b = chunk;  // P[a] <= p < P[b] holds for a = 0 when p < total
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (L.P[mid] <= p) a = mid;
+          else b = mid;
+        }
+        t = a;
+      }
+      for (; p < hi; p += 4 * kWave) {
+        int tt[4];
+        int64_t bi[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t q = p + u * kWave;
+          ok[u] = q < hi;
+          if (ok[u]) {
+            while (L.P[t + 1] <= q) ++t;
+            tt[u] = t;
+            bi[u] = L.start[t] + (q - L.P[t]);
+          }
+        }
+        int32_t c[4];
+        uint64_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (ok[u]) {
+            c[u] = bcol[bi[u]];
+            v[u] = BV ? bval[bi[u]] : 1ull;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (ok[u]) f(c[u], BV ? L.av[tt[u]] * v[u] : L.av[tt[u]]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // wave tier: ub <= 64.  One wave per row, 4 rows per block.
 template <bool NUMERIC, bool AV, bool BV>
@@ -397,15 +518,17 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
   const uint64_t* bm64 = (const uint64_t*)smem;
   uint32_t* rank = (uint32_t*)(smem + bm_rank_off(nw));
   uint64_t* sval = (uint64_t*)(smem + bm_val_off(nw));
-  __shared__ int32_t q[1024];
-  __shared__ int qn;
+  __shared__ TravLds L;
   __shared__ int32_t cur;
   __shared__ int64_t red64[1024 / kWave];
   __shared__ int32_t red32[1024 / kWave];
   const int nthreads = blockDim.x;
   const int nwaves = nthreads / kWave;
   const int lane = lane_id(), wid = threadIdx.x / kWave;
-  if (threadIdx.x == 0) qn = 0;
+  auto whole_row = [](int64_t, int32_t, int64_t, int64_t len, int64_t& off, int64_t& cnt) {
+    off = 0;
+    cnt = len;
+  };
   const int32_t total_rows = n0 + n1;
   while (true) {
     if (threadIdx.x == 0) cur = atomicAdd(queue, 1);
@@ -431,11 +554,11 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
       const int64_t nwords64 = (wlen + 63) / 64;
       for (int64_t i = threadIdx.x; i < nwords64 * 2; i += nthreads) bm[i] = 0u;
       __syncthreads();
-      for_each_product<false, false, false>(rs, re, acol, nullptr, bptr, bcol, nullptr, nullptr, q, &qn,
-                                            [&](int32_t c, uint64_t, int64_t) {
-                                              const int64_t x = (int64_t)c - w0;
-                                              if (x >= 0 && x < wlen) atomicOr(&bm[x >> 5], 1u << (x & 31));
-                                            });
+      traverse<false, false>(rs, re, acol, nullptr, bptr, bcol, nullptr, L, whole_row,
+                             [&](int32_t c, uint64_t) {
+                               const int64_t x = (int64_t)c - w0;
+                               if (x >= 0 && x < wlen) atomicOr(&bm[x >> 5], 1u << (x & 31));
+                             });
       if constexpr (!NUMERIC) {
         int64_t local = 0;
         for (int64_t i = threadIdx.x; i < nwords32; i += nthreads) local += __popc(bm[i]);
@@ -469,8 +592,7 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
           __threadfence();
         }
         __syncthreads();
-        for_each_product<AV, BV, false>(rs, re, acol, aval, bptr, bcol, bval, nullptr, q, &qn,
-                                        [&](int32_t c, uint64_t v, int64_t) {
+        traverse<AV, BV>(rs, re, acol, aval, bptr, bcol, bval, L, whole_row, [&](int32_t c, uint64_t v) {
                                           const int64_t x = (int64_t)c - w0;
                                           if (x >= 0 && x < wlen) {
                                             const int64_t wd = x >> 6;
@@ -499,125 +621,105 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
 
 // ------------------------------------------------------------------------------------------
 // window tier (numeric pass of rows wider than the bitmap tier's LDS value slots).
-// A dense accumulator over a window of W columns lives in LDS (W uint64 values + presence
-// bits).  Every A entry keeps a cursor into its (sorted) B row; a window consumes, for every
-// entry, the run of elements below the window end, so each product is visited exactly once.
-// Finished windows are compacted in column order (wave per 64-bit presence word, lane = bit)
-// and the next window starts at the smallest column still pending.  Cursors live in LDS for
-// rows with at most kCurLds entries, otherwise in a global scratch array aligned with A.
-constexpr int kCurLds = 2048;
-
+// A dense accumulator over a window of W = 2^log2w columns lives in LDS (W uint64 values +
+// presence bits).  Windows are aligned (window w covers columns [w*W, (w+1)*W)); a bound table
+// built once per product gives, for every row k of B, the first element of k in each window, so
+// the segment of an entry in a window is found in O(1) and every product is visited exactly
+// once.  A finished window is compacted in column order (wave per 64-bit presence word, lane =
+// bit) and the row continues at the next window that holds any of its products.
 struct WindowLayout {
-  int64_t w = 0;  // window width in columns (power of two >= 64)
-  size_t bytes() const { return (size_t)w * 8 + (size_t)w / 8 + (size_t)w / 16 + (size_t)kCurLds * 4; }
+  int64_t w = 0;     // window width in columns (power of two >= 64)
+  int log2w = 6;
+  int64_t nwin = 0;  // windows covering the columns of B
+  size_t bytes() const { return (size_t)w * 8 + (size_t)w / 8 + (size_t)w / 16; }
 };
+
+// bnd[k * (nwin + 1) + w] = offset (within row k of B) of the first element with column >= w*W;
+// bnd[k * (nwin + 1) + nwin] = length of row k.  One wave per row; each entry written once.
+__global__ void __launch_bounds__(kThreads) window_bounds_kernel(const int64_t* __restrict__ bptr,
+                                                                 const int32_t* __restrict__ bcol, int32_t nrows,
+                                                                 int log2w, int32_t nwin, int32_t* __restrict__ bnd) {
+  const int lane = lane_id();
+  const int64_t wpb = kThreads / kWave;
+  for (int64_t k = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; k < nrows; k += (int64_t)gridDim.x * wpb) {
+    const int64_t b0 = bptr[k];
+    const int32_t len = (int32_t)(bptr[k + 1] - b0);
+    int32_t* out = bnd + (size_t)k * (nwin + 1);
+    for (int32_t x = lane; x < len; x += kWave) {
+      const int32_t w = bcol[b0 + x] >> log2w;
+      const int32_t wp = x == 0 ? -1 : (bcol[b0 + x - 1] >> log2w);
+      for (int32_t ww = wp + 1; ww <= w; ++ww) out[ww] = x;
+    }
+    const int32_t wl = len > 0 ? (bcol[b0 + len - 1] >> log2w) : -1;
+    for (int32_t ww = wl + 1 + lane; ww <= nwin; ww += kWave) out[ww] = len;
+  }
+}
 
 template <bool AV, bool BV>
 __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol,
                                    const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
                                    const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
-                                   const int32_t* __restrict__ rows, int32_t nrows, int32_t ncols, int32_t W,
-                                   int32_t* __restrict__ queue, int32_t* __restrict__ gcur,
+                                   const int32_t* __restrict__ rows, int32_t nrows, int32_t ncols, int log2w,
+                                   int32_t nwin, const int32_t* __restrict__ bnd, int32_t* __restrict__ queue,
                                    const int64_t* __restrict__ cptr, int32_t* __restrict__ ccol,
                                    uint64_t* __restrict__ cval) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const int32_t W = 1 << log2w;
   uint64_t* vals = (uint64_t*)smem;
   uint32_t* pres = (uint32_t*)(smem + (size_t)W * 8);
   const uint64_t* pres64 = (const uint64_t*)pres;
   uint32_t* woff = pres + W / 32;
-  int32_t* lcur = (int32_t*)(woff + W / 64);
-  __shared__ int32_t q[1024];
-  __shared__ int qn;
+  __shared__ TravLds L;
   __shared__ int32_t cur;
-  __shared__ int32_t next_min;
+  __shared__ int32_t next_win;
   __shared__ int32_t red32[1024 / kWave];
   const int nthreads = blockDim.x, nwaves = nthreads / kWave;
   const int lane = lane_id(), wid = threadIdx.x / kWave;
   for (int i = threadIdx.x; i < W; i += nthreads) vals[i] = 0ull;
   for (int i = threadIdx.x; i < W / 32; i += nthreads) pres[i] = 0u;
-  if (threadIdx.x == 0) qn = 0;
   __syncthreads();
   while (true) {
-    if (threadIdx.x == 0) cur = atomicAdd(queue, 1);
+    if (threadIdx.x == 0) {
+      cur = atomicAdd(queue, 1);
+      next_win = INT32_MAX;
+    }
     __syncthreads();
     const int32_t ri = cur;
-    __syncthreads();
     if (ri >= nrows) break;
     const int32_t u = rows[ri];
     const int64_t rs = aptr[u], re = aptr[u + 1];
-    const int64_t d = re - rs;
-    int32_t* cu = d <= kCurLds ? lcur : gcur + rs;
-    for (int64_t i = threadIdx.x; i < d; i += nthreads) cu[i] = 0;
     const int64_t rowbase = cptr[u];
-    int64_t done = 0;
-    int64_t w0 = 0;
+    // first window holding a product of this row
+    for (int64_t e = rs + threadIdx.x; e < re; e += nthreads) {
+      const int32_t k = acol[e];
+      const int64_t b0 = bptr[k];
+      if (bptr[k + 1] > b0) atomicMin(&next_win, bcol[b0] >> log2w);
+    }
     __syncthreads();
-    while (w0 < ncols) {
-      const int64_t wend = min(w0 + (int64_t)W, (int64_t)ncols);
-      if (threadIdx.x == 0) next_min = INT32_MAX;
+    int32_t w = next_win;
+    int64_t done = 0;
+    __syncthreads();
+    while (w < nwin) {
+      if (threadIdx.x == 0) next_win = INT32_MAX;
       __syncthreads();
-      for (int64_t base = rs; base < re; base += nthreads) {
-        const int64_t e = base + threadIdx.x;
-        if (e < re) {
-          const int32_t k = acol[e];
-          const int64_t b0 = bptr[k], len = bptr[k + 1] - b0;
-          int64_t off = cu[e - rs];
-          if (off < len) {
-            if (len - off <= kShortRow) {
-              const uint64_t a = AV ? aval[e] : 1ull;
-              for (; off < len; ++off) {
-                const int32_t c = bcol[b0 + off];
-                if (c >= wend) {
-                  atomicMin(&next_min, c);
-                  break;
-                }
-                const int32_t x = (int32_t)(c - w0);
-                atomicAdd((unsigned long long*)&vals[x], (unsigned long long)(BV ? a * bval[b0 + off] : a));
-                atomicOr(&pres[x >> 5], 1u << (x & 31));
-              }
-              cu[e - rs] = (int32_t)off;
-            } else {
-              q[atomicAdd(&qn, 1)] = (int32_t)(e - base);
-            }
-          }
-        }
-        __syncthreads();
-        const int n = qn;
-        for (int qi = wid; qi < n; qi += nwaves) {
-          const int64_t ee = base + q[qi];
-          const int32_t k = acol[ee];
-          const int64_t b0 = bptr[k], len = bptr[k + 1] - b0;
-          const uint64_t a = AV ? aval[ee] : 1ull;
-          int64_t off = cu[ee - rs];
-          while (true) {
-            // B rows are sorted: the elements below the window end form a prefix of the lanes
-            const int64_t x = off + lane;
-            int32_t c = INT32_MAX;
-            uint64_t bv = 1ull;
-            if (x < len) {
-              c = bcol[b0 + x];
-              if (BV) bv = bval[b0 + x];
-            }
-            const bool in = x < len && c < wend;
-            if (in) {
-              const int32_t xx = (int32_t)(c - w0);
-              atomicAdd((unsigned long long*)&vals[xx], (unsigned long long)(BV ? a * bv : a));
-              atomicOr(&pres[xx >> 5], 1u << (xx & 31));
-            }
-            const int cnt = __popcll(__ballot(in));
-            off += cnt;
-            if (cnt < kWave) {
-              if (lane == cnt && off < len) atomicMin(&next_min, c);
-              break;
-            }
-          }
-          if (lane == 0) cu[ee - rs] = (int32_t)off;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) qn = 0;
-        __syncthreads();
-      }
+      const int64_t w0 = (int64_t)w << log2w;
+      const int32_t wcur = w;
+      traverse<AV, BV>(
+          rs, re, acol, aval, bptr, bcol, bval, L,
+          [&](int64_t, int32_t k, int64_t b0, int64_t len, int64_t& off, int64_t& cnt) {
+            const int32_t* bk = bnd + (size_t)k * (nwin + 1);
+            const int32_t s0 = bk[wcur], s1 = bk[wcur + 1];
+            off = s0;
+            cnt = s1 - s0;
+            if (s1 < len) atomicMin(&next_win, bcol[b0 + s1] >> log2w);
+          },
+          [&](int32_t c, uint64_t v) {
+            const int32_t x = (int32_t)(c - w0);
+            atomicAdd((unsigned long long*)&vals[x], (unsigned long long)v);
+            atomicOr(&pres[x >> 5], 1u << (x & 31));
+          });
       // compact the window in column order
+      const int64_t wend = min(w0 + (int64_t)W, (int64_t)ncols);
       const int64_t nwords = (wend - w0 + 63) / 64;
       const int64_t per = (nwords + nthreads - 1) / nthreads;
       const int64_t wa = threadIdx.x * per, wb = min(wa + per, nwords);
@@ -644,9 +746,9 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
       __syncthreads();
       for (int64_t i = threadIdx.x; i < nwords * 2; i += nthreads) pres[i] = 0u;
       done += wtot;
-      const int32_t nm = next_min;
+      const int32_t nxt = next_win;
       __syncthreads();
-      w0 = nm == INT32_MAX ? (int64_t)ncols : (int64_t)(nm & ~63);
+      w = nxt;
     }
   }
 }
@@ -706,8 +808,10 @@ struct SplitScratch {
   int32_t* wide = nullptr;
   int32_t* counts = nullptr;          // 2
   unsigned long long* sums = nullptr;  // 6
-  int32_t* gcur = nullptr;            // cursor per A entry
+  int32_t* bnd = nullptr;             // window bound table of B (built on first use)
   WindowLayout wl;
+  struct SpgemmCore* owner = nullptr;
+  void build_bounds(const Csr& B, hipStream_t st);
 };
 
 // algorithmic bytes of a pass over one tier: A and C row pointers, A columns (+values) with
@@ -799,13 +903,15 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
       T.stop(tok, st, tier_bytes(true, cls, 0, AV, BV));
     }
     if (cnt[1] > 0) {
+      if (!split->bnd) split->build_bounds(B, st);
       fill_i32(queue, 1, 0, st);
       const size_t lds = split->wl.bytes();
       const unsigned g = (unsigned)std::min<int64_t>(cnt[1], (int64_t)o.cus);
       const int tok = T.start("spgemm.numeric.window", st);
       window_tier_kernel<AV, BV><<<g, o.w_threads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, split->wide,
-                                                              cnt[1], B.ncols, (int32_t)split->wl.w, queue,
-                                                              split->gcur, cptr, ccol, cval);
+                                                              cnt[1], B.ncols, split->wl.log2w,
+                                                              (int32_t)split->wl.nwin, split->bnd, queue, cptr, ccol,
+                                                              cval);
       SMCM_LAUNCH_CHECK();
       T.stop(tok, st, tier_bytes(true, cls, 1, AV, BV));
     }
@@ -853,8 +959,8 @@ void set_all_lds_attrs(size_t bytes) {
 }
 
 BitmapLayout choose_layout(int64_t ncols, const SpgemmOpts& o) {
-  // LDS budget for the dynamic part; static arrays and scan scratch take ~5 KB
-  const int64_t budget = o.lds_bytes - 8 * 1024;
+  // LDS budget for the dynamic part (the traversal's static arrays and scan scratch take the rest)
+  const int64_t budget = o.lds_bytes - kStaticLds;
   BitmapLayout l;
   const int64_t want = std::max<int64_t>(128, (ncols + 127) / 128 * 128);
   // numeric: nw/8 + nw/16 + 8*vcap <= budget with vcap >= 2048
@@ -869,16 +975,22 @@ BitmapLayout choose_layout(int64_t ncols, const SpgemmOpts& o) {
 }
 
 WindowLayout choose_window(int64_t ncols, const SpgemmOpts& o) {
-  const int64_t budget = o.lds_bytes - 8 * 1024;
+  const int64_t budget = o.lds_bytes - kStaticLds;
   WindowLayout l;
   l.w = 64;
+  l.log2w = 6;
   while (l.w < (1 << 20)) {
     WindowLayout n2;
     n2.w = l.w * 2;
     if ((int64_t)n2.bytes() > budget || n2.w > std::max<int64_t>(64, 2 * ncols)) break;
-    l = n2;
+    l.w = n2.w;
+    l.log2w += 1;
   }
-  if (o.window_cols > 0) l.w = std::min<int64_t>(l.w, o.window_cols);
+  while (o.window_cols > 0 && l.w > o.window_cols && l.w > 64) {
+    l.w >>= 1;
+    l.log2w -= 1;
+  }
+  l.nwin = std::max<int64_t>(1, (ncols + l.w - 1) / l.w);
   return l;
 }
 
@@ -898,7 +1010,7 @@ struct SpgemmCore {
   DevBuf<int64_t> S;
   DevBuf<int32_t> bins, counts, queue;
   DevBuf<unsigned long long> sums;
-  DevBuf<int32_t> split_rows, split_counts, gcur;
+  DevBuf<int32_t> split_rows, split_counts, bnd;
   DevBuf<unsigned long long> split_sums;
   SplitScratch split;
   TierCounts tc;
@@ -929,12 +1041,11 @@ struct SpgemmCore {
     split_rows = DevBuf<int32_t>(al, 2 * (int64_t)(A.nrows > 0 ? A.nrows : 1));
     split_counts = DevBuf<int32_t>(al, 2);
     split_sums = DevBuf<unsigned long long>(al, 6);
-    gcur = DevBuf<int32_t>(al, A.nnz > 0 ? A.nnz : 1);
     split.narrow = split_rows.p;
     split.wide = split_rows.p + (A.nrows > 0 ? A.nrows : 1);
     split.counts = split_counts.p;
     split.sums = split_sums.p;
-    split.gcur = gcur.p;
+    split.owner = this;
     SMCM_HIP(hipMemcpyAsync(&products, S.p + A.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   }
 
@@ -988,6 +1099,16 @@ struct SpgemmCore {
     st->rows[3] += tc.rows[kBitmapTier] + tc.rows[kHeavyTier];
   }
 };
+
+void SplitScratch::build_bounds(const Csr& B, hipStream_t st) {
+  owner->bnd = DevBuf<int32_t>(owner->al, (int64_t)B.nrows * (wl.nwin + 1));
+  bnd = owner->bnd.p;
+  if (B.nrows > 0) {
+    window_bounds_kernel<<<(unsigned)std::min<int64_t>(ceil_div(B.nrows, kThreads / kWave), 65536), kThreads, 0, st>>>(
+        B.ptr, B.col, B.nrows, wl.log2w, (int32_t)wl.nwin, bnd);
+    SMCM_LAUNCH_CHECK();
+  }
+}
 
 }  // namespace
 

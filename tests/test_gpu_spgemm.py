@@ -124,3 +124,28 @@ def test_spgemm_window_many_entries(ctx):
             ctx.set_option("bitmap_vcap", -1)
             ctx.set_option("window_cols", 0)
         assert_same_csr(ptr, col, val, ref, what=f"many entries window={window}")
+
+
+@pytest.mark.parametrize("threads", [256, 512, 1024])
+def test_spgemm_workgroup_sizes(ctx, threads):
+    """Bitmap and window kernels at every supported workgroup size (rows with more A entries
+    than one traversal round, long and short B rows)."""
+    rng = np.random.default_rng(threads)
+    A = random_csr(rng, 40, 3000, row_len=lambda r: r.integers(200, 1500), with_vals=True)
+    B = random_csr(rng, 3000, 2500, row_len=lambda r: int(r.zipf(1.4)) % 900, with_vals=True)
+    ref = O.spgemm(A, B)
+    for vcap in (-1, 0):
+        ctx.set_option("force_tier", 2)
+        ctx.set_option("bitmap_vcap", vcap)
+        ctx.set_option("d_threads", threads)
+        ctx.set_option("w_threads", threads)
+        ctx.set_option("window_cols", 256)
+        try:
+            ptr, col, val = _run(ctx, A, B)
+        finally:
+            ctx.set_option("force_tier", -1)
+            ctx.set_option("bitmap_vcap", -1)
+            ctx.set_option("d_threads", 512)
+            ctx.set_option("w_threads", 1024)
+            ctx.set_option("window_cols", 0)
+        assert_same_csr(ptr, col, val, ref, what=f"threads={threads} vcap={vcap}")
