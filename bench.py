@@ -134,6 +134,9 @@ def main() -> int:
     kernels = {k: {"ms": round(v[0], 3), "launches": v[1], "GBps": round(v[2] / v[0] / 1e6, 1) if v[0] else 0.0}
                for k, v in sorted(ks.items(), key=lambda kv: -kv[1][0])}
 
+    copy_gbps = ctx.copy_bandwidth(1 << 32, 10)
+    roofline["measured_copy_GBps"] = round(copy_gbps, 1)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hg, args.cpu_sample)

@@ -82,6 +82,10 @@ int smcm_ctx_sync(smcm_ctx* ctx);
 int smcm_kernel_stats(smcm_ctx* ctx, int32_t idx, char* name, int32_t cap, double* total_ms,
                       int64_t* count, double* bytes);
 int smcm_kernel_stats_reset(smcm_ctx* ctx);
+/* HBM copy bandwidth of a plain 16-byte-vector copy kernel over two buffers of `bytes` bytes,
+ * `iters` launches: *gbps = (read + write bytes) / time.  No counterpart in the reference: the
+ * measured bandwidth reference for the roofline figures. */
+int smcm_copy_bandwidth(smcm_ctx* ctx, uint64_t bytes, int32_t iters, double* gbps);
 
 /* ---- graph (replaces HeteroGraph + the per-chain BoolMatrix construction,
  *      efficient/resmcm.cpp:181-240, :374-396) ------------------------------------------- */

@@ -157,6 +157,13 @@ __global__ void __launch_bounds__(kThreads) row_digest_kernel(const int64_t* __r
   }
 }
 
+// plain device copy in 16-byte vectors (bandwidth reference)
+__global__ void __launch_bounds__(kThreads) copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst,
+                                                        int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    dst[i] = src[i];
+}
+
 unsigned grid_for(int64_t items, int per_block, int64_t cap = 1 << 20) {
   int64_t g = ceil_div(items, per_block);
   if (g < 1) g = 1;
@@ -258,6 +265,13 @@ void row_digest(const Csr& c, uint64_t* out, hipStream_t st) {
   if (c.nrows <= 0) return;
   row_digest_kernel<<<grid_for(c.nrows, kThreads / kWave, 65536), kThreads, 0, st>>>(c.ptr, c.col, c.val, c.nrows,
                                                                                      out);
+  SMCM_LAUNCH_CHECK();
+}
+
+void device_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
+  const int64_t n = bytes / 16;
+  if (n <= 0) return;
+  copy_kernel<<<grid_for(n, kThreads, 256 * 64), kThreads, 0, st>>>((const int4*)src, (int4*)dst, n);
   SMCM_LAUNCH_CHECK();
 }
 

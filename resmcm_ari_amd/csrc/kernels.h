@@ -31,6 +31,9 @@ void build_typed_csr(const int32_t* vtype, const int64_t* adj_ptr, const int32_t
 // out = rows rows_sel[0..nsel) of in (values copied when present)
 void csr_gather_rows(const Csr& in, const int32_t* rows_sel, int32_t nsel, Csr& out, DevAlloc& al);
 
+// bytes/16 16-byte copies (plain HBM copy: bandwidth reference and counter calibration)
+void device_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
+
 // dst[i] = map[src[i]]
 void map_columns(const int32_t* src, int64_t n, const int32_t* map, int32_t* dst, hipStream_t st);
 

@@ -779,6 +779,35 @@ int smcm_kernel_stats(smcm_ctx* ctx, int32_t idx, char* name, int32_t cap, doubl
   });
 }
 
+int smcm_copy_bandwidth(smcm_ctx* ctx, uint64_t bytes, int32_t iters, double* gbps) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (bytes < 16 || iters < 1 || !gbps) fail(SMCM_EINVAL, "bad arguments");
+    DevAlloc& al = ctx->al;
+    bytes = bytes / 16 * 16;
+    DevBuf<unsigned char> a(al, (int64_t)bytes), b(al, (int64_t)bytes);
+    smcm::fill_i32((int32_t*)a.p, (int64_t)bytes / 4, 1, al.stream);
+    smcm::device_copy(a.p, b.p, (int64_t)bytes, al.stream);  // warm-up
+    hipEvent_t e0, e1;
+    SMCM_HIP(hipEventCreate(&e0));
+    SMCM_HIP(hipEventCreate(&e1));
+    SMCM_HIP(hipEventRecord(e0, al.stream));
+    for (int32_t i = 0; i < iters; ++i) {
+      const int tok = ctx->timer.start("copy", al.stream);
+      smcm::device_copy(a.p, b.p, (int64_t)bytes, al.stream);
+      ctx->timer.stop(tok, al.stream, 2.0 * (double)bytes);
+    }
+    SMCM_HIP(hipEventRecord(e1, al.stream));
+    SMCM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SMCM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->timer.flush();
+    *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;  // read + write
+  });
+}
+
 int smcm_kernel_stats_reset(smcm_ctx* ctx) {
   return guarded([&] {
     if (!ctx) fail(SMCM_EINVAL, "null context");

@@ -31,7 +31,7 @@ EXPORTS = [
     "smcm_chain_run", "smcm_chain_run_plan", "smcm_chain_plan", "smcm_estimate", "smcm_estimator_rows",
     "smcm_result_nrows", "smcm_result_nnz", "smcm_result_copy", "smcm_result_copy_rows", "smcm_result_row_ids",
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
-    "smcm_kernel_stats", "smcm_kernel_stats_reset",
+    "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth",
 ]
 
 
@@ -88,6 +88,7 @@ def load() -> C.CDLL:
             "smcm_chain_run_blocked": [vp, vp, vp, vp, i32, vp, i64, u64, vp, vp, vp],
             "smcm_kernel_stats": [vp, i32, C.c_char_p, i32, vp, vp, vp],
             "smcm_kernel_stats_reset": [vp],
+            "smcm_copy_bandwidth": [vp, u64, i32, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -147,6 +148,12 @@ class Context:
             out[name.value.decode()] = (ms.value, int(cnt.value), by.value)
             i += 1
         return out
+
+    def copy_bandwidth(self, nbytes: int = 1 << 32, iters: int = 10) -> float:
+        """Measured HBM copy bandwidth (GB/s, read + write) of a plain copy kernel."""
+        g = C.c_double()
+        _check(load().smcm_copy_bandwidth(self._h, int(nbytes), int(iters), C.byref(g)))
+        return float(g.value)
 
     def kernel_stats_reset(self) -> None:
         _check(load().smcm_kernel_stats_reset(self._h))
