@@ -236,101 +236,133 @@ __device__ __forceinline__ int64_t block_exscan_i64(int64_t x, int64_t* red, int
 
 // ------------------------------------------------------------------------------------------
 // Balanced product traversal for the workgroup-per-row tiers.  For one row of A, every entry e
-// (B row k) contributes a segment [off, off + cnt) of row k of B, given by seg().  Segments of at
-// most kShortRow products are walked by the thread that owns the entry.  Longer segments are
-// laid out back to back (prefix sums in LDS) and the concatenation is split evenly over the
-// waves: lanes take consecutive products (coalesced loads, four in flight per lane) and find
-// their segment by a binary search followed by forward steps.  A single very long B row (a hub)
-// is therefore shared by the whole workgroup instead of being walked by one wave.
+// (B row k) contributes a segment [off, off + cnt) of row k of B, given by seg().  Segments are
+// classed by length and compacted (one packed block scan):
+//   cnt <= kTiny        walked by the thread that owns the entry;
+//   cnt <= kQuarter     a 16-lane quarter of a wave per segment, four segments per wave step;
+//   longer              cut into pieces of kPiece products taken round-robin by the waves,
+//                       lanes reading consecutive elements (four loads in flight per lane).
+// All lanes of a step read consecutive elements of one B row (coalesced), a single very long B
+// row (a hub) is shared by the whole workgroup, and there is no per-product search: a piece's
+// segment is found once per piece (wave-uniform LDS reads).
 constexpr int kChunk = 512;  // A entries per traversal round
+constexpr int kTiny = 2;
+constexpr int kQuarter = 16;
+constexpr int kPiece = 4 * kWave;
 constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
 
 struct TravLds {
-  int64_t start[kChunk];  // first B index of the segment
-  uint64_t av[kChunk];    // A value of the entry
-  int64_t P[kChunk + 1];  // exclusive prefix of long-segment lengths
+  int64_t start[kChunk];  // first B index of a compacted segment (quarter segments, then long)
+  uint64_t av[kChunk];    // A value of its entry
+  int32_t len[kChunk];    // its length
+  int32_t Q[kChunk + 1];  // exclusive prefix of piece counts over the long segments
   int64_t red[1024 / kWave];
 };
+
+// One traversal round: every participating thread brings at most one segment (st, cnt) of B
+// with the A value a of its entry (cnt = 0: none).  Ends with a barrier.
+template <bool BV, class F>
+__device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, const int32_t* __restrict__ bcol,
+                                           const uint64_t* __restrict__ bval, TravLds& L, F&& f) {
+  const int nwaves = blockDim.x / kWave;
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  int64_t packed = 0;  // pieces << 32 | long << 16 | quarter
+  if (cnt > 0) {
+    if (cnt <= kTiny) {
+      for (int64_t x = 0; x < cnt; ++x) f(bcol[st + x], BV ? a * bval[st + x] : a);
+    } else if (cnt <= kQuarter) {
+      packed = 1;
+    } else {
+      packed = ((int64_t)((cnt + kPiece - 1) / kPiece) << 32) | (1 << 16);
+    }
+  }
+  int64_t total;
+  const int64_t ex = block_exscan_i64(packed, L.red, &total);
+  const int32_t nquarter = (int32_t)(total & 0xffff), nlong = (int32_t)((total >> 16) & 0xffff);
+  const int32_t npieces = (int32_t)(total >> 32);
+  if (packed & 0xffff) {
+    const int32_t slot = (int32_t)(ex & 0xffff);
+    L.start[slot] = st;
+    L.av[slot] = a;
+    L.len[slot] = (int32_t)cnt;
+  } else if (packed) {
+    const int32_t li = (int32_t)((ex >> 16) & 0xffff);
+    L.start[nquarter + li] = st;
+    L.av[nquarter + li] = a;
+    L.len[nquarter + li] = (int32_t)cnt;
+    L.Q[li] = (int32_t)(ex >> 32);
+  }
+  if (threadIdx.x == 0) L.Q[nlong] = npieces;
+  __syncthreads();
+  // quarter segments, four per wave step (lane / 16 selects the segment)
+  const int q = lane >> 4, x = lane & 15;
+  for (int32_t g = wid; 4 * g < nquarter; g += 2 * nwaves) {
+    int32_t c[2];
+    uint64_t v[2], av2[2];
+    bool ok[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int32_t sidx = 4 * (g + h * nwaves) + q;
+      ok[h] = sidx < nquarter && x < L.len[sidx];
+      if (ok[h]) {
+        const int64_t b = L.start[sidx] + x;
+        c[h] = bcol[b];
+        v[h] = BV ? bval[b] : 1ull;
+        av2[h] = L.av[sidx];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (ok[h]) f(c[h], BV ? av2[h] * v[h] : av2[h]);
+  }
+  // pieces of the long segments
+  int t = 0;
+  for (int32_t i = wid; i < npieces; i += nwaves) {
+    while (L.Q[t + 1] <= i) ++t;  // wave-uniform; segments are visited in order
+    const int32_t j = i - L.Q[t];
+    const int32_t sidx = nquarter + t;
+    const int32_t n = min(kPiece, L.len[sidx] - j * kPiece);
+    const int64_t b = L.start[sidx] + (int64_t)j * kPiece;
+    const uint64_t av1 = L.av[sidx];
+    int32_t c[4];
+    uint64_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int xx = lane + u * kWave;
+      if (xx < n) {
+        c[u] = bcol[b + xx];
+        v[u] = BV ? bval[b + xx] : 1ull;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (lane + u * kWave < n) f(c[u], BV ? av1 * v[u] : av1);
+  }
+  __syncthreads();
+}
 
 template <bool AV, bool BV, class SegF, class F>
 __device__ __forceinline__ void traverse(const int64_t rs, const int64_t re, const int32_t* __restrict__ acol,
                                          const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
                                          const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
                                          TravLds& L, SegF&& seg, F&& f) {
-  const int nwaves = blockDim.x / kWave;
-  const int wid = threadIdx.x / kWave, lane = lane_id();
   const int chunk = min(kChunk, (int)blockDim.x);  // entries handled per round (one per thread)
   for (int64_t base = rs; base < re; base += chunk) {
-    int64_t mylen = 0;
+    int64_t st = 0, cnt = 0;
+    uint64_t a = 1ull;
     if ((int)threadIdx.x < chunk) {
       const int64_t e = base + threadIdx.x;
-      int64_t st = 0;
-      uint64_t a = 1ull;
       if (e < re) {
         const int32_t k = acol[e];
         const int64_t b0 = bptr[k];
         const int64_t len = bptr[k + 1] - b0;
-        int64_t off = 0, cnt = 0;
+        int64_t off = 0;
         seg(e, k, b0, len, off, cnt);
         a = AV ? aval[e] : 1ull;
         st = b0 + off;
-        if (cnt <= kShortRow) {
-          for (int64_t x = 0; x < cnt; ++x) f(bcol[st + x], BV ? a * bval[st + x] : a);
-        } else {
-          mylen = cnt;
-        }
-      }
-      L.start[threadIdx.x] = st;
-      L.av[threadIdx.x] = a;
-    }
-    int64_t total;
-    const int64_t ex = block_exscan_i64(mylen, L.red, &total);
-    if ((int)threadIdx.x < chunk) L.P[threadIdx.x] = ex;
-    if (threadIdx.x == 0) L.P[chunk] = total;
-    __syncthreads();
-    if (total > 0) {
-      const int64_t lo = total * wid / nwaves, hi = total * (wid + 1) / nwaves;
-      int64_t p = lo + lane;
-      // first segment with P[t] <= p < P[t+1]
-      int t = 0;
-      {
-        int a = 0, b = chunk;  // P[a] <= p < P[b] holds for a = 0 when p < total
-        while (b - a > 1) {
-          const int mid = (a + b) >> 1;
-          if (L.P[mid] <= p) a = mid;
-          else b = mid;
-        }
-        t = a;
-      }
-      for (; p < hi; p += 4 * kWave) {
-        int tt[4];
-        int64_t bi[4];
-        bool ok[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int64_t q = p + u * kWave;
-          ok[u] = q < hi;
-          if (ok[u]) {
-            while (L.P[t + 1] <= q) ++t;
-            tt[u] = t;
-            bi[u] = L.start[t] + (q - L.P[t]);
-          }
-        }
-        int32_t c[4];
-        uint64_t v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (ok[u]) {
-            c[u] = bcol[bi[u]];
-            v[u] = BV ? bval[bi[u]] : 1ull;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (ok[u]) f(c[u], BV ? L.av[tt[u]] * v[u] : L.av[tt[u]]);
       }
     }
-    __syncthreads();
+    trav_round<BV>(st, cnt, a, bcol, bval, L, f);
   }
 }
 
@@ -667,21 +699,22 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
   const int32_t W = 1 << log2w;
   uint64_t* vals = (uint64_t*)smem;
   uint32_t* pres = (uint32_t*)(smem + (size_t)W * 8);
-  const uint64_t* pres64 = (const uint64_t*)pres;
-  uint32_t* woff = pres + W / 32;
+  uint64_t* pres64 = (uint64_t*)pres;
   __shared__ TravLds L;
-  __shared__ int32_t cur;
-  __shared__ int32_t next_win;
-  __shared__ int32_t red32[1024 / kWave];
+  __shared__ int32_t cur, wmin, wmax;
+  __shared__ int32_t wcnt[1024 / kWave];
   const int nthreads = blockDim.x, nwaves = nthreads / kWave;
   const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int chunk = min(kChunk, nthreads);
+  const uint64_t below = (1ull << lane) - 1ull;
   for (int i = threadIdx.x; i < W; i += nthreads) vals[i] = 0ull;
   for (int i = threadIdx.x; i < W / 32; i += nthreads) pres[i] = 0u;
   __syncthreads();
   while (true) {
     if (threadIdx.x == 0) {
       cur = atomicAdd(queue, 1);
-      next_win = INT32_MAX;
+      wmin = INT32_MAX;
+      wmax = -1;
     }
     __syncthreads();
     const int32_t ri = cur;
@@ -689,66 +722,105 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
     const int32_t u = rows[ri];
     const int64_t rs = aptr[u], re = aptr[u + 1];
     const int64_t rowbase = cptr[u];
-    // first window holding a product of this row
+    const int64_t width = cptr[u + 1] - rowbase;
+    // rows with at most `chunk` entries keep their entry (B row, A value, position) in registers
+    const bool fast = re - rs <= chunk;
+    int32_t my_k = 0, my_len = 0;
+    int64_t my_b0 = 0;
+    uint64_t my_a = 1ull;
+    // window range of the row: first / last window holding one of its products
     for (int64_t e = rs + threadIdx.x; e < re; e += nthreads) {
       const int32_t k = acol[e];
       const int64_t b0 = bptr[k];
-      if (bptr[k + 1] > b0) atomicMin(&next_win, bcol[b0] >> log2w);
+      const int32_t len = (int32_t)(bptr[k + 1] - b0);
+      if (len > 0) {
+        atomicMin(&wmin, bcol[b0] >> log2w);
+        atomicMax(&wmax, bcol[b0 + len - 1] >> log2w);
+      }
+      if (fast) {
+        my_k = k;
+        my_b0 = b0;
+        my_len = len;
+        my_a = AV ? aval[e] : 1ull;
+      }
     }
     __syncthreads();
-    int32_t w = next_win;
-    int64_t done = 0;
-    __syncthreads();
-    while (w < nwin) {
-      if (threadIdx.x == 0) next_win = INT32_MAX;
-      __syncthreads();
-      const int64_t w0 = (int64_t)w << log2w;
-      const int32_t wcur = w;
-      traverse<AV, BV>(
-          rs, re, acol, aval, bptr, bcol, bval, L,
-          [&](int64_t, int32_t k, int64_t b0, int64_t len, int64_t& off, int64_t& cnt) {
-            const int32_t* bk = bnd + (size_t)k * (nwin + 1);
-            const int32_t s0 = bk[wcur], s1 = bk[wcur + 1];
-            off = s0;
-            cnt = s1 - s0;
-            if (s1 < len) atomicMin(&next_win, bcol[b0 + s1] >> log2w);
-          },
-          [&](int32_t c, uint64_t v) {
-            const int32_t x = (int32_t)(c - w0);
-            atomicAdd((unsigned long long*)&vals[x], (unsigned long long)v);
-            atomicOr(&pres[x >> 5], 1u << (x & 31));
-          });
-      // compact the window in column order
-      const int64_t wend = min(w0 + (int64_t)W, (int64_t)ncols);
-      const int64_t nwords = (wend - w0 + 63) / 64;
-      const int64_t per = (nwords + nthreads - 1) / nthreads;
-      const int64_t wa = threadIdx.x * per, wb = min(wa + per, nwords);
-      int32_t s = 0;
-      for (int64_t i = wa; i < wb; ++i) s += __popcll(pres64[i]);
-      int32_t wtot;
-      int32_t off = block_exscan_i32(s, red32, &wtot);
-      for (int64_t i = wa; i < wb; ++i) {
-        woff[i] = (uint32_t)off;
-        off += __popcll(pres64[i]);
-      }
-      __syncthreads();
-      for (int64_t i = wid; i < nwords; i += nwaves) {
-        const uint64_t word = pres64[i];
-        if (!word) continue;
-        if ((word >> lane) & 1ull) {
-          const int64_t pos = rowbase + done + woff[i] + __popcll(word & ((1ull << lane) - 1ull));
-          const int64_t x = i * 64 + lane;
-          ccol[pos] = (int32_t)(w0 + x);
-          cval[pos] = vals[x];
-          vals[x] = 0ull;
+    const int32_t wfirst = wmin, wlast = wmax;
+    // pass 0 accumulates values only and takes the nonzero slots as the row's columns; if that
+    // finds fewer columns than the symbolic width (a count summed to 0 mod 2^64), pass 1 redoes
+    // the row with explicit presence bits
+    for (int pass = 0; pass < 2; ++pass) {
+      const bool with_pres = pass == 1;
+      int64_t done = 0;
+      int32_t my_s = 0;  // fast path: position of the entry's next element in its B row
+      for (int32_t w = wfirst; w <= wlast; ++w) {
+        const int64_t w0 = (int64_t)w << log2w;
+        auto acc = [&](int32_t c, uint64_t v) {
+          const int32_t x = (int32_t)(c - w0);
+          atomicAdd((unsigned long long*)&vals[x], (unsigned long long)v);
+          if (with_pres) atomicOr(&pres[x >> 5], 1u << (x & 31));
+        };
+        if (fast) {
+          int64_t st = 0, cnt = 0;
+          if (rs + (int64_t)threadIdx.x < re) {
+            const int32_t s1 = bnd[(size_t)my_k * (nwin + 1) + w + 1];
+            st = my_b0 + my_s;
+            cnt = s1 - my_s;
+            my_s = s1;
+          }
+          trav_round<BV>(st, cnt, my_a, bcol, bval, L, acc);
+        } else {
+          const int32_t wcur = w;
+          traverse<AV, BV>(
+              rs, re, acol, aval, bptr, bcol, bval, L,
+              [&](int64_t, int32_t k, int64_t, int64_t, int64_t& off, int64_t& cnt) {
+                const int32_t* bk = bnd + (size_t)k * (nwin + 1);
+                off = bk[wcur];
+                cnt = bk[wcur + 1] - bk[wcur];
+              },
+              acc);
         }
+        // compaction in column order: each wave owns a contiguous run of 64-column words
+        const int64_t wend = min(w0 + (int64_t)W, (int64_t)ncols);
+        const int32_t nwords = (int32_t)((wend - w0 + 63) / 64);
+        const int32_t per = (nwords + nwaves - 1) / nwaves;
+        const int32_t ja = min(wid * per, nwords), jb = min(ja + per, nwords);
+        int32_t mine = 0;
+        for (int32_t j = ja; j < jb; ++j) {
+          const uint64_t m = with_pres ? pres64[j] : __ballot(vals[j * 64 + lane] != 0ull);
+          mine += __popcll(m);
+        }
+        if (lane == 0) wcnt[wid] = mine;
+        __syncthreads();
+        int32_t woff = 0, wtot = 0;
+        for (int v2 = 0; v2 < nwaves; ++v2) {
+          const int32_t cw = wcnt[v2];
+          woff += v2 < wid ? cw : 0;
+          wtot += cw;
+        }
+        int64_t pos0 = done + woff;
+        for (int32_t j = ja; j < jb; ++j) {
+          const int32_t xi = j * 64 + lane;
+          const uint64_t v = vals[xi];
+          const uint64_t m = with_pres ? pres64[j] : __ballot(v != 0ull);
+          if ((m >> lane) & 1ull) {
+            const int64_t slot = pos0 + __popcll(m & below);
+            if (slot < width) {
+              ccol[rowbase + slot] = (int32_t)(w0 + xi);
+              cval[rowbase + slot] = v;
+            }
+          }
+          if (m) vals[xi] = 0ull;
+          if (with_pres && lane == 0) pres64[j] = 0ull;
+          pos0 += __popcll(m);
+        }
+        done += wtot;
+        // the next window's first accumulations (inline tiny segments) precede its first barrier:
+        // every wave must be done with this window's slots before any wave moves on
+        __syncthreads();
       }
       __syncthreads();
-      for (int64_t i = threadIdx.x; i < nwords * 2; i += nthreads) pres[i] = 0u;
-      done += wtot;
-      const int32_t nxt = next_win;
-      __syncthreads();
-      w = nxt;
+      if (done == width) break;
     }
   }
 }
@@ -906,7 +978,11 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
       if (!split->bnd) split->build_bounds(B, st);
       fill_i32(queue, 1, 0, st);
       const size_t lds = split->wl.bytes();
-      const unsigned g = (unsigned)std::min<int64_t>(cnt[1], (int64_t)o.cus);
+      // workgroups per CU: bounded by LDS and by 32 resident waves per CU
+      const int64_t by_lds = o.lds_bytes / (int64_t)(lds + kStaticLds);
+      const int64_t by_waves = 2048 / o.w_threads;
+      const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(by_lds, by_waves));
+      const unsigned g = (unsigned)std::min<int64_t>(cnt[1], (int64_t)o.cus * per_cu);
       const int tok = T.start("spgemm.numeric.window", st);
       window_tier_kernel<AV, BV><<<g, o.w_threads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, split->wide,
                                                               cnt[1], B.ncols, split->wl.log2w,

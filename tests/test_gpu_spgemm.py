@@ -149,3 +149,23 @@ def test_spgemm_workgroup_sizes(ctx, threads):
             ctx.set_option("w_threads", 1024)
             ctx.set_option("window_cols", 0)
         assert_same_csr(ptr, col, val, ref, what=f"threads={threads} vcap={vcap}")
+
+
+def test_window_tier_wraparound(ctx):
+    """A count that sums to 0 mod 2^64 is still a structural entry in the window kernel (its
+    value-only pass finds fewer columns than the symbolic width and the row is redone with
+    presence bits)."""
+    big = np.uint64(2**63)
+    A = O.CSR(2, 2, np.array([0, 2, 3], np.int64), np.array([0, 1, 1], np.int32),
+              np.array([big, big, 5], np.uint64))
+    B = O.CSR(2, 3, np.array([0, 2, 3], np.int64), np.array([0, 2, 2], np.int32), np.array([1, 1, 1], np.uint64))
+    ref = O.spgemm(A, B)
+    ctx.set_option("force_tier", 2)
+    ctx.set_option("bitmap_vcap", 0)
+    try:
+        ptr, col, val = _run(ctx, A, B)
+    finally:
+        ctx.set_option("force_tier", -1)
+        ctx.set_option("bitmap_vcap", -1)
+    assert list(col) == [0, 2, 2] and list(val) == [int(big), 0, 5]
+    assert_same_csr(ptr, col, val, ref, what="window wrap")
