@@ -315,28 +315,44 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
     for (int h = 0; h < 2; ++h)
       if (ok[h]) f(c[h], BV ? av2[h] * v[h] : av2[h]);
   }
-  // pieces of the long segments
+  // pieces of the long segments, two per wave step (eight loads in flight per lane)
   int t = 0;
-  for (int32_t i = wid; i < npieces; i += nwaves) {
-    while (L.Q[t + 1] <= i) ++t;  // wave-uniform; segments are visited in order
-    const int32_t j = i - L.Q[t];
-    const int32_t sidx = nquarter + t;
-    const int32_t n = min(kPiece, L.len[sidx] - j * kPiece);
-    const int64_t b = L.start[sidx] + (int64_t)j * kPiece;
-    const uint64_t av1 = L.av[sidx];
-    int32_t c[4];
-    uint64_t v[4];
+  for (int32_t i = wid; i < npieces; i += 2 * nwaves) {
+    int32_t n[2];
+    int64_t b[2];
+    uint64_t av1[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int xx = lane + u * kWave;
-      if (xx < n) {
-        c[u] = bcol[b + xx];
-        v[u] = BV ? bval[b + xx] : 1ull;
+    for (int h = 0; h < 2; ++h) {
+      const int32_t ii = i + h * nwaves;
+      n[h] = 0;
+      if (ii < npieces) {
+        while (L.Q[t + 1] <= ii) ++t;  // wave-uniform; segments are visited in order
+        const int32_t j = ii - L.Q[t];
+        const int32_t sidx = nquarter + t;
+        n[h] = min(kPiece, L.len[sidx] - j * kPiece);
+        b[h] = L.start[sidx] + (int64_t)j * kPiece;
+        av1[h] = L.av[sidx];
+      }
+    }
+    int32_t c[2][4];
+    uint64_t v[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int xx = lane + u * kWave;
+        if (xx < n[h]) {
+          c[h][u] = bcol[b[h] + xx];
+          v[h][u] = BV ? bval[b[h] + xx] : 1ull;
+        }
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (lane + u * kWave < n) f(c[u], BV ? av1 * v[u] : av1);
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (lane + u * kWave < n[h]) f(c[h][u], BV ? av1[h] * v[h][u] : av1[h]);
+    }
   }
   __syncthreads();
 }
