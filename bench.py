@@ -31,6 +31,22 @@ from resmcm_ari_amd import configs, smcm  # noqa: E402
 
 PEAK_HBM_GBPS = 8000.0
 CHAINS = ["A-P-V-P-A", "A-P-T-P-A"]
+WORKLOAD_TAG = "bib_A500k_P800k_V5k_T20k.APVPA_APTPA"
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/<kernel>.<workload>.md, written by resmcm_ari_amd.rocprof_summary);
+    None when there is no such summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}.{WORKLOAD_TAG}.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
+                return float(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(hg, sample_frac: float, seed: int = 1234) -> dict:
@@ -131,8 +147,15 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
                 "launches": cnt, "avg_launch_ms": round(ms / cnt, 3),
                 "algorithmic_bytes_per_launch": round(byts / cnt)}
-    kernels = {k: {"ms": round(v[0], 3), "launches": v[1], "GBps": round(v[2] / v[0] / 1e6, 1) if v[0] else 0.0}
+    kernels = {k: {"ms": round(v[0], 3), "launches": v[1], "GBps": round(v[2] / v[0] / 1e6, 1) if v[0] else 0.0,
+                   "bytes_per_launch": round(v[2] / v[1]) if v[1] else 0}
                for k, v in sorted(ks.items(), key=lambda kv: -kv[1][0])}
+
+    if chains == CHAINS:
+        traffic, src = profiled_traffic(name)
+        if traffic:
+            roofline["traffic"] = round(traffic)
+            roofline["traffic_source"] = src
 
     copy_gbps = ctx.copy_bandwidth(1 << 32, 10)
     roofline["measured_copy_GBps"] = round(copy_gbps, 1)

@@ -1,0 +1,188 @@
+"""Short text summaries of rocprofv3 CSV output for the kernels of this library.
+
+    python -m resmcm_ari_amd.rocprof_summary --trace DIR [--fetch DIR --write DIR] [--bench FILE]
+        [--kernel NAME] [--command TEXT ...] [--title TEXT]
+
+--trace   directory of `rocprofv3 --kernel-trace --stats --output-format csv` (its *_kernel_stats.csv)
+--fetch   directory of `rocprofv3 --pmc FETCH_SIZE --output-format csv` (its *_counter_collection.csv)
+--write   the same for WRITE_SIZE
+--bench   a file holding the JSON line bench.py printed in the traced run (HIP-event durations and the
+          algorithmic bytes per launch of each timed kernel)
+--kernel  the timed kernel name the traffic line is written for (default: bench's roofline kernel)
+
+Traffic calibration: the copy kernel (`copy_kernel`, 16-byte loads and stores, launched by
+`smcm_copy_bandwidth` with a known byte count) runs in the same process; the ratio of its known
+bytes to what FETCH_SIZE / WRITE_SIZE report for it is the scale applied to every other kernel.
+The last line of the output, `traffic_bytes_per_launch: <kernel> <bytes>`, is what bench.py reads.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import re
+import sys
+from collections import defaultdict
+
+# timed-kernel name -> pattern on the HIP kernel name (template arguments included)
+TIMED = {
+    "spgemm.numeric.window": r"window_tier_kernel<",
+    "spgemm.numeric.bitmap": r"bitmap_tier_kernel<true",
+    "spgemm.symbolic.bitmap": r"bitmap_tier_kernel<false",
+    "spgemm.numeric.sort": r"sort_tier_kernel<true",
+    "spgemm.symbolic.sort": r"sort_tier_kernel<false",
+    "spgemm.numeric.wave": r"wave_tier_kernel<true",
+    "spgemm.symbolic.wave": r"wave_tier_kernel<false",
+    "spgemm.analysis": r"^entry_len_kernel$",
+    "estimator.pass": r"est_(light|heavy)_kernel<",
+    "copy": r"^copy_kernel$",
+}
+
+
+def short_name(full: str) -> str:
+    s = full.strip()
+    if s.startswith("void "):
+        s = s[5:]
+    s = s.replace("smcm::(anonymous namespace)::", "").replace("smcm::", "")
+    depth, out = 0, []
+    for ch in s:  # drop the parameter list, keep template arguments
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            break
+        out.append(ch)
+    return "".join(out)
+
+
+def one_csv(d: str, suffix: str) -> str:
+    files = sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    return files[0]
+
+
+def kernel_stats(d: str):
+    rows = []
+    with open(one_csv(d, "_kernel_stats.csv")) as f:
+        for r in csv.DictReader(f):
+            rows.append((short_name(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]),
+                         float(r["Percentage"])))
+    return rows
+
+
+def counter_per_kernel(d: str, counter: str):
+    """kernel short name -> (dispatches, summed counter value)"""
+    acc = defaultdict(lambda: [0, 0.0])
+    with open(one_csv(d, "_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            a = acc[short_name(r["Kernel_Name"])]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"])
+    return acc
+
+
+def match(pattern: str, names):
+    rx = re.compile(pattern)
+    return [n for n in names if rx.search(n)]
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--bench")
+    ap.add_argument("--kernel")
+    ap.add_argument("--copy-bytes", type=float, default=float(1 << 32),
+                    help="bytes read (and written) by one copy_kernel launch")
+    ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--title", default="rocprofv3 summary")
+    ap.add_argument("--top", type=int, default=12)
+    a = ap.parse_args(argv)
+
+    bench = None
+    if a.bench:
+        with open(a.bench) as f:
+            for line in f:
+                line = line.strip()
+                if line.startswith("{") and '"metric"' in line:
+                    bench = json.loads(line)
+    kernel = a.kernel or (bench["roofline"]["kernel"] if bench else "spgemm.numeric.window")
+
+    out = [f"# {a.title}", ""]
+    if a.command:
+        out += ["Commands (from the repository root, one MI355X):", "", "```"] + a.command + ["```", ""]
+
+    ks = kernel_stats(a.trace)
+    total = sum(r[2] for r in ks)
+    out += [f"## Kernel trace (`--kernel-trace --stats`), top {a.top} of {len(ks)} kernels by total time", "",
+            "| kernel | calls | total ms | avg ms | % |", "|---|---:|---:|---:|---:|"]
+    for n, c, t, avg, pct in sorted(ks, key=lambda r: -r[2])[: a.top]:
+        out.append(f"| `{n}` | {c} | {t / 1e6:.3f} | {avg / 1e6:.3f} | {pct:.2f} |")
+    out += ["", f"All kernels: {total / 1e6:.3f} ms.", ""]
+
+    if bench:
+        out += ["## HIP-event timing in the same run (bench.py `kernels`) vs the trace", "",
+                "| timed kernel | launches | HIP-event avg ms | trace avg ms | algorithmic GB/launch |",
+                "|---|---:|---:|---:|---:|"]
+        for tname, v in bench.get("kernels", {}).items():
+            pat = TIMED.get(tname)
+            names = match(pat, [r[0] for r in ks]) if pat else []
+            calls = sum(r[1] for r in ks if r[0] in names)
+            tr = sum(r[2] for r in ks if r[0] in names) / calls / 1e6 if calls else float("nan")
+            ev = v["ms"] / v["launches"] if v["launches"] else float("nan")
+            gb = v.get("bytes_per_launch", 0) / 1e9
+            out.append(f"| {tname} | {v['launches']} | {ev:.3f} | {tr:.3f} | {gb:.3f} |")
+        out += ["", "The trace also covers the warm-up step; estimator.pass spans several kernels (light and heavy "
+                "rows, partial and final sums), so its trace column is a per-kernel average.", ""]
+
+    traffic_line = None
+    if a.fetch and a.write:
+        fe = counter_per_kernel(a.fetch, "FETCH_SIZE")
+        wr = counter_per_kernel(a.write, "WRITE_SIZE")
+        cname = match(TIMED["copy"], fe.keys())
+        if not cname:
+            raise SystemExit("the copy kernel is missing from the FETCH_SIZE run: no calibration")
+        cf, cw = fe[cname[0]], wr[cname[0]]
+        f_scale = a.copy_bytes / (cf[1] / cf[0] * 1024.0)
+        w_scale = a.copy_bytes / (cw[1] / cw[0] * 1024.0)
+        out += ["## HBM traffic (`--pmc FETCH_SIZE`, `--pmc WRITE_SIZE`, KiB per dispatch)", "",
+                f"Calibration with `copy_kernel` ({a.copy_bytes / 2**30:.0f} GiB read + written per launch): "
+                f"FETCH_SIZE reads {cf[1] / cf[0] / 2**20:.3f} GiB -> scale {f_scale:.3f}; "
+                f"WRITE_SIZE reads {cw[1] / cw[0] / 2**20:.3f} GiB -> scale {w_scale:.3f}.", "",
+                "| kernel | dispatches | FETCH GB/dispatch (scaled) | WRITE GB/dispatch (scaled) |",
+                "|---|---:|---:|---:|"]
+        for n in sorted(fe, key=lambda k: -fe[k][1])[: a.top]:
+            d, fv = fe[n]
+            wv = wr.get(n, [d, 0.0])[1]
+            out.append(f"| `{n}` | {d} | {fv / d * 1024 * f_scale / 1e9:.3f} | {wv / d * 1024 * w_scale / 1e9:.3f} |")
+        out.append("")
+        pat = TIMED.get(kernel)
+        names = match(pat, fe.keys()) if pat else []
+        if names:
+            d = sum(fe[n][0] for n in names)
+            fb = sum(fe[n][1] for n in names) * 1024 * f_scale
+            wb = sum(wr[n][1] for n in names if n in wr) * 1024 * w_scale
+            per = (fb + wb) / d
+            alg = None
+            if bench and kernel in bench.get("kernels", {}):
+                v = bench["kernels"][kernel]
+                alg = v.get("bytes_per_launch") or None
+            out += [f"{kernel} ({', '.join('`' + n + '`' for n in names)}): {d} dispatches, "
+                    f"{fb / d / 1e9:.3f} GB fetched + {wb / d / 1e9:.3f} GB written = {per / 1e9:.3f} GB per launch"
+                    + (f"; algorithmic {alg / 1e9:.3f} GB per launch (ratio {per / alg:.2f})." if alg else "."), ""]
+            traffic_line = f"traffic_bytes_per_launch: {kernel} {per:.0f}"
+    if traffic_line:
+        out.append(traffic_line)
+    print("\n".join(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
