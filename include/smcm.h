@@ -122,6 +122,14 @@ int smcm_chain_run_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
 int smcm_chain_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
                     int32_t L, const int32_t* src_rows, int64_t n_src, double* c_table,
                     double* m_table, int32_t* s_table, int32_t* pairs);
+/* Estimated product counts of every split, p_table[(i*L + k)*L + j] for i <= k < j: the cost
+ * of X_ik * X_{k+1,j} (X_ik = M_i..M_k) from column densities of X_ik and the estimator's row
+ * densities of X_{k+1,j}; other entries are 0.  p_table: L*L*L doubles.  No counterpart in the
+ * reference: the option plan_cost = 1 makes the DP minimise these products plus the estimated
+ * intermediate sizes instead of the sizes alone (:519-538); every plan yields the same result. */
+int smcm_chain_products_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
+                                 const int32_t* etypes, int32_t L, const int32_t* src_rows,
+                                 int64_t n_src, double* p_table);
 /* Estimated nnz of the whole chain, c_[0][L-1] (accuracy/res.cpp:494-523, 564-600). */
 int smcm_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
                   int32_t L, const int32_t* src_rows, int64_t n_src, double* est_nnz);

@@ -206,3 +206,59 @@ def set_threads(t: int) -> None:
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+# ---- product-cost estimate (no counterpart in the reference) ----------------------------
+# Restates the library's column-density estimate of product counts with numpy (float64 sums in
+# numpy's order, logarithms unquantised) so the GPU tables can be checked within a tolerance.
+
+def products_estimate(mats: list[CSR], R: list[np.ndarray], coldims) -> np.ndarray:
+    """P[i, k, j] = rows_i * sum_v rho_ik[v] * R[k+1][v, j-k-1] * coldims[j] for i <= k < j."""
+    L = len(mats)
+    P = np.zeros((L, L, L))
+    for i in range(L - 1):
+        rows_i = mats[i].nrows
+        cnt = np.bincount(mats[i].col, minlength=mats[i].ncols).astype(np.float64)
+        rho = np.minimum(1.0, cnt / rows_i) if rows_i > 0 else np.zeros(mats[i].ncols)
+        for k in range(i, L - 1):
+            if k > i:
+                M = mats[k]
+                rowid = np.repeat(np.arange(M.nrows), np.diff(M.ptr))
+                lg = np.maximum(np.log1p(-rho), -64.0)
+                S = np.zeros(M.ncols)
+                np.add.at(S, M.col, lg[rowid])
+                rho = -np.expm1(S)
+            for j in range(k + 1, L):
+                w = R[k + 1][:, j - k - 1] * float(coldims[j])
+                P[i, k, j] = rows_i * float(np.dot(rho, w))
+    return P
+
+
+def dp_products(c: np.ndarray, P: np.ndarray):
+    """cost(i,j) = min_k cost(i,k) + cost(k+1,j) + P[i,k,j] (+ c[i,j] below the root), lowest k on
+    ties; returns (cost table, split table, plan pairs)."""
+    L = c.shape[0]
+    m = np.zeros((L, L))
+    s = np.zeros((L, L), np.int32)
+    for l in range(1, L):
+        for i in range(L - l):
+            j = i + l
+            best = np.inf
+            for k in range(i, j):
+                v = m[i, k] + m[k + 1, j] + P[i, k, j]
+                if v < best:
+                    best, s[i, j] = v, k
+            m[i, j] = best + (c[i, j] if l < L - 1 else 0.0)
+    pairs = []
+
+    def rec(i, j):
+        if i == j:
+            return i
+        k = rec(i, s[i, j])
+        q = rec(s[i, j] + 1, j)
+        pairs.append((k, q))
+        return -1
+
+    if L > 1:
+        rec(0, L - 1)
+    return m, s, np.array(pairs, np.int32).reshape(-1, 2)

@@ -116,3 +116,26 @@ def random_walk_path(rng: np.random.Generator, L: int, schema: str, start: int |
 
 def log_uniform_int(rng: np.random.Generator, lo: int, hi: int) -> int:
     return int(min(hi, max(lo, math.floor(math.exp(rng.uniform(math.log(lo), math.log(hi + 1)))))))
+
+
+@dataclass(frozen=True)
+class ChainQuery:
+    path: MetaPath
+    src_rows: np.ndarray  # ascending local ids of the start type
+
+
+def cfg4_batch(dims, n_chains: int = 4096, seed: int = SEED_BASE + 4, lmin: int = 2, lmax: int = 6,
+               smin: int = 16, smax: int = 65_536) -> list[ChainQuery]:
+    """Workload 4: random bibliographic meta-paths with lmin..lmax matrices, each from a source
+    subset of log-uniform size smin..smax (capped at the start type's size), rows drawn without
+    replacement and sorted."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_chains):
+        L = int(rng.integers(lmin, lmax + 1))
+        mp = random_walk_path(rng, L, "bib")
+        n = log_uniform_int(rng, smin, smax)
+        d0 = int(dims[mp.vertexTypes[0]])
+        rows = np.sort(rng.choice(d0, size=min(n, d0), replace=False)).astype(np.int32)
+        out.append(ChainQuery(mp, rows))
+    return out

@@ -110,17 +110,20 @@ __global__ void __launch_bounds__(kThreads) est_heavy_kernel(const int64_t* __re
 }
 
 // per-block partial sums of r_ij[u] * dim_j in a fixed tree order
+struct DimJ {
+  double d[kMaxW > 0 ? kMaxW : 1];
+};
+
 template <int W>
 __global__ void __launch_bounds__(kThreads) est_partial_kernel(const double* __restrict__ Ri, int32_t nrows,
-                                                               const double* __restrict__ dimj,
-                                                               double* __restrict__ partial) {
+                                                               const DimJ dimj, double* __restrict__ partial) {
   __shared__ double sh[kThreads];
   const int32_t u = blockIdx.x * kThreads + threadIdx.x;
   for (int j = 0; j < W; ++j) {
     double x = 0.0;
     if (u < nrows) {
       double r = Ri[(size_t)u * (W + 1) + 1 + j];
-      x = r * dimj[j];
+      x = r * dimj.d[j];
     }
     sh[threadIdx.x] = x;
     __syncthreads();
@@ -143,17 +146,106 @@ __global__ void est_final_kernel(const double* __restrict__ partial, int32_t nbl
 }
 
 // exact mode: c_j = sequential sum over rows in row order, starting from 0 (one lane per j)
-__global__ void est_exact_kernel(const double* __restrict__ Ri, int32_t nrows, int W, const double* __restrict__ dimj,
+__global__ void est_exact_kernel(const double* __restrict__ Ri, int32_t nrows, int W, const DimJ dimj,
                                  double* __restrict__ c) {
   const int j = threadIdx.x;
   if (j >= W) return;
-  const double d = dimj[j];
+  const double d = dimj.d[j];
   double s = 0.0;
   for (int32_t u = 0; u < nrows; ++u) {
     double x = Ri[(size_t)u * (W + 1) + 1 + j] * d;
     s = s + x;
   }
   c[j] = s;
+}
+
+
+// ---- product-cost estimate ---------------------------------------------------------------
+// Column analogue of the row estimator, for the cost of a product rather than its size.  For
+// the sub-chain X_ik = M_i ... M_k (rows: those of M_i), rho_ik[v] is the estimated fraction of
+// rows of X_ik that have column v:
+//   rho_ii[v] = min(1, colcount_{M_i}[v] / rows_i)
+//   rho_ik[v] = 1 - prod_{w : M_k[w,v] != 0} (1 - rho_{i,k-1}[w])      (independence, as above)
+// and the products of X_ik * X_{k+1,j} are estimated as
+//   P(i,k,j) = rows_i * sum_v rho_ik[v] * r_{k+1,j}[v] * dims[t_{j+1}].
+// For two leaves this is sum_v colcount[v] * min(deg[v], dim), the exact product count when no
+// row holds more entries than the column type has vertices.  The product over w is accumulated
+// as a sum of logarithms in 2^-32 fixed point with integer atomics, so it does not depend on the
+// order of the additions; the dot products use a fixed tree order.  Results are deterministic.
+constexpr double kFx = 4294967296.0;
+constexpr double kLogFloor = -64.0;
+
+__device__ inline int64_t fx_log1m(double rho) {
+  double l = log1p(-rho);
+  if (!(l > kLogFloor)) l = kLogFloor;
+  return (int64_t)llrint(l * kFx);
+}
+
+// one thread per entry of M_k: column v of the entry collects, for every start i < k, the
+// logarithm of (1 - rho_{i,k-1}[w]) of the entry's row w, and the entry count for i = k
+__global__ void __launch_bounds__(kThreads) colest_scatter_kernel(const int64_t* __restrict__ ptr,
+                                                                  const int32_t* __restrict__ col, int32_t nrows,
+                                                                  int64_t nnz, const int64_t* __restrict__ lin, int nin,
+                                                                  int64_t* __restrict__ S) {
+  const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= nnz) return;
+  int32_t lo = 0, hi = nrows;  // row w: ptr[w] <= e < ptr[w+1]
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (ptr[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  const int32_t w = lo;
+  const int64_t v = col[e];
+  unsigned long long* out = (unsigned long long*)(S + v * (nin + 1));
+  for (int i = 0; i < nin; ++i) atomicAdd(out + i, (unsigned long long)lin[(int64_t)w * nin + i]);
+  atomicAdd(out + nin, 1ull);
+}
+
+// blockIdx.y = start i (0..k): rho_ik[v] for this block's columns, the next pass's logarithms,
+// and per-block partial sums of rows_i * rho_ik[v] * r_{k+1,j}[v] * dims[t_{j+1}]
+__global__ void __launch_bounds__(kThreads) colest_finalize_kernel(const int64_t* __restrict__ S, int32_t ncols, int k,
+                                                                   const DimJ nrows_i, const double* __restrict__ Rk1,
+                                                                   int Wj, const DimJ dim_j, int64_t* __restrict__ lout,
+                                                                   double* __restrict__ partial) {
+  __shared__ double sh[kThreads];
+  const int i = blockIdx.y;
+  const int32_t v = blockIdx.x * kThreads + threadIdx.x;
+  const int nk = k + 1;
+  double rho = 0.0;
+  if (v < ncols) {
+    const int64_t x = S[(int64_t)v * nk + i];
+    if (i == k) {
+      const double r = nrows_i.d[i] > 0 ? (double)x / nrows_i.d[i] : 0.0;
+      rho = r > 1 ? 1 : r;
+    } else {
+      rho = -expm1((double)x / kFx);
+    }
+    if (lout) lout[(int64_t)v * nk + i] = fx_log1m(rho);
+  }
+  for (int jj = 0; jj < Wj; ++jj) {
+    double y = 0.0;
+    if (v < ncols) y = nrows_i.d[i] * rho * (Rk1[(int64_t)v * Wj + jj] * dim_j.d[jj]);
+    sh[threadIdx.x] = y;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[((int64_t)i * gridDim.x + blockIdx.x) * Wj + jj] = sh[0];
+    __syncthreads();
+  }
+}
+
+// P[(i*L + k)*L + k+1+jj] = sum of block partials in block order
+__global__ void colest_final_kernel(const double* __restrict__ partial, int32_t nblocks, int nk, int Wj, int k, int L,
+                                    double* __restrict__ P) {
+  const int t = threadIdx.x;
+  if (t >= nk * Wj) return;
+  const int i = t / Wj, jj = t % Wj;
+  double s = 0.0;
+  for (int32_t b = 0; b < nblocks; ++b) s = s + partial[((int64_t)i * nblocks + b) * Wj + jj];
+  P[((int64_t)i * L + k) * L + k + 1 + jj] = s;
 }
 
 template <int W>
@@ -178,20 +270,18 @@ void run_pass(const EstimatorPass& p, DevAlloc& al) {
       SMCM_LAUNCH_CHECK();
     }
     if (p.tm) p.tm->stop(tok, st, bytes);
-    DevBuf<double> dimj(al, W);
-    SMCM_HIP(hipMemcpyAsync(dimj.p, p.dimj, sizeof(double) * W, hipMemcpyHostToDevice, st));
+    DimJ dimj{};
+    for (int j = 0; j < W; ++j) dimj.d[j] = p.dimj[j];
     if (p.exact_sum || n == 0) {
-      est_exact_kernel<<<1, 64, 0, st>>>(p.Ri, n, W, dimj.p, p.c_dev);
+      est_exact_kernel<<<1, 64, 0, st>>>(p.Ri, n, W, dimj, p.c_dev);
       SMCM_LAUNCH_CHECK();
     } else {
       DevBuf<double> partial(al, (int64_t)blocks * W);
-      est_partial_kernel<W><<<blocks, kThreads, 0, st>>>(p.Ri, n, dimj.p, partial.p);
+      est_partial_kernel<W><<<blocks, kThreads, 0, st>>>(p.Ri, n, dimj, partial.p);
       SMCM_LAUNCH_CHECK();
       est_final_kernel<<<1, 64, 0, st>>>(partial.p, (int32_t)blocks, W, p.c_dev);
       SMCM_LAUNCH_CHECK();
     }
-    // the host copy of dimj above must complete before dimj is released
-    SMCM_HIP(hipStreamSynchronize(st));
   } else {
     if (p.tm) p.tm->stop(tok, st, bytes);
   }
@@ -214,6 +304,49 @@ void estimator_pass(const EstimatorPass& p, DevAlloc& al) {
 #undef SMCM_W
     default:
       throw HipError(-22, "estimator: chains longer than 16 matrices are not supported");
+  }
+}
+
+void product_estimate(const ProductEstimate& pe, DevAlloc& al) {
+  const int L = pe.L;
+  if (L < 2) return;
+  if (L - 1 > kMaxW) throw HipError(-22, "product estimate: chains longer than 16 matrices are not supported");
+  hipStream_t st = al.stream;
+  DimJ rows{};
+  for (int i = 0; i < L - 1; ++i) rows.d[i] = (double)pe.mats[i].nrows;
+  DevBuf<int64_t> lin;
+  const int tok = pe.tm ? pe.tm->start("estimator.products", st) : -1;
+  double bytes = 0;
+  for (int k = 0; k <= L - 2; ++k) {
+    const Csr& M = pe.mats[k];
+    const int nk = k + 1;
+    const int32_t nc = M.ncols;
+    const int Wj = L - 1 - k;
+    DevBuf<int64_t> S(al, (int64_t)nc * nk);
+    fill_i64(S.p, (int64_t)nc * nk, 0, st);
+    if (M.nnz > 0) {
+      colest_scatter_kernel<<<(unsigned)ceil_div(M.nnz, kThreads), kThreads, 0, st>>>(M.ptr, M.col, M.nrows, M.nnz,
+                                                                                   lin.p, k, S.p);
+      SMCM_LAUNCH_CHECK();
+    }
+    DevBuf<int64_t> lout;
+    if (k + 1 <= L - 2) lout = DevBuf<int64_t>(al, (int64_t)nc * nk);
+    DimJ dj{};
+    for (int jj = 0; jj < Wj; ++jj) dj.d[jj] = pe.dims[k + 2 + jj];
+    const int32_t nb = (int32_t)ceil_div(nc > 0 ? nc : 1, kThreads);
+    DevBuf<double> partial(al, (int64_t)nk * nb * Wj);
+    colest_finalize_kernel<<<dim3((unsigned)nb, (unsigned)nk), kThreads, 0, st>>>(S.p, nc, k, rows, pe.R[k + 1], Wj, dj,
+                                                                                 lout.p, partial.p);
+    SMCM_LAUNCH_CHECK();
+    colest_final_kernel<<<1, 256, 0, st>>>(partial.p, nb, nk, Wj, k, L, pe.p_dev);
+    SMCM_LAUNCH_CHECK();
+    lin = std::move(lout);
+    bytes += 16.0 * (M.nrows + 1) + (4.0 + 8.0 * k) * (double)M.nnz + 8.0 * nk * nc * (2 + Wj);
+  }
+  if (pe.tm) {
+    pe.tm->stop(tok, st, bytes);
+    SMCM_HIP(hipStreamSynchronize(st));
+    pe.tm->flush();
   }
 }
 

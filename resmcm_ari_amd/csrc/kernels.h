@@ -58,6 +58,19 @@ struct EstimatorPass {
 };
 void estimator_pass(const EstimatorPass& p, DevAlloc& al);
 
+// Estimated product counts of every split: P[(i*L + k)*L + j] for i <= k < j (device table,
+// L*L*L doubles, only those entries written) for X_ik * X_{k+1,j}.  Needs the estimator's
+// row densities of every position (R[q] as written by estimator_pass, q = 1..L-1).
+struct ProductEstimate {
+  const Csr* mats = nullptr;        // the chain's L matrices (M_0 already restricted to the sources)
+  int L = 0;
+  const double* const* R = nullptr; // host array of L device pointers (R[0] unused)
+  const double* dims = nullptr;     // host: dims[t_q] for q = 0..L
+  double* p_dev = nullptr;
+  KernelTimer* tm = nullptr;
+};
+void product_estimate(const ProductEstimate& pe, DevAlloc& al);
+
 // ---- spgemm.hip -------------------------------------------------------------------------
 struct SpgemmOpts {
   int s_max = 64;          // rows with ub <= s_max: one wave per row, register sort

@@ -84,6 +84,7 @@ struct smcm_ctx {
   uint64_t budget = 0;
   smcm::SpgemmOpts opts;
   bool exact_cost_sum = false;
+  int plan_cost = 0;  // 0: the reference DP over estimated sizes; 1: estimated products + sizes
   smcm_stats last{};
   int64_t live = 0;  // graphs and results created from this context and not yet freed
   smcm::KernelTimer timer;
@@ -173,6 +174,8 @@ struct Chain {
   DevBuf<int32_t> d_src;
   std::vector<DevBuf<double>> R;   // R[i]: rows_i x (L-i)
   std::vector<double> c;           // L*L
+  std::vector<double> ptab;        // L*L*L estimated products per split (plan_cost 1)
+  bool want_products = false;
   std::vector<double> mtab;
   std::vector<int32_t> stab, pairs;
   double es_time = 0, dp_time = 0, spgemm_time = 0, total = 0;
@@ -186,6 +189,7 @@ struct Chain {
     if (!vtypes || !etypes) fail(SMCM_EINVAL, "null meta-path");
     if (len < 1 || len > 16) fail(SMCM_EINVAL, "the meta-path is incorrect: L must be in [1, 16]");
     L = len;
+    want_products = ctx->plan_cost == 1;
     vts.assign(vtypes, vtypes + L + 1);
     ets.assign(etypes, etypes + L);
     for (int32_t t : vts)
@@ -238,6 +242,26 @@ struct Chain {
       p.tm = ctx->timer.enabled ? &ctx->timer : nullptr;
       smcm::estimator_pass(p, al);
     }
+    DevBuf<double> pdev;
+    if (want_products && L > 1) {
+      const int64_t n3 = (int64_t)L * L * L;
+      pdev = DevBuf<double>(al, n3);
+      smcm::fill_i64((int64_t*)pdev.p, n3, 0, al.stream);
+      std::vector<const double*> rp(L, nullptr);
+      for (int32_t q = 1; q < L; ++q) rp[q] = R[q].p;
+      std::vector<double> dq(L + 1);
+      for (int32_t q = 0; q <= L; ++q) dq[q] = (double)g->dims[vts[q]];
+      smcm::ProductEstimate pe;
+      pe.mats = mats.data();
+      pe.L = L;
+      pe.R = rp.data();
+      pe.dims = dq.data();
+      pe.p_dev = pdev.p;
+      pe.tm = ctx->timer.enabled ? &ctx->timer : nullptr;
+      smcm::product_estimate(pe, al);
+      ptab.assign((size_t)n3, 0.0);
+      SMCM_HIP(hipMemcpyAsync(ptab.data(), pdev.p, sizeof(double) * n3, hipMemcpyDeviceToHost, al.stream));
+    }
     SMCM_HIP(hipMemcpyAsync(c.data(), cdev.p, sizeof(double) * c.size(), hipMemcpyDeviceToHost, al.stream));
     SMCM_HIP(hipStreamSynchronize(al.stream));
     es_time = now_s() - t0;
@@ -248,6 +272,30 @@ struct Chain {
     const double t0 = now_s();
     mtab.assign((size_t)L * L, 0.0);
     stab.assign((size_t)L * L, 0);
+    if (want_products && !ptab.empty()) {
+      // cost(i,j) = min_k cost(i,k) + cost(k+1,j) + P(i,k,j), plus the estimated size of every
+      // intermediate (not of the root); same evaluation order and lowest-k tie rule as the
+      // reference DP.  Any plan gives the same product (exact integer arithmetic), so this only
+      // changes the amount of work.
+      for (int32_t l = 1; l < L; ++l) {
+        for (int32_t i = 0; i < L - l; ++i) {
+          const int32_t j = i + l;
+          double best = DBL_MAX;
+          for (int32_t k = i; k < j; ++k) {
+            const double cost = mtab[i * L + k] + mtab[(k + 1) * L + j] + ptab[((size_t)i * L + k) * L + j];
+            if (cost < best) {
+              best = cost;
+              stab[i * L + j] = k;
+            }
+          }
+          mtab[i * L + j] = best + (l < L - 1 ? c[i * L + j] : 0.0);
+        }
+      }
+      pairs.clear();
+      if (L > 1) plan_rec(0, L - 1);
+      dp_time = now_s() - t0;
+      return;
+    }
     for (int32_t l = 1; l < L; ++l) {
       for (int32_t i = 0; i < L - l; ++i) {
         const int32_t j = i + l;
@@ -466,6 +514,7 @@ void run_core(Chain& ch, const int32_t* forced_pairs, Csr* C, const smcm::BlockC
     st.n_spgemm = 0;
   } else {
     const double t0 = now_s();
+    if (forced_pairs) ch.want_products = false;
     ch.estimate();
     ch.dp();
     std::vector<int32_t> plan = ch.pairs;
@@ -604,6 +653,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
       o.force_tier = (int)v;
     } else if (k == "exact_cost_sum") {
       ctx->exact_cost_sum = v != 0;
+    } else if (k == "plan_cost") {
+      if (v != 0 && v != 1) fail(SMCM_EINVAL, "plan_cost must be 0 (reference) or 1 (products)");
+      ctx->plan_cost = (int)v;
     } else if (k == "kernel_timing") {
       ctx->timer.enabled = v != 0;
     } else if (k == "lds_bytes") {
@@ -633,6 +685,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "bitmap_vcap") *v = o.bitmap_vcap;
     else if (k == "force_tier") *v = o.force_tier;
     else if (k == "exact_cost_sum") *v = ctx->exact_cost_sum;
+    else if (k == "plan_cost") *v = ctx->plan_cost;
     else if (k == "kernel_timing") *v = ctx->timer.enabled;
     else if (k == "lds_bytes") *v = o.lds_bytes;
     else if (k == "cus") *v = o.cus;
@@ -829,6 +882,21 @@ int smcm_chain_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const i
     if (m_table) std::memcpy(m_table, ch.mtab.data(), sizeof(double) * LL);
     if (s_table) std::memcpy(s_table, ch.stab.data(), sizeof(int32_t) * LL);
     if (pairs && !ch.pairs.empty()) std::memcpy(pairs, ch.pairs.data(), sizeof(int32_t) * ch.pairs.size());
+  });
+}
+
+int smcm_chain_products_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
+                                 int32_t L, const int32_t* src_rows, int64_t n_src, double* p_table) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!p_table) fail(SMCM_EINVAL, "null output");
+    Chain ch{ctx, g};
+    ch.prepare(vtypes, etypes, L, src_rows, n_src);
+    ch.want_products = true;
+    ch.estimate();
+    const size_t n3 = (size_t)L * L * L;
+    if (L > 1) std::memcpy(p_table, ch.ptab.data(), sizeof(double) * n3);
+    else p_table[0] = 0.0;
   });
 }
 

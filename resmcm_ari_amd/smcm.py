@@ -31,7 +31,7 @@ EXPORTS = [
     "smcm_chain_run", "smcm_chain_run_plan", "smcm_chain_plan", "smcm_estimate", "smcm_estimator_rows",
     "smcm_result_nrows", "smcm_result_nnz", "smcm_result_copy", "smcm_result_copy_rows", "smcm_result_row_ids",
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
-    "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth",
+    "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth", "smcm_chain_products_estimate",
 ]
 
 
@@ -76,6 +76,7 @@ def load() -> C.CDLL:
             "smcm_chain_run_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp],
             "smcm_chain_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp, vp],
             "smcm_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
+            "smcm_chain_products_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_estimator_rows": [vp, vp, vp, vp, i32, vp, i64, i32, vp, i64],
             "smcm_result_nrows": [vp],
             "smcm_result_nnz": [vp],
@@ -266,6 +267,16 @@ class Graph:
         x = C.c_double()
         _check(load().smcm_estimate(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen, _ptr(s), ns, C.byref(x)))
         return float(x.value)
+
+    def products_estimate(self, mp: MetaPath, src_rows=None) -> np.ndarray:
+        """Estimated product count of every split: P[i, k, j] for X_ik * X_{k+1,j}."""
+        L = mp.pathLen
+        vt, et = self._path(mp)
+        s, ns = self._src(src_rows)
+        out = np.zeros(max(L, 1) ** 3, np.float64)
+        _check(load().smcm_chain_products_estimate(self.ctx._h, self._h, _ptr(vt), _ptr(et), L, _ptr(s), ns,
+                                                   _ptr(out)))
+        return out.reshape(L, L, L)
 
     def estimator_rows(self, mp: MetaPath, i: int, src_rows=None) -> np.ndarray:
         L = mp.pathLen
