@@ -139,6 +139,27 @@ int smcm_estimator_rows(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
                         const int32_t* etypes, int32_t L, const int32_t* src_rows, int64_t n_src,
                         int32_t i, double* out, int64_t cap);
 
+/* ---- batched form (no counterpart in the reference) ------------------------------------- */
+typedef struct smcm_batch_stats {
+  int32_t chains, small_chains, large_chains, rerun_chains;
+  int32_t groups, pad;
+  int64_t nodes, tiles, products_est;
+  double plan_s, exec_s, large_s, total_s; /* host wall-clock seconds */
+} smcm_batch_stats;
+/* Runs n_chains chains in one call.  Chain i has path_len[i] = L_i matrices; its vertex types
+ * are the next L_i+1 entries of vtypes and its edge types the next L_i entries of etypes.
+ * src_rows == NULL: every chain starts from all vertices of its start type; otherwise chain i
+ * starts from src_rows[src_off[i] .. src_off[i+1]) (global ids).  Outputs (any may be NULL):
+ * nnz[i]; row_digest, the per-row digests (as smcm_result_row_digest) of every result, chain
+ * after chain; results[i], result handles (freed with smcm_result_free).  Plans minimise
+ * estimated products (option plan_cost = 1); chains with few estimated products run as nodes of
+ * one persistent device launch, the others through the per-chain path.  Every result equals
+ * smcm_chain_run's for the same chain bit for bit. */
+int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t* path_len,
+                   const int32_t* vtypes, const int32_t* etypes, const int64_t* src_off,
+                   const int32_t* src_rows, int64_t* nnz, uint64_t* row_digest,
+                   smcm_result** results, smcm_batch_stats* stats);
+
 /* ---- result ----------------------------------------------------------------------------- */
 int32_t smcm_result_nrows(const smcm_result* r);
 int64_t smcm_result_nnz(const smcm_result* r);

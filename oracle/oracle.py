@@ -224,7 +224,8 @@ def products_estimate(mats: list[CSR], R: list[np.ndarray], coldims) -> np.ndarr
             if k > i:
                 M = mats[k]
                 rowid = np.repeat(np.arange(M.nrows), np.diff(M.ptr))
-                lg = np.maximum(np.log1p(-rho), -64.0)
+                with np.errstate(divide="ignore"):
+                    lg = np.maximum(np.log1p(-rho), -64.0)
                 S = np.zeros(M.ncols)
                 np.add.at(S, M.col, lg[rowid])
                 rho = -np.expm1(S)

@@ -29,7 +29,9 @@ HIP_SOURCES = [
     "estimator.hip",
     "spgemm.hip",
     "scan.hip",
+    "batch.hip",
     "smcm_api.cpp",
+    "batch_api.cpp",
 ]
 CLI_SOURCES = ["cli_resmcm.cpp"]
 

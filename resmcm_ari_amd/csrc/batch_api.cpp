@@ -1,0 +1,484 @@
+// batch_api.cpp - smcm_batch_run: many chains in one call.
+//
+// 1. Planning, per chain on the device: the row estimator and the product-cost table
+//    (estimator.hip), all chains enqueued back to back and the tables copied once; then the
+//    product-aware DP on the host.  Every plan yields the same exact product, so the choice only
+//    changes the work.
+// 2. Chains whose estimated products are small go to the persistent executor (batch.hip): all of
+//    their products are nodes of one launch, the subset of the first relation is a row map (no
+//    gather), intermediate rows stay in the executor's arena.  Their results are digested (and,
+//    if asked for, compacted into result handles) straight from the arena.
+// 3. The other chains, and any small chain whose products outgrew the arena, run through the
+//    per-chain tiered path with the same plan.
+#include "smcm.h"
+
+#include "api_internal.h"
+
+using namespace smcm_host;
+
+namespace {
+
+struct BatchChain {
+  int32_t L = 0;
+  const int32_t* vts = nullptr;
+  const int32_t* ets = nullptr;
+  const int32_t* src = nullptr;  // global ids (null = every vertex of the start type)
+  int64_t nsrc = 0;
+  int32_t nrows = 0;             // rows of the result
+  int64_t dig_off = 0;           // first row in the caller's digest array
+  std::vector<int32_t> pairs;
+  double est_products = 0;
+  bool small = false;
+  int32_t root = -1;             // executor node of the result
+  DevBuf<int32_t> rowmap;        // local ids of the source rows (executor row map)
+  std::vector<int32_t> src_local;
+};
+
+__global__ void segment_sum_kernel(const int32_t* len, const int64_t* off, const int32_t* cnt, int64_t* out) {
+  __shared__ int64_t red[256];
+  const int32_t s = blockIdx.x;
+  const int64_t o = off[s];
+  int64_t x = 0;
+  for (int32_t i = threadIdx.x; i < cnt[s]; i += 256) x += len[o + i];
+  red[threadIdx.x] = x;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[s] = red[0];
+}
+
+template <class T>
+T* upload(DevAlloc& al, const std::vector<T>& h) {
+  T* d = al.alloc_n<T>((int64_t)h.size());
+  if (!h.empty()) SMCM_HIP(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, al.stream));
+  return d;
+}
+
+// global ids of the rows of a result (as make_result)
+std::vector<int32_t> result_rows(smcm_graph* g, int32_t t0, const std::vector<int32_t>& src_local, bool whole) {
+  std::vector<int32_t> rows;
+  if (whole) {
+    rows.reserve(g->dims[t0]);
+    for (int32_t v = 0; v < g->n; ++v)
+      if (g->h_vtype[v] == t0) rows.push_back(v);
+    return rows;
+  }
+  std::vector<int32_t> b(g->dims[t0]);
+  for (int32_t v = 0; v < g->n; ++v)
+    if (g->h_vtype[v] == t0) b[g->h_local[v]] = v;
+  rows.resize(src_local.size());
+  for (size_t q = 0; q < src_local.size(); ++q) rows[q] = b[src_local[q]];
+  return rows;
+}
+
+smcm_result* wrap_result(smcm_ctx* ctx, smcm_graph* g, const BatchChain& bc, Csr C) {
+  auto* r = new smcm_result;
+  r->ctx = ctx;
+  r->C = C;
+  ctx->live++;
+  const int32_t tcol = bc.vts[bc.L];
+  const int32_t dcol = g->dims[tcol];
+  r->col_map = ctx->al.alloc_n<int32_t>(dcol);
+  if (dcol)
+    SMCM_HIP(hipMemcpyAsync(r->col_map, g->bins[tcol], sizeof(int32_t) * dcol, hipMemcpyDeviceToDevice,
+                            ctx->al.stream));
+  r->row_ids = result_rows(g, bc.vts[0], bc.src_local, bc.src == nullptr);
+  return r;
+}
+
+struct Executor {
+  smcm_ctx* ctx;
+  smcm_graph* g;
+  std::vector<smcm::BatchNode> nodes;
+  std::vector<std::vector<int32_t>> consumers;
+  std::vector<int64_t> row_off;  // node -> first row in the row arrays
+
+  // post-order plan -> nodes; returns the root node
+  int32_t add_chain(BatchChain& bc) {
+    struct Op {
+      int32_t node;  // -1: leaf
+      int32_t i, j;
+    };
+    std::vector<Op> st;
+    auto leaf = [&](int32_t x) { return Op{-1, x, x}; };
+    auto make = [&](Op a, Op b) {
+      smcm::BatchNode n;
+      const int32_t id = (int32_t)nodes.size();
+      if (a.node >= 0) {
+        n.a_node = a.node;
+        n.nrows = nodes[a.node].nrows;
+        consumers[a.node].push_back(id);
+      } else {
+        const Csr& m = typed(g, bc.vts[a.i], bc.ets[a.i], bc.vts[a.i + 1]);
+        n.a_ptr = m.ptr;
+        n.a_col = m.col;
+        n.a_val = m.val;
+        n.nrows = m.nrows;
+        if (a.i == 0 && bc.src) {
+          n.a_rows = bc.rowmap.p;
+          n.nrows = (int32_t)bc.nsrc;
+        }
+      }
+      if (b.node >= 0) {
+        n.b_node = b.node;
+        consumers[b.node].push_back(id);
+      } else {
+        const Csr& m = typed(g, bc.vts[b.i], bc.ets[b.i], bc.vts[b.i + 1]);
+        n.b_ptr = m.ptr;
+        n.b_col = m.col;
+        n.b_val = m.val;
+      }
+      n.ncols = g->dims[bc.vts[b.j + 1]];
+      n.ninputs = (a.node >= 0) + (b.node >= 0);
+      nodes.push_back(n);
+      consumers.emplace_back();
+      return Op{id, a.i, b.j};
+    };
+    for (size_t q = 0; 2 * q < bc.pairs.size(); ++q) {
+      const int32_t k = bc.pairs[2 * q], l = bc.pairs[2 * q + 1];
+      if (k >= 0 && l >= 0) {
+        st.push_back(make(leaf(k), leaf(l)));
+      } else if (k < 0 && l >= 0) {
+        Op top = st.back();
+        st.pop_back();
+        st.push_back(make(top, leaf(l)));
+      } else if (k >= 0 && l < 0) {
+        Op top = st.back();
+        st.pop_back();
+        st.push_back(make(leaf(k), top));
+      } else {
+        Op r = st.back();
+        st.pop_back();
+        Op lft = st.back();
+        st.pop_back();
+        st.push_back(make(lft, r));
+      }
+    }
+    return st.at(0).node;
+  }
+};
+
+// per-chain tiered path with a given plan; digests / nnz / result as requested
+void run_large(smcm_ctx* ctx, smcm_graph* g, BatchChain& bc, int64_t* nnz, uint64_t* digest, smcm_result** res) {
+  DevAlloc& al = ctx->al;
+  Chain ch{ctx, g};
+  ch.prepare(bc.vts, bc.ets, bc.L, bc.src, bc.src ? bc.nsrc : 0);
+  ch.want_products = false;
+  Csr C;
+  if (bc.L == 1) {
+    run_core(ch, nullptr, &C, nullptr, 0, nullptr);
+  } else {
+    ch.estimate();  // the estimator's densities drive row binning
+    smcm_stats st{};
+    C = ch.execute(bc.pairs, &st);
+  }
+  try {
+    if (digest && C.nrows) {
+      DevBuf<uint64_t> d(al, C.nrows);
+      smcm::row_digest(C, d.p, al.stream);
+      SMCM_HIP(hipMemcpyAsync(digest, d.p, sizeof(uint64_t) * C.nrows, hipMemcpyDeviceToHost, al.stream));
+      SMCM_HIP(hipStreamSynchronize(al.stream));
+    }
+    if (nnz) *nnz = C.nnz;
+    if (res) {
+      *res = wrap_result(ctx, g, bc, C);
+      SMCM_HIP(hipStreamSynchronize(al.stream));
+      return;
+    }
+  } catch (...) {
+    free_csr(al, C);
+    throw;
+  }
+  free_csr(al, C);
+}
+
+}  // namespace
+
+extern "C" {
+
+int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t* path_len, const int32_t* vtypes,
+                   const int32_t* etypes, const int64_t* src_off, const int32_t* src_rows, int64_t* nnz,
+                   uint64_t* row_digest, smcm_result** results, smcm_batch_stats* stats) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!g) fail(SMCM_EINVAL, "null graph");
+    if (n_chains < 0 || (n_chains > 0 && (!path_len || !vtypes || !etypes)))
+      fail(SMCM_EINVAL, "bad chain arguments");
+    if (src_rows && !src_off) fail(SMCM_EINVAL, "src_rows without src_off");
+    DevAlloc& al = ctx->al;
+    hipStream_t st = al.stream;
+    const double t_all = now_s();
+    smcm_batch_stats bs{};
+    bs.chains = n_chains;
+    if (results)
+      for (int32_t i = 0; i < n_chains; ++i) results[i] = nullptr;
+
+    // ---- chains ---------------------------------------------------------------------------
+    std::vector<BatchChain> ch(n_chains);
+    int64_t vo = 0, eo = 0, dig = 0;
+    for (int32_t i = 0; i < n_chains; ++i) {
+      BatchChain& b = ch[i];
+      b.L = path_len[i];
+      if (b.L < 1 || b.L > 16) fail(SMCM_EINVAL, "chain " + std::to_string(i) + ": L must be in [1, 16]");
+      b.vts = vtypes + vo;
+      b.ets = etypes + eo;
+      vo += b.L + 1;
+      eo += b.L;
+      if (src_rows) {
+        if (src_off[i + 1] < src_off[i]) fail(SMCM_EINVAL, "src_off must be non-decreasing");
+        b.src = src_rows + src_off[i];
+        b.nsrc = src_off[i + 1] - src_off[i];
+      }
+      for (int32_t q = 0; q <= b.L; ++q)
+        if (b.vts[q] < 0 || b.vts[q] >= g->ntypes)
+          fail(SMCM_EINVAL, "chain " + std::to_string(i) + ": vertex type not in graph");
+      b.nrows = b.src ? (int32_t)b.nsrc : g->dims[b.vts[0]];
+      b.dig_off = dig;
+      dig += b.nrows;
+    }
+
+    // ---- planning: estimator + product-cost tables, one copy back -------------------------
+    const double t_plan = now_s();
+    std::vector<int64_t> toff(n_chains + 1, 0);
+    for (int32_t i = 0; i < n_chains; ++i) {
+      const int64_t L = ch[i].L;
+      toff[i + 1] = toff[i] + (L > 1 ? L * L + L * L * L : 0);
+    }
+    DevBuf<double> tables(al, std::max<int64_t>(toff[n_chains], 1));
+    for (int32_t i = 0; i < n_chains; ++i) {
+      BatchChain& b = ch[i];
+      Chain c{ctx, g};
+      c.prepare(b.vts, b.ets, b.L, b.src, b.src ? b.nsrc : 0);
+      b.src_local = c.src_local;
+      if (b.src) b.rowmap = std::move(c.d_src);
+      if (b.L > 1) {
+        c.want_products = true;
+        const int64_t LL = (int64_t)b.L * b.L;
+        c.estimate(tables.p + toff[i], tables.p + toff[i] + LL);
+        c.R.clear();
+      }
+    }
+    std::vector<double> htab(std::max<int64_t>(toff[n_chains], 1));
+    if (toff[n_chains])
+      SMCM_HIP(hipMemcpyAsync(htab.data(), tables.p, sizeof(double) * toff[n_chains], hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipStreamSynchronize(st));
+    const int64_t small_max = ctx->batch.small_products;
+    for (int32_t i = 0; i < n_chains; ++i) {
+      BatchChain& b = ch[i];
+      if (b.L == 1) {
+        b.small = false;
+        continue;
+      }
+      Chain c{ctx, g};
+      c.L = b.L;
+      c.want_products = true;
+      const int64_t LL = (int64_t)b.L * b.L;
+      c.c.assign(htab.begin() + toff[i], htab.begin() + toff[i] + LL);
+      c.ptab.assign(htab.begin() + toff[i] + LL, htab.begin() + toff[i] + LL + LL * b.L);
+      c.dp();
+      b.pairs = c.pairs;
+      // estimated products of the chosen plan
+      double tot = 0;
+      std::vector<std::pair<int32_t, int32_t>> stk;
+      for (size_t q = 0; 2 * q < b.pairs.size(); ++q) {
+        const int32_t k = b.pairs[2 * q], l = b.pairs[2 * q + 1];
+        std::pair<int32_t, int32_t> A, B;
+        if (k >= 0 && l >= 0) {
+          A = {k, k};
+          B = {l, l};
+        } else if (k < 0 && l >= 0) {
+          A = stk.back();
+          stk.pop_back();
+          B = {l, l};
+        } else if (k >= 0 && l < 0) {
+          B = stk.back();
+          stk.pop_back();
+          A = {k, k};
+        } else {
+          B = stk.back();
+          stk.pop_back();
+          A = stk.back();
+          stk.pop_back();
+        }
+        tot += c.ptab[((size_t)A.first * b.L + A.second) * b.L + B.second];
+        stk.push_back({A.first, B.second});
+      }
+      b.est_products = tot;
+      b.small = tot <= (double)small_max;
+    }
+    bs.plan_s = now_s() - t_plan;
+
+    // ---- small chains: one persistent launch per arena-sized group ------------------------
+    const double t_exec = now_s();
+    const int64_t arena_cap = ctx->batch.arena_entries;
+    std::vector<int32_t> small_ids, rerun;
+    for (int32_t i = 0; i < n_chains; ++i)
+      if (ch[i].small) small_ids.push_back(i);
+    size_t gi = 0;
+    while (gi < small_ids.size()) {
+      // group: chains whose estimated products (x2 margin) fit the arena
+      std::vector<int32_t> grp;
+      double need = 0;
+      while (gi < small_ids.size()) {
+        const double e = 2.0 * ch[small_ids[gi]].est_products + 1024.0;
+        if (!grp.empty() && need + e > (double)arena_cap) break;
+        need += e;
+        grp.push_back(small_ids[gi++]);
+      }
+      bs.groups++;
+      Executor ex{ctx, g};
+      for (int32_t i : grp) ch[i].root = ex.add_chain(ch[i]);
+      const int32_t nn = (int32_t)ex.nodes.size();
+      const int32_t tr = ctx->batch.tile_rows;
+      int64_t rows_total = 0, tiles = 0;
+      ex.row_off.resize(nn);
+      for (int32_t n = 0; n < nn; ++n) {
+        ex.row_off[n] = rows_total;
+        rows_total += ex.nodes[n].nrows;
+        ex.nodes[n].ntiles = std::max<int32_t>(1, (int32_t)smcm::ceil_div(ex.nodes[n].nrows, tr));
+        tiles += ex.nodes[n].ntiles;
+      }
+      if (tiles > INT32_MAX) fail(SMCM_EINVAL, "batch too large");
+      DevBuf<int64_t> ostart(al, std::max<int64_t>(rows_total, 1));
+      DevBuf<int32_t> olen(al, std::max<int64_t>(rows_total, 1));
+      for (int32_t n = 0; n < nn; ++n) {
+        ex.nodes[n].out_start = ostart.p + ex.row_off[n];
+        ex.nodes[n].out_len = olen.p + ex.row_off[n];
+      }
+      std::vector<int32_t> poff(nn + 1, 0), par;
+      for (int32_t n = 0; n < nn; ++n) {
+        poff[n + 1] = poff[n] + (int32_t)ex.consumers[n].size();
+        par.insert(par.end(), ex.consumers[n].begin(), ex.consumers[n].end());
+      }
+      // initial tiles: nodes over base relations only, largest first
+      std::vector<int32_t> first;
+      for (int32_t n = 0; n < nn; ++n)
+        if (ex.nodes[n].ninputs == 0) first.push_back(n);
+      std::stable_sort(first.begin(), first.end(),
+                       [&](int32_t a, int32_t b) { return ex.nodes[a].nrows > ex.nodes[b].nrows; });
+      std::vector<unsigned long long> q0;
+      for (int32_t n : first)
+        for (int32_t t = 0; t < ex.nodes[n].ntiles; ++t) q0.push_back((((unsigned long long)n << 32) | (unsigned)t) + 1);
+      DevBuf<smcm::BatchNode> dnodes(al, nn);
+      SMCM_HIP(hipMemcpyAsync(dnodes.p, ex.nodes.data(), sizeof(smcm::BatchNode) * nn, hipMemcpyHostToDevice, st));
+      DevBuf<int32_t> dpoff(al, nn + 1), dpar(al, std::max<int64_t>((int64_t)par.size(), 1));
+      SMCM_HIP(hipMemcpyAsync(dpoff.p, poff.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice, st));
+      if (!par.empty())
+        SMCM_HIP(hipMemcpyAsync(dpar.p, par.data(), sizeof(int32_t) * par.size(), hipMemcpyHostToDevice, st));
+      DevBuf<int32_t> counters(al, 3 * (int64_t)nn);
+      smcm::fill_i32(counters.p, 3 * (int64_t)nn, 0, st);
+      DevBuf<unsigned long long> queue(al, tiles);
+      smcm::fill_i64((int64_t*)queue.p, tiles, 0, st);
+      if (!q0.empty())
+        SMCM_HIP(hipMemcpyAsync(queue.p, q0.data(), sizeof(unsigned long long) * q0.size(), hipMemcpyHostToDevice, st));
+      smcm::BatchState hst{};
+      hst.tail = (int32_t)q0.size();
+      DevBuf<smcm::BatchState> dstate(al, 1);
+      SMCM_HIP(hipMemcpyAsync(dstate.p, &hst, sizeof(hst), hipMemcpyHostToDevice, st));
+      const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(arena_cap, (int64_t)need));
+      DevBuf<int32_t> acol(al, cap);
+      DevBuf<uint64_t> aval(al, cap);
+
+      smcm::BatchLaunch bl;
+      bl.nodes = dnodes.p;
+      bl.parent_off = dpoff.p;
+      bl.parents = dpar.p;
+      bl.done_tiles = counters.p;
+      bl.ready = counters.p + nn;
+      bl.failed = counters.p + 2 * nn;
+      bl.queue = queue.p;
+      bl.state = dstate.p;
+      bl.arena_col = acol.p;
+      bl.arena_val = aval.p;
+      bl.arena_cap = (uint64_t)cap;
+      bl.spin_limit = 1000000000LL;  // 10 s at 100 MHz
+      bl.total_tiles = (int32_t)tiles;
+      bl.tile_rows = tr;
+      bl.threads = ctx->batch.threads;
+      bl.sort_cap = ctx->batch.sort_cap;
+      bl.log2w = ctx->batch.log2w;
+      const size_t lds = smcm::batch_lds_bytes(bl.threads, bl.sort_cap, bl.log2w);
+      if ((int64_t)lds > ctx->opts.lds_bytes) fail(SMCM_EINVAL, "batch: LDS shape exceeds the device limit");
+      const int64_t per_cu = std::max<int64_t>(
+          1, std::min<int64_t>(ctx->opts.lds_bytes / (int64_t)(lds + 2048), (2048 / bl.threads)));
+      bl.grid = (int)(ctx->opts.cus * per_cu);
+      smcm::KernelTimer* tm = ctx->timer.enabled ? &ctx->timer : nullptr;
+      const int tok = tm ? tm->start("batch.executor", st) : -1;
+      smcm::batch_execute(bl, st);
+      if (tm) tm->stop(tok, st, 0.0);
+
+      // results of the group: nnz per chain, digests, result handles
+      const int32_t ng = (int32_t)grp.size();
+      std::vector<int64_t> soff(ng);
+      std::vector<int32_t> scnt(ng);
+      for (int32_t q = 0; q < ng; ++q) {
+        const int32_t r = ch[grp[q]].root;
+        soff[q] = ex.row_off[r];
+        scnt[q] = ex.nodes[r].nrows;
+      }
+      int64_t* dsoff = upload(al, soff);
+      int32_t* dscnt = upload(al, scnt);
+      DevBuf<int64_t> dsum(al, ng);
+      segment_sum_kernel<<<(unsigned)ng, 256, 0, st>>>(olen.p, dsoff, dscnt, dsum.p);
+      SMCM_LAUNCH_CHECK();
+      std::vector<int64_t> sums(ng);
+      SMCM_HIP(hipMemcpyAsync(sums.data(), dsum.p, sizeof(int64_t) * ng, hipMemcpyDeviceToHost, st));
+      std::vector<int32_t> hfail(nn);
+      SMCM_HIP(hipMemcpyAsync(hfail.data(), bl.failed, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, st));
+      SMCM_HIP(hipMemcpyAsync(&hst, dstate.p, sizeof(hst), hipMemcpyDeviceToHost, st));
+      SMCM_HIP(hipStreamSynchronize(st));
+      al.free(dsoff);
+      al.free(dscnt);
+      if (hst.error & 2) fail(SMCM_EDEVICE, "batch executor: a queue wait timed out");
+      bs.nodes += nn;
+      bs.tiles += tiles;
+      DevBuf<uint64_t> ddig;
+      if (row_digest) ddig = DevBuf<uint64_t>(al, std::max<int64_t>(rows_total, 1));
+      for (int32_t q = 0; q < ng; ++q) {
+        BatchChain& b = ch[grp[q]];
+        const smcm::BatchNode& R = ex.nodes[b.root];
+        if (hfail[b.root]) {
+          rerun.push_back(grp[q]);
+          continue;
+        }
+        bs.small_chains++;
+        if (nnz) nnz[grp[q]] = sums[q];
+        if (row_digest && R.nrows) {
+          uint64_t* d = ddig.p + ex.row_off[b.root];
+          smcm::padded_digest(R.out_start, R.out_len, acol.p, aval.p, nullptr, R.nrows, d, st);
+          SMCM_HIP(hipMemcpyAsync(row_digest + b.dig_off, d, sizeof(uint64_t) * R.nrows, hipMemcpyDeviceToHost, st));
+        }
+        if (results) {
+          Csr C;
+          smcm::padded_to_csr(R.out_start, R.out_len, acol.p, aval.p, R.nrows, R.ncols, C, al);
+          results[grp[q]] = wrap_result(ctx, g, b, C);
+        }
+      }
+      SMCM_HIP(hipStreamSynchronize(st));
+      if (tm) tm->flush();
+    }
+    bs.exec_s = now_s() - t_exec;
+
+    // ---- large chains (and reruns) through the per-chain path -----------------------------
+    const double t_large = now_s();
+    std::vector<int32_t> large;
+    for (int32_t i = 0; i < n_chains; ++i)
+      if (!ch[i].small) large.push_back(i);
+    large.insert(large.end(), rerun.begin(), rerun.end());
+    bs.rerun_chains = (int32_t)rerun.size();
+    for (int32_t i : large) {
+      BatchChain& b = ch[i];
+      run_large(ctx, g, b, nnz ? nnz + i : nullptr, row_digest ? row_digest + b.dig_off : nullptr,
+                results ? results + i : nullptr);
+      bs.large_chains++;
+    }
+    bs.large_s = now_s() - t_large;
+    for (int32_t i = 0; i < n_chains; ++i) bs.products_est += (int64_t)ch[i].est_products;
+    bs.total_s = now_s() - t_all;
+    if (stats) *stats = bs;
+  });
+}
+
+}  // extern "C"

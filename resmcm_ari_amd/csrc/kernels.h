@@ -71,6 +71,58 @@ struct ProductEstimate {
 };
 void product_estimate(const ProductEstimate& pe, DevAlloc& al);
 
+// ---- batch.hip --------------------------------------------------------------------------
+// One product of a batch: A (a base CSR, optionally through a row map, or node a_node's output)
+// times B (a base CSR or node b_node's output).  Node outputs are rows (start, length) in the
+// batch's output arena.
+struct BatchNode {
+  const int64_t* a_ptr = nullptr;
+  const int32_t* a_rows = nullptr;
+  const int32_t* a_col = nullptr;
+  const uint64_t* a_val = nullptr;
+  const int64_t* b_ptr = nullptr;
+  const int32_t* b_col = nullptr;
+  const uint64_t* b_val = nullptr;
+  int64_t* out_start = nullptr;
+  int32_t* out_len = nullptr;
+  int32_t a_node = -1, b_node = -1;
+  int32_t nrows = 0, ncols = 0;
+  int32_t ntiles = 0;
+  int32_t ninputs = 0;  // operands that are node outputs
+};
+struct BatchState {
+  int32_t head = 0, tail = 0;  // next queue slot to claim / to fill
+  int32_t error = 0, pad = 0;  // bit 0: output arena exhausted; bit 1: a queue wait gave up
+  unsigned long long arena_used = 0;
+};
+struct BatchLaunch {
+  const BatchNode* nodes = nullptr;
+  const int32_t* parent_off = nullptr;  // consumers of node n: parents[parent_off[n] .. parent_off[n+1])
+  const int32_t* parents = nullptr;
+  int32_t* done_tiles = nullptr;        // per node, zeroed
+  int32_t* ready = nullptr;             // per node: completed node operands, zeroed
+  int32_t* failed = nullptr;            // per node, zeroed
+  unsigned long long* queue = nullptr;  // total_tiles slots: 0 = not yet filled, else ((node << 32) | tile) + 1
+  BatchState* state = nullptr;
+  int32_t* arena_col = nullptr;
+  uint64_t* arena_val = nullptr;
+  uint64_t arena_cap = 0;               // entries
+  int64_t spin_limit = 0;               // wall-clock ticks (100 MHz) a queue wait may last
+  int32_t total_tiles = 0;
+  int32_t tile_rows = 32;
+  int threads = 512, sort_cap = 1024, log2w = 13;
+  int grid = 0;
+};
+size_t batch_lds_bytes(int threads, int sort_cap, int log2w);
+bool batch_shape_supported(int threads, int sort_cap);
+void batch_execute(const BatchLaunch& bl, hipStream_t st);
+// digests (as row_digest) of rows given as (start, length) over col/val
+void padded_digest(const int64_t* start, const int32_t* len, const int32_t* col, const uint64_t* val,
+                   const int32_t* colmap, int32_t nrows, uint64_t* out, hipStream_t st);
+// rows given as (start, length) -> a compact CSR (allocated from al)
+void padded_to_csr(const int64_t* start, const int32_t* len, const int32_t* col, const uint64_t* val, int32_t nrows,
+                   int32_t ncols, Csr& out, DevAlloc& al);
+
 // ---- spgemm.hip -------------------------------------------------------------------------
 struct SpgemmOpts {
   int s_max = 64;          // rows with ub <= s_max: one wave per row, register sort

@@ -32,6 +32,7 @@ EXPORTS = [
     "smcm_result_nrows", "smcm_result_nnz", "smcm_result_copy", "smcm_result_copy_rows", "smcm_result_row_ids",
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
     "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth", "smcm_chain_products_estimate",
+    "smcm_batch_run",
 ]
 
 
@@ -43,6 +44,13 @@ class Stats(C.Structure):
     _fields_ = [("products", C.c_int64), ("nnz", C.c_int64), ("rows_tier", C.c_int64 * 4),
                 ("n_spgemm", C.c_int32), ("pairs", C.c_int32 * 30), ("c_root", C.c_double),
                 ("ref_nnz", C.c_int64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("chains", C.c_int32), ("small_chains", C.c_int32), ("large_chains", C.c_int32),
+                ("rerun_chains", C.c_int32), ("groups", C.c_int32), ("pad", C.c_int32), ("nodes", C.c_int64),
+                ("tiles", C.c_int64), ("products_est", C.c_int64), ("plan_s", C.c_double), ("exec_s", C.c_double),
+                ("large_s", C.c_double), ("total_s", C.c_double)]
 
 
 class SmcmError(RuntimeError):
@@ -77,6 +85,7 @@ def load() -> C.CDLL:
             "smcm_chain_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp, vp],
             "smcm_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_chain_products_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
+            "smcm_batch_run": [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "smcm_estimator_rows": [vp, vp, vp, vp, i32, vp, i64, i32, vp, i64],
             "smcm_result_nrows": [vp],
             "smcm_result_nnz": [vp],
@@ -277,6 +286,40 @@ class Graph:
         _check(load().smcm_chain_products_estimate(self.ctx._h, self._h, _ptr(vt), _ptr(et), L, _ptr(s), ns,
                                                    _ptr(out)))
         return out.reshape(L, L, L)
+
+    def run_batch(self, paths, src_rows=None, digests: bool = True, results: bool = False):
+        """Batched form: paths[i] (MetaPath) from src_rows[i] (global ids; None = all rows of the
+        start type for every chain).  Returns (nnz per chain, per-chain digest arrays or None,
+        result handles or None, stats dict)."""
+        n = len(paths)
+        plen = np.array([mp.pathLen for mp in paths], np.int32)
+        vts = np.concatenate([np.asarray(mp.vertexTypes, np.int32) for mp in paths]) if n else np.zeros(1, np.int32)
+        ets = np.concatenate([np.asarray(mp.edgeTypes, np.int32) for mp in paths]) if n else np.zeros(1, np.int32)
+        if src_rows is None:
+            soff, srows = None, None
+            nrows = [int(self.hg.dimensions[mp.vertexTypes[0]]) for mp in paths]
+        else:
+            parts = [np.ascontiguousarray(r, np.int32) for r in src_rows]
+            nrows = [len(r) for r in parts]
+            soff = np.zeros(n + 1, np.int64)
+            soff[1:] = np.cumsum(nrows)
+            srows = np.concatenate(parts) if parts and soff[-1] > 0 else np.zeros(1, np.int32)
+        nnz = np.zeros(max(n, 1), np.int64)
+        dig = np.zeros(max(int(sum(nrows)), 1), np.uint64) if digests else None
+        hs = (C.c_void_p * max(n, 1))() if results else None
+        st = BatchStats()
+        _check(load().smcm_batch_run(self.ctx._h, self._h, n, _ptr(plen), _ptr(vts), _ptr(ets),
+                                     None if soff is None else _ptr(soff), None if srows is None else _ptr(srows),
+                                     _ptr(nnz), None if dig is None else _ptr(dig), hs, C.byref(st)))
+        out_dig = None
+        if digests:
+            out_dig, o = [], 0
+            for r in nrows:
+                out_dig.append(dig[o:o + r])
+                o += r
+        res = [Result(C.c_void_p(hs[i]), self.ctx) for i in range(n)] if results else None
+        stats = {k: getattr(st, k) for k, _ in BatchStats._fields_ if k != "pad"}
+        return nnz[:n], out_dig, res, stats
 
     def estimator_rows(self, mp: MetaPath, i: int, src_rows=None) -> np.ndarray:
         L = mp.pathLen

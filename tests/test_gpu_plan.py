@@ -32,7 +32,7 @@ def product_plans(ctx):
 def cases():
     for p in BIB:
         yield "bib", p, None
-        yield "bib", p, [1, 4, 9]
+        yield "bib", p, [1, 4, 5]
     for p in LOC:
         yield "loc", p, None
         yield "loc", p, [0, 2, 3, 7]
