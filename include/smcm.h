@@ -141,8 +141,8 @@ int smcm_estimator_rows(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
 
 /* ---- batched form (no counterpart in the reference) ------------------------------------- */
 typedef struct smcm_batch_stats {
-  int32_t chains, small_chains, large_chains, rerun_chains;
-  int32_t groups, pad;
+  int32_t chains, jobs;                    /* jobs: distinct meta-paths */
+  int32_t small_jobs, large_jobs, rerun_jobs, groups;
   int64_t nodes, tiles, products_est;
   double plan_s, exec_s, large_s, total_s; /* host wall-clock seconds */
 } smcm_batch_stats;
@@ -151,10 +151,12 @@ typedef struct smcm_batch_stats {
  * src_rows == NULL: every chain starts from all vertices of its start type; otherwise chain i
  * starts from src_rows[src_off[i] .. src_off[i+1]) (global ids).  Outputs (any may be NULL):
  * nnz[i]; row_digest, the per-row digests (as smcm_result_row_digest) of every result, chain
- * after chain; results[i], result handles (freed with smcm_result_free).  Plans minimise
- * estimated products (option plan_cost = 1); chains with few estimated products run as nodes of
- * one persistent device launch, the others through the per-chain path.  Every result equals
- * smcm_chain_run's for the same chain bit for bit. */
+ * after chain; results[i], result handles (freed with smcm_result_free).  Chains with the same
+ * meta-path run as one job over the union of their source rows (each chain's result is the
+ * job's rows at its sources).  Plans minimise estimated products (as option plan_cost = 1);
+ * jobs with few estimated products run as nodes of one persistent device launch, the others
+ * through the per-chain path.  Every result equals smcm_chain_run's for the same chain bit for
+ * bit. */
 int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t* path_len,
                    const int32_t* vtypes, const int32_t* etypes, const int64_t* src_off,
                    const int32_t* src_rows, int64_t* nnz, uint64_t* row_digest,

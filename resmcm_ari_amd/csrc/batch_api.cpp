@@ -1,6 +1,9 @@
 // batch_api.cpp - smcm_batch_run: many chains in one call.
 //
-// 1. Planning, per chain on the device: the row estimator and the product-cost table
+// 0. Chains with the same meta-path become one job over the union of their source rows: row u
+//    of a chain's result depends only on row u of its first relation, so each chain's result is
+//    the job result's rows at its sources (digests and counts are picked, handles gathered).
+// 1. Planning, per job on the device: the row estimator and the product-cost table
 //    (estimator.hip), all chains enqueued back to back and the tables copied once; then the
 //    product-aware DP on the host.  Every plan yields the same exact product, so the choice only
 //    changes the work.
@@ -32,6 +35,20 @@ struct BatchChain {
   int32_t root = -1;             // executor node of the result
   DevBuf<int32_t> rowmap;        // local ids of the source rows (executor row map)
   std::vector<int32_t> src_local;
+  std::vector<int32_t> src_global;  // the job's rows (owned here; src points into it)
+  // outputs per row of the job: digest, length; the result itself when handles are wanted
+  std::vector<uint64_t> h_dig;
+  std::vector<int32_t> h_len;
+  Csr C;
+  bool have_C = false;
+};
+
+// one chain of the call: its job and its rows' positions in the job's rows
+struct Member {
+  int32_t job = -1;
+  bool whole = false;
+  std::vector<int32_t> local;  // local ids of its source rows
+  int64_t dig_off = 0;
 };
 
 __global__ void segment_sum_kernel(const int32_t* len, const int64_t* off, const int32_t* cnt, int64_t* out) {
@@ -73,18 +90,19 @@ std::vector<int32_t> result_rows(smcm_graph* g, int32_t t0, const std::vector<in
   return rows;
 }
 
-smcm_result* wrap_result(smcm_ctx* ctx, smcm_graph* g, const BatchChain& bc, Csr C) {
+smcm_result* wrap_result(smcm_ctx* ctx, smcm_graph* g, const int32_t* vts, int32_t L,
+                         const std::vector<int32_t>& src_local, bool whole, Csr C) {
   auto* r = new smcm_result;
   r->ctx = ctx;
   r->C = C;
   ctx->live++;
-  const int32_t tcol = bc.vts[bc.L];
+  const int32_t tcol = vts[L];
   const int32_t dcol = g->dims[tcol];
   r->col_map = ctx->al.alloc_n<int32_t>(dcol);
   if (dcol)
     SMCM_HIP(hipMemcpyAsync(r->col_map, g->bins[tcol], sizeof(int32_t) * dcol, hipMemcpyDeviceToDevice,
                             ctx->al.stream));
-  r->row_ids = result_rows(g, bc.vts[0], bc.src_local, bc.src == nullptr);
+  r->row_ids = result_rows(g, vts[0], src_local, whole);
   return r;
 }
 
@@ -160,8 +178,9 @@ struct Executor {
   }
 };
 
-// per-chain tiered path with a given plan; digests / nnz / result as requested
-void run_large(smcm_ctx* ctx, smcm_graph* g, BatchChain& bc, int64_t* nnz, uint64_t* digest, smcm_result** res) {
+// per-chain tiered path with the job's plan: row digests and lengths, and the result itself
+// when keep is set
+void run_large(smcm_ctx* ctx, smcm_graph* g, BatchChain& bc, bool keep) {
   DevAlloc& al = ctx->al;
   Chain ch{ctx, g};
   ch.prepare(bc.vts, bc.ets, bc.L, bc.src, bc.src ? bc.nsrc : 0);
@@ -175,23 +194,27 @@ void run_large(smcm_ctx* ctx, smcm_graph* g, BatchChain& bc, int64_t* nnz, uint6
     C = ch.execute(bc.pairs, &st);
   }
   try {
-    if (digest && C.nrows) {
+    bc.h_dig.assign(C.nrows, 0);
+    bc.h_len.assign(C.nrows, 0);
+    if (C.nrows) {
       DevBuf<uint64_t> d(al, C.nrows);
       smcm::row_digest(C, d.p, al.stream);
-      SMCM_HIP(hipMemcpyAsync(digest, d.p, sizeof(uint64_t) * C.nrows, hipMemcpyDeviceToHost, al.stream));
+      std::vector<int64_t> ptr(C.nrows + 1);
+      SMCM_HIP(hipMemcpyAsync(bc.h_dig.data(), d.p, sizeof(uint64_t) * C.nrows, hipMemcpyDeviceToHost, al.stream));
+      SMCM_HIP(hipMemcpyAsync(ptr.data(), C.ptr, sizeof(int64_t) * ptr.size(), hipMemcpyDeviceToHost, al.stream));
       SMCM_HIP(hipStreamSynchronize(al.stream));
-    }
-    if (nnz) *nnz = C.nnz;
-    if (res) {
-      *res = wrap_result(ctx, g, bc, C);
-      SMCM_HIP(hipStreamSynchronize(al.stream));
-      return;
+      for (int32_t u = 0; u < C.nrows; ++u) bc.h_len[u] = (int32_t)(ptr[u + 1] - ptr[u]);
     }
   } catch (...) {
     free_csr(al, C);
     throw;
   }
-  free_csr(al, C);
+  if (keep) {
+    bc.C = C;
+    bc.have_C = true;
+  } else {
+    free_csr(al, C);
+  }
 }
 
 }  // namespace
@@ -215,39 +238,96 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
     if (results)
       for (int32_t i = 0; i < n_chains; ++i) results[i] = nullptr;
 
-    // ---- chains ---------------------------------------------------------------------------
-    std::vector<BatchChain> ch(n_chains);
+    // ---- chains -> jobs --------------------------------------------------------------------
+    std::vector<std::vector<int32_t>> global_of(g->ntypes);  // local id -> global id, per type
+    for (int32_t t = 0; t < g->ntypes; ++t) global_of[t].resize(g->dims[t]);
+    for (int32_t v = 0; v < g->n; ++v) global_of[g->h_vtype[v]][g->h_local[v]] = v;
+    std::vector<Member> mem(n_chains);
+    std::vector<BatchChain> ch;  // jobs
+    std::map<std::vector<int32_t>, int32_t> job_of;
+    std::vector<std::vector<int32_t>> job_rows;  // local ids, unsorted until the end
+    std::vector<bool> job_whole;
     int64_t vo = 0, eo = 0, dig = 0;
     for (int32_t i = 0; i < n_chains; ++i) {
-      BatchChain& b = ch[i];
-      b.L = path_len[i];
-      if (b.L < 1 || b.L > 16) fail(SMCM_EINVAL, "chain " + std::to_string(i) + ": L must be in [1, 16]");
-      b.vts = vtypes + vo;
-      b.ets = etypes + eo;
-      vo += b.L + 1;
-      eo += b.L;
-      if (src_rows) {
-        if (src_off[i + 1] < src_off[i]) fail(SMCM_EINVAL, "src_off must be non-decreasing");
-        b.src = src_rows + src_off[i];
-        b.nsrc = src_off[i + 1] - src_off[i];
-      }
-      for (int32_t q = 0; q <= b.L; ++q)
-        if (b.vts[q] < 0 || b.vts[q] >= g->ntypes)
+      const int32_t L = path_len[i];
+      if (L < 1 || L > 16) fail(SMCM_EINVAL, "chain " + std::to_string(i) + ": L must be in [1, 16]");
+      const int32_t* vts = vtypes + vo;
+      const int32_t* ets = etypes + eo;
+      vo += L + 1;
+      eo += L;
+      for (int32_t q = 0; q <= L; ++q)
+        if (vts[q] < 0 || vts[q] >= g->ntypes)
           fail(SMCM_EINVAL, "chain " + std::to_string(i) + ": vertex type not in graph");
-      b.nrows = b.src ? (int32_t)b.nsrc : g->dims[b.vts[0]];
-      b.dig_off = dig;
-      dig += b.nrows;
+      for (int32_t q = 0; q < L; ++q)
+        if (ets[q] < 0) fail(SMCM_EINVAL, "chain " + std::to_string(i) + ": negative edge type");
+      Member& m = mem[i];
+      m.whole = src_rows == nullptr;
+      if (!m.whole) {
+        if (src_off[i + 1] < src_off[i]) fail(SMCM_EINVAL, "src_off must be non-decreasing");
+        m.local.resize(src_off[i + 1] - src_off[i]);
+        for (int64_t r = src_off[i]; r < src_off[i + 1]; ++r) {
+          const int32_t v = src_rows[r];
+          if (v < 0 || v >= g->n || g->h_vtype[v] != vts[0])
+            fail(SMCM_EINVAL, "chain " + std::to_string(i) + ": source row " + std::to_string(v) +
+                                  " is not a vertex of the start type");
+          m.local[r - src_off[i]] = g->h_local[v];
+        }
+      }
+      std::vector<int32_t> key;
+      key.push_back(L);
+      key.insert(key.end(), vts, vts + L + 1);
+      key.insert(key.end(), ets, ets + L);
+      auto it = job_of.find(key);
+      int32_t j;
+      if (it == job_of.end()) {
+        j = (int32_t)ch.size();
+        job_of.emplace(key, j);
+        ch.emplace_back();
+        ch[j].L = L;
+        ch[j].vts = vts;
+        ch[j].ets = ets;
+        job_rows.emplace_back();
+        job_whole.push_back(false);
+      } else {
+        j = it->second;
+      }
+      m.job = j;
+      if (m.whole) job_whole[j] = true;
+      else job_rows[j].insert(job_rows[j].end(), m.local.begin(), m.local.end());
+      m.dig_off = dig;
+      dig += m.whole ? g->dims[vts[0]] : (int64_t)m.local.size();
     }
+    const int32_t n_jobs = (int32_t)ch.size();
+    for (int32_t j = 0; j < n_jobs; ++j) {
+      BatchChain& b = ch[j];
+      std::vector<int32_t>& r = job_rows[j];
+      std::sort(r.begin(), r.end());
+      r.erase(std::unique(r.begin(), r.end()), r.end());
+      if (job_whole[j] || (int64_t)r.size() == g->dims[b.vts[0]]) {
+        b.src = nullptr;
+        b.nsrc = 0;
+        b.nrows = g->dims[b.vts[0]];
+        r.clear();
+        job_whole[j] = true;
+      } else {
+        b.src_global.resize(r.size());
+        for (size_t q = 0; q < r.size(); ++q) b.src_global[q] = global_of[b.vts[0]][r[q]];
+        b.src = b.src_global.data();
+        b.nsrc = (int64_t)r.size();
+        b.nrows = (int32_t)r.size();
+      }
+    }
+    bs.groups = 0;
 
     // ---- planning: estimator + product-cost tables, one copy back -------------------------
     const double t_plan = now_s();
-    std::vector<int64_t> toff(n_chains + 1, 0);
-    for (int32_t i = 0; i < n_chains; ++i) {
+    std::vector<int64_t> toff(n_jobs + 1, 0);
+    for (int32_t i = 0; i < n_jobs; ++i) {
       const int64_t L = ch[i].L;
       toff[i + 1] = toff[i] + (L > 1 ? L * L + L * L * L : 0);
     }
-    DevBuf<double> tables(al, std::max<int64_t>(toff[n_chains], 1));
-    for (int32_t i = 0; i < n_chains; ++i) {
+    DevBuf<double> tables(al, std::max<int64_t>(toff[n_jobs], 1));
+    for (int32_t i = 0; i < n_jobs; ++i) {
       BatchChain& b = ch[i];
       Chain c{ctx, g};
       c.prepare(b.vts, b.ets, b.L, b.src, b.src ? b.nsrc : 0);
@@ -260,12 +340,12 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
         c.R.clear();
       }
     }
-    std::vector<double> htab(std::max<int64_t>(toff[n_chains], 1));
-    if (toff[n_chains])
-      SMCM_HIP(hipMemcpyAsync(htab.data(), tables.p, sizeof(double) * toff[n_chains], hipMemcpyDeviceToHost, st));
+    std::vector<double> htab(std::max<int64_t>(toff[n_jobs], 1));
+    if (toff[n_jobs])
+      SMCM_HIP(hipMemcpyAsync(htab.data(), tables.p, sizeof(double) * toff[n_jobs], hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipStreamSynchronize(st));
     const int64_t small_max = ctx->batch.small_products;
-    for (int32_t i = 0; i < n_chains; ++i) {
+    for (int32_t i = 0; i < n_jobs; ++i) {
       BatchChain& b = ch[i];
       if (b.L == 1) {
         b.small = false;
@@ -310,15 +390,16 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
     }
     bs.plan_s = now_s() - t_plan;
 
-    // ---- small chains: one persistent launch per arena-sized group ------------------------
+    // ---- small jobs: one persistent launch per arena-sized group ----------------------------
     const double t_exec = now_s();
     const int64_t arena_cap = ctx->batch.arena_entries;
+    const bool keep = results != nullptr;
     std::vector<int32_t> small_ids, rerun;
-    for (int32_t i = 0; i < n_chains; ++i)
+    for (int32_t i = 0; i < n_jobs; ++i)
       if (ch[i].small) small_ids.push_back(i);
     size_t gi = 0;
     while (gi < small_ids.size()) {
-      // group: chains whose estimated products (x2 margin) fit the arena
+      // group: jobs whose estimated products (x2 margin) fit the arena
       std::vector<int32_t> grp;
       double need = 0;
       while (gi < small_ids.size()) {
@@ -409,51 +490,33 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
       smcm::batch_execute(bl, st);
       if (tm) tm->stop(tok, st, 0.0);
 
-      // results of the group: nnz per chain, digests, result handles
-      const int32_t ng = (int32_t)grp.size();
-      std::vector<int64_t> soff(ng);
-      std::vector<int32_t> scnt(ng);
-      for (int32_t q = 0; q < ng; ++q) {
-        const int32_t r = ch[grp[q]].root;
-        soff[q] = ex.row_off[r];
-        scnt[q] = ex.nodes[r].nrows;
-      }
-      int64_t* dsoff = upload(al, soff);
-      int32_t* dscnt = upload(al, scnt);
-      DevBuf<int64_t> dsum(al, ng);
-      segment_sum_kernel<<<(unsigned)ng, 256, 0, st>>>(olen.p, dsoff, dscnt, dsum.p);
-      SMCM_LAUNCH_CHECK();
-      std::vector<int64_t> sums(ng);
-      SMCM_HIP(hipMemcpyAsync(sums.data(), dsum.p, sizeof(int64_t) * ng, hipMemcpyDeviceToHost, st));
       std::vector<int32_t> hfail(nn);
       SMCM_HIP(hipMemcpyAsync(hfail.data(), bl.failed, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, st));
       SMCM_HIP(hipMemcpyAsync(&hst, dstate.p, sizeof(hst), hipMemcpyDeviceToHost, st));
       SMCM_HIP(hipStreamSynchronize(st));
-      al.free(dsoff);
-      al.free(dscnt);
       if (hst.error & 2) fail(SMCM_EDEVICE, "batch executor: a queue wait timed out");
       bs.nodes += nn;
       bs.tiles += tiles;
-      DevBuf<uint64_t> ddig;
-      if (row_digest) ddig = DevBuf<uint64_t>(al, std::max<int64_t>(rows_total, 1));
-      for (int32_t q = 0; q < ng; ++q) {
-        BatchChain& b = ch[grp[q]];
+      DevBuf<uint64_t> ddig(al, std::max<int64_t>(rows_total, 1));
+      for (int32_t i : grp) {
+        BatchChain& b = ch[i];
         const smcm::BatchNode& R = ex.nodes[b.root];
         if (hfail[b.root]) {
-          rerun.push_back(grp[q]);
+          rerun.push_back(i);
           continue;
         }
-        bs.small_chains++;
-        if (nnz) nnz[grp[q]] = sums[q];
-        if (row_digest && R.nrows) {
+        bs.small_jobs++;
+        b.h_dig.assign(R.nrows, 0);
+        b.h_len.assign(R.nrows, 0);
+        if (R.nrows) {
           uint64_t* d = ddig.p + ex.row_off[b.root];
           smcm::padded_digest(R.out_start, R.out_len, acol.p, aval.p, nullptr, R.nrows, d, st);
-          SMCM_HIP(hipMemcpyAsync(row_digest + b.dig_off, d, sizeof(uint64_t) * R.nrows, hipMemcpyDeviceToHost, st));
+          SMCM_HIP(hipMemcpyAsync(b.h_dig.data(), d, sizeof(uint64_t) * R.nrows, hipMemcpyDeviceToHost, st));
+          SMCM_HIP(hipMemcpyAsync(b.h_len.data(), R.out_len, sizeof(int32_t) * R.nrows, hipMemcpyDeviceToHost, st));
         }
-        if (results) {
-          Csr C;
-          smcm::padded_to_csr(R.out_start, R.out_len, acol.p, aval.p, R.nrows, R.ncols, C, al);
-          results[grp[q]] = wrap_result(ctx, g, b, C);
+        if (keep) {
+          smcm::padded_to_csr(R.out_start, R.out_len, acol.p, aval.p, R.nrows, R.ncols, b.C, al);
+          b.have_C = true;
         }
       }
       SMCM_HIP(hipStreamSynchronize(st));
@@ -461,21 +524,58 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
     }
     bs.exec_s = now_s() - t_exec;
 
-    // ---- large chains (and reruns) through the per-chain path -----------------------------
+    // ---- large jobs (and reruns) through the per-chain path -------------------------------
     const double t_large = now_s();
     std::vector<int32_t> large;
-    for (int32_t i = 0; i < n_chains; ++i)
+    for (int32_t i = 0; i < n_jobs; ++i)
       if (!ch[i].small) large.push_back(i);
     large.insert(large.end(), rerun.begin(), rerun.end());
-    bs.rerun_chains = (int32_t)rerun.size();
+    bs.rerun_jobs = (int32_t)rerun.size();
     for (int32_t i : large) {
-      BatchChain& b = ch[i];
-      run_large(ctx, g, b, nnz ? nnz + i : nullptr, row_digest ? row_digest + b.dig_off : nullptr,
-                results ? results + i : nullptr);
-      bs.large_chains++;
+      run_large(ctx, g, ch[i], keep);
+      bs.large_jobs++;
     }
     bs.large_s = now_s() - t_large;
-    for (int32_t i = 0; i < n_chains; ++i) bs.products_est += (int64_t)ch[i].est_products;
+
+    // ---- every chain: its rows of its job's result ----------------------------------------
+    try {
+      for (int32_t i = 0; i < n_chains; ++i) {
+        const Member& m = mem[i];
+        BatchChain& b = ch[m.job];
+        const bool jw = b.src == nullptr;
+        const int32_t nr = m.whole ? g->dims[b.vts[0]] : (int32_t)m.local.size();
+        std::vector<int32_t> pos(nr);
+        for (int32_t r = 0; r < nr; ++r) {
+          const int32_t loc = m.whole ? r : m.local[r];
+          pos[r] = jw ? loc
+                      : (int32_t)(std::lower_bound(b.src_global.begin(), b.src_global.end(),
+                                                   global_of[b.vts[0]][loc]) -
+                                  b.src_global.begin());
+        }
+        int64_t tot = 0;
+        for (int32_t r = 0; r < nr; ++r) tot += b.h_len[pos[r]];
+        if (nnz) nnz[i] = tot;
+        if (row_digest)
+          for (int32_t r = 0; r < nr; ++r) row_digest[m.dig_off + r] = b.h_dig[pos[r]];
+        if (results) {
+          Csr C;
+          DevBuf<int32_t> dpos(al, std::max<int32_t>(nr, 1));
+          if (nr)
+            SMCM_HIP(hipMemcpyAsync(dpos.p, pos.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, st));
+          smcm::csr_gather_rows(b.C, dpos.p, nr, C, al);
+          results[i] = wrap_result(ctx, g, b.vts, b.L, m.local, m.whole, C);
+          SMCM_HIP(hipStreamSynchronize(st));
+        }
+      }
+    } catch (...) {
+      for (auto& b : ch)
+        if (b.have_C) free_csr(al, b.C);
+      throw;
+    }
+    for (auto& b : ch)
+      if (b.have_C) free_csr(al, b.C);
+    bs.jobs = n_jobs;
+    for (int32_t i = 0; i < n_jobs; ++i) bs.products_est += (int64_t)ch[i].est_products;
     bs.total_s = now_s() - t_all;
     if (stats) *stats = bs;
   });

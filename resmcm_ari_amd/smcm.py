@@ -47,8 +47,8 @@ class Stats(C.Structure):
 
 
 class BatchStats(C.Structure):
-    _fields_ = [("chains", C.c_int32), ("small_chains", C.c_int32), ("large_chains", C.c_int32),
-                ("rerun_chains", C.c_int32), ("groups", C.c_int32), ("pad", C.c_int32), ("nodes", C.c_int64),
+    _fields_ = [("chains", C.c_int32), ("jobs", C.c_int32), ("small_jobs", C.c_int32), ("large_jobs", C.c_int32),
+                ("rerun_jobs", C.c_int32), ("groups", C.c_int32), ("nodes", C.c_int64),
                 ("tiles", C.c_int64), ("products_est", C.c_int64), ("plan_s", C.c_double), ("exec_s", C.c_double),
                 ("large_s", C.c_double), ("total_s", C.c_double)]
 
@@ -318,7 +318,7 @@ class Graph:
                 out_dig.append(dig[o:o + r])
                 o += r
         res = [Result(C.c_void_p(hs[i]), self.ctx) for i in range(n)] if results else None
-        stats = {k: getattr(st, k) for k, _ in BatchStats._fields_ if k != "pad"}
+        stats = {k: getattr(st, k) for k, _ in BatchStats._fields_}
         return nnz[:n], out_dig, res, stats
 
     def estimator_rows(self, mp: MetaPath, i: int, src_rows=None) -> np.ndarray:

@@ -1,7 +1,8 @@
 """Batched form (smcm_batch_run) on the GPU: every chain's result, nnz and row digests equal the
-per-chain path's bit for bit, through the persistent executor (all chains small), the per-chain
-path (all chains large), the rerun path (an arena too small for the batch) and every executor
-shape (workgroup size, sort capacity, tile rows, window width)."""
+per-chain path's bit for bit, through the persistent executor (all jobs small), the per-chain
+path (all jobs large), the rerun path (an arena too small for the batch) and every executor
+shape (workgroup size, sort capacity, tile rows, window width).  Batches repeat meta-paths with
+overlapping, disjoint and empty source sets, so jobs serve several chains."""
 import numpy as np
 import pytest
 
@@ -62,7 +63,7 @@ def check(ctx, G, paths, srcs, ref, expect_small=None):
         np.testing.assert_array_equal(res[i].row_ids(), rows, err_msg=what)
         res[i].free()
     if expect_small is not None:
-        assert st["small_chains"] == expect_small, st
+        assert st["small_jobs"] == expect_small, st
     return st
 
 
@@ -82,7 +83,7 @@ def test_batch_executor_all_small(ctx, graphs, batch_opts, name, seed):
     paths, srcs = make_batch(hg, 60, seed)
     ref = reference(G, paths, srcs)
     ctx.set_option("batch_small_products", 1 << 40)
-    n_multi = sum(mp.pathLen > 1 for mp in paths)
+    n_multi = len({(tuple(mp.vertexTypes), tuple(mp.edgeTypes)) for mp in paths if mp.pathLen > 1})
     check(ctx, G, paths, srcs, ref, expect_small=n_multi)
 
 
@@ -109,7 +110,7 @@ def test_batch_arena_overflow_reruns(ctx, graphs, batch_opts):
     ctx.set_option("batch_small_products", 1 << 40)
     ctx.set_option("batch_arena_mb", 1)
     st = check(ctx, G, paths, srcs, ref)
-    assert st["rerun_chains"] > 0, st
+    assert st["rerun_jobs"] > 0, st
 
 
 @pytest.mark.parametrize("threads,cap,tile,log2w", [(256, 512, 1, 8), (256, 2048, 7, 10), (512, 512, 64, 8),
@@ -135,3 +136,25 @@ def test_batch_empty_and_single(ctx, graphs, batch_opts):
     ref = reference(G, [mp], [empty])
     ctx.set_option("batch_small_products", 1 << 40)
     check(ctx, G, [mp], [empty], ref)
+
+
+@pytest.mark.parametrize("small", [True, False])
+def test_batch_shared_paths(ctx, graphs, batch_opts, small):
+    """Many chains over a few meta-paths: overlapping, nested, duplicate and disjoint source sets."""
+    hg, G = graphs["mid"]
+    rng = np.random.default_rng(21)
+    letters = ["A-P-V-P-A", "P-T-P", "A-P-T-P-V", "T-P-A"]
+    paths, srcs = [], []
+    for q in range(40):
+        mp = configs.metapath_from_letters(letters[q % len(letters)], "bib")
+        t0 = mp.vertexTypes[0]
+        d0 = int(hg.dimensions[t0])
+        k = int(rng.integers(0, min(d0, 400) + 1))
+        rows = np.sort(rng.choice(d0, size=k, replace=False)).astype(np.int32)
+        paths.append(mp)
+        srcs.append((rows + hg.type_offset[t0]).astype(np.int32))
+    srcs[5] = srcs[1].copy()  # identical sources
+    ref = reference(G, paths, srcs)
+    ctx.set_option("batch_small_products", (1 << 40) if small else 0)
+    st = check(ctx, G, paths, srcs, ref)
+    assert st["jobs"] == len(letters), st
