@@ -35,6 +35,7 @@ extern "C" {
 #define SMCM_ENOMEM (-12)
 #define SMCM_EBUDGET (-28)
 #define SMCM_EDEVICE (-5)
+#define SMCM_EAGAIN (-11) /* smcm_server_poll: the query has not finished */
 
 typedef struct smcm_ctx smcm_ctx;
 typedef struct smcm_graph smcm_graph;
@@ -161,6 +162,63 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
                    const int32_t* vtypes, const int32_t* etypes, const int64_t* src_off,
                    const int32_t* src_rows, int64_t* nnz, uint64_t* row_digest,
                    smcm_result** results, smcm_batch_stats* stats);
+
+/* ---- serving form (no counterpart in the reference) --------------------------------------
+ * A request loop over graphs resident in HBM.  Worker threads (slots), each with its own HIP
+ * stream and allocator, take submitted queries as they arrive; each query is planned (estimator
+ * + DP), admitted when its estimator-predicted bytes fit the budget next to the running queries
+ * (alone it always runs), and executed with results reduced to per-row digests and nnz.  A
+ * query predicted above half the budget runs as independent blocks of its source rows, each
+ * planned on its own and predicted at about a quarter of the budget.
+ * Products of leading positions (sources included) are kept in a device prefix cache (LRU) and
+ * reused by later queries with the same graph, sources and leading matrices.  Results are the
+ * same bit for bit as smcm_chain_run's.  The context must not be used by other calls while a
+ * server created from it is running. */
+typedef struct smcm_server smcm_server;
+typedef struct smcm_server_opts {
+  int32_t slots;             /* worker threads / concurrent queries (1..16) */
+  int32_t plan_cost;         /* 0: the reference DP; 1: estimated products + sizes */
+  uint64_t budget_bytes;     /* admission budget of predicted bytes in flight */
+  uint64_t cache_bytes;      /* prefix cache capacity (0 = no cache) */
+  uint64_t cache_entry_max;  /* largest product kept in the cache */
+  uint64_t block_bytes;      /* roots predicted larger than this are streamed in row blocks */
+  int32_t large_slots;       /* workers reserved for large queries (0 = no lanes) */
+  int32_t large_cu_percent;  /* share of the CUs the large-lane workers size their persistent
+                                kernels for (0 or 100 = all) */
+  uint64_t large_bytes;      /* queries predicted above this run on the large-lane workers */
+} smcm_server_opts;
+typedef struct smcm_query_out {
+  int64_t nnz;
+  double t_submit, t_start, t_done; /* seconds; t_start = admission */
+  uint64_t predicted_bytes;
+  int32_t status;                   /* SMCM_OK or the error of the query */
+  int32_t cache_k;                  /* leading positions 0..cache_k served from the cache (0: none) */
+  int32_t blocks;                   /* > 0: run as this many independent blocks of source rows */
+  int32_t pad;
+  char error[160];                  /* message when status != SMCM_OK */
+} smcm_query_out;
+/* graphs: uploaded to ctx; opts NULL = 4 slots (2 of them for large queries, above 4 GiB
+ * predicted, with kernels sized for 75% of the CUs), products plan, 200 GiB budget, 16 GiB cache (entries up to 2 GiB), 32 GiB row
+ * blocks.  Small-lane workers take arrivals in order and hand a query they find large to the
+ * large-lane workers, so short queries do not queue behind long ones. */
+int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graphs,
+                       const smcm_server_opts* opts, smcm_server** out);
+int smcm_server_destroy(smcm_server* s);
+/* Enqueues a query; row_digest (NULL allowed) receives the result's per-row digests and must
+ * stay valid until the query is polled. */
+int smcm_server_submit(smcm_server* s, int32_t graph, const int32_t* vtypes, const int32_t* etypes,
+                       int32_t L, const int32_t* src_rows, int64_t n_src, uint64_t* row_digest,
+                       int64_t* ticket);
+/* wait != 0 blocks until the query is done; otherwise SMCM_EAGAIN while it runs.  A ticket is
+ * released once its result has been returned. */
+int smcm_server_poll(smcm_server* s, int64_t ticket, int32_t wait, smcm_query_out* out);
+int smcm_server_stats(smcm_server* s, int64_t* cache_hits, int64_t* cache_inserts, uint64_t* cache_bytes);
+/* Submits n queries at their arrival times (seconds from the call) and waits for all; times in
+ * out[] are relative to the call.  Arrays as smcm_batch_run, plus graph[i] and arrival[i]. */
+int smcm_server_replay(smcm_server* s, int32_t n, const int32_t* graph, const int32_t* path_len,
+                       const int32_t* vtypes, const int32_t* etypes, const int64_t* src_off,
+                       const int32_t* src_rows, const double* arrival, uint64_t* row_digest,
+                       smcm_query_out* out);
 
 /* ---- result ----------------------------------------------------------------------------- */
 int32_t smcm_result_nrows(const smcm_result* r);

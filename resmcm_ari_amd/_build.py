@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "batch.hip",
     "smcm_api.cpp",
     "batch_api.cpp",
+    "server.cpp",
 ]
 CLI_SOURCES = ["cli_resmcm.cpp"]
 

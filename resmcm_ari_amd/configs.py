@@ -139,3 +139,48 @@ def cfg4_batch(dims, n_chains: int = 4096, seed: int = SEED_BASE + 4, lmin: int 
         rows = np.sort(rng.choice(d0, size=min(n, d0), replace=False)).astype(np.int32)
         out.append(ChainQuery(mp, rows))
     return out
+
+
+@dataclass(frozen=True)
+class ServedQuery:
+    graph: int             # 0: workload-1 bibliographic graph, 1: workload-3 location graph
+    path: MetaPath
+    src_rows: np.ndarray   # ascending local ids of the start type
+    arrival: float         # seconds from the start of the stream
+    parent: int            # index of the earlier query whose prefix (and sources) it shares, -1: none
+
+
+def cfg5_stream(dims_bib, dims_loc, n_queries: int = 512, qps: float = 20.0, seed: int = SEED_BASE + 5,
+                lmin: int = 2, lmax: int = 10, smin: int = 1, smax: int = 100_000,
+                share: float = 0.7) -> list[ServedQuery]:
+    """Workload 5: Poisson arrivals at `qps`; each query picks a graph, a meta-path with lmin..lmax
+    matrices and a source subset of log-uniform size smin..smax (capped at the start type's size).
+    With probability `share` it extends a prefix of an earlier query on the same graph (same
+    sources, same first k matrices, 1 <= k < its own length)."""
+    rng = np.random.default_rng(seed)
+    out: list[ServedQuery] = []
+    t = 0.0
+    for q in range(n_queries):
+        t += float(rng.exponential(1.0 / qps))
+        L = int(rng.integers(lmin, lmax + 1))
+        parent = -1
+        if out and rng.random() < share:
+            parent = int(rng.integers(0, len(out)))
+            p = out[parent]
+            schema = "bib" if p.graph == 0 else "loc"
+            k = int(rng.integers(1, min(p.path.pathLen, L - 1) + 1)) if L > 1 else 1
+            k = min(k, L - 1) if L > 1 else 1
+            tail = random_walk_path(rng, L - k, schema, start=p.path.vertexTypes[k])
+            vts = list(p.path.vertexTypes[: k + 1]) + list(tail.vertexTypes[1:])
+            ets = list(p.path.edgeTypes[:k]) + list(tail.edgeTypes)
+            out.append(ServedQuery(p.graph, MetaPath(vertexTypes=vts, edgeTypes=ets), p.src_rows, t, parent))
+            continue
+        gidx = int(rng.integers(0, 2))
+        schema = "bib" if gidx == 0 else "loc"
+        dims = dims_bib if gidx == 0 else dims_loc
+        mp = random_walk_path(rng, L, schema)
+        d0 = int(dims[mp.vertexTypes[0]])
+        n = log_uniform_int(rng, smin, smax)
+        rows = np.sort(rng.choice(d0, size=min(n, d0), replace=False)).astype(np.int32)
+        out.append(ServedQuery(gidx, mp, rows, t, -1))
+    return out

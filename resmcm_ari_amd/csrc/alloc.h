@@ -2,10 +2,11 @@
 //
 // Small blocks go through hipMallocAsync / hipFreeAsync on the context's stream.  Large blocks
 // (>= kCacheMin) are kept mapped after they are freed and handed out again (best fit, at most
-// 1.5x the request): remapping tens of GB of HBM for every call costs far more than the
-// products themselves.  All work of a context runs on its one stream, so a block freed here can
-// be reused by any later allocation in stream order.  The cache is released when an allocation
-// fails and when the context is destroyed.
+// 1.5x the request), up to cache_cap bytes: remapping tens of GB of HBM for every call costs far
+// more than the products themselves.  All work of a context runs on its one stream, so a block
+// freed here can be reused by any later allocation in stream order.  When an allocation fails
+// the cache is released, this stream synchronised and the device pool trimmed before one more
+// try (no device-wide wait: other contexts' streams keep running).
 #pragma once
 
 #include <map>
@@ -21,6 +22,7 @@ struct DevAlloc {
   size_t live_bytes = 0;
   size_t peak_bytes = 0;
   size_t cached_bytes = 0;
+  size_t cache_cap = SIZE_MAX;
   std::unordered_map<void*, size_t> sizes;
   std::multimap<size_t, void*> cache;
 
@@ -40,11 +42,14 @@ struct DevAlloc {
     }
     if (!p) {
       hipError_t e = hipMallocAsync(&p, bytes, stream);
-      if (e == hipErrorOutOfMemory && cached_bytes > 0) {
+      if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
         release_cache();
+        (void)hipStreamSynchronize(stream);
+        trim_pool();
         e = hipMallocAsync(&p, bytes, stream);
       }
+
       if (e != hipSuccess) {
         (void)hipGetLastError();
         throw HipError((int)e, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
@@ -66,12 +71,19 @@ struct DevAlloc {
     const size_t sz = it != sizes.end() ? it->second : 0;
     if (it != sizes.end()) sizes.erase(it);
     live_bytes = live_bytes > sz ? live_bytes - sz : 0;
-    if (sz >= kCacheMin) {
+    if (sz >= kCacheMin && cached_bytes + sz <= cache_cap) {
       cache.emplace(sz, p);
       cached_bytes += sz;
       return;
     }
     SMCM_HIP(hipFreeAsync(p, stream));
+  }
+  static void trim_pool() {
+    int dev = 0;
+    hipMemPool_t pool;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+      (void)hipMemPoolTrimTo(pool, 0);
+    (void)hipGetLastError();
   }
   void release_cache() {
     for (auto& kv : cache) (void)hipFreeAsync(kv.second, stream);

@@ -8,8 +8,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -102,6 +104,7 @@ struct smcm_graph {
   std::vector<int32_t> h_vtype, h_local, dims;
   std::vector<int32_t*> bins;
   std::map<std::tuple<int32_t, int32_t, int32_t>, Csr> cache;
+  std::recursive_mutex cache_mu;  // typed-matrix cache, shared by the threads of a server
 };
 
 struct smcm_result {
@@ -140,6 +143,7 @@ inline Csr sorted_copy(DevAlloc& al, const Csr& a) {
 
 // typed relation matrix (ts --et--> td) over all rows of ts, cached on the graph
 inline const Csr& typed(smcm_graph* g, int32_t ts, int32_t et, int32_t td) {
+  std::lock_guard<std::recursive_mutex> lk(g->cache_mu);
   auto key = std::make_tuple(ts, et, td);
   auto it = g->cache.find(key);
   if (it != g->cache.end()) return it->second;
@@ -147,11 +151,14 @@ inline const Csr& typed(smcm_graph* g, int32_t ts, int32_t et, int32_t td) {
   Csr c;
   smcm::build_typed_csr(g->vtype, g->adj_ptr, g->adj_nbr, g->adj_et, g->local, g->bins[ts], nullptr, g->dims[ts],
                         et, td, g->dims[td], c, al);
+  // complete before other streams use it
+  SMCM_HIP(hipStreamSynchronize(al.stream));
   return g->cache.emplace(key, c).first->second;
 }
 
 // sorted version of a cached typed matrix (for right operands); cached under et + 1<<20
 inline const Csr& typed_sorted(smcm_graph* g, int32_t ts, int32_t et, int32_t td) {
+  std::lock_guard<std::recursive_mutex> lk(g->cache_mu);
   const Csr& c = typed(g, ts, et, td);
   if (c.sorted) return c;
   auto key = std::make_tuple(ts, et | (1 << 20), td);
@@ -178,6 +185,8 @@ struct Chain {
   std::vector<double> mtab;
   std::vector<int32_t> stab, pairs;
   double es_time = 0, dp_time = 0, spgemm_time = 0, total = 0;
+  // called with every product of positions i..j that execute() materialises (not a streamed root)
+  std::function<void(int32_t i, int32_t j, const Csr& C)> on_product;
 
   ~Chain() {
     if (m0_subset.ptr) free_csr(ctx->al, m0_subset);
@@ -220,6 +229,23 @@ struct Chain {
   // table (and ptab) are copied to the host and the stream is synchronised.  With c_out (device,
   // L*L doubles, and p_out, L*L*L doubles, when want_products) the tables are written there
   // instead and nothing waits: the caller copies them later.
+  // Positions 0..k of the meta-path replaced by P = M_0 ... M_k over the chosen sources
+  // (e.g. a cached product); the chain is then P * M_{k+1} ... M_{len-1}.
+  void prepare_prefix(const int32_t* vtypes, const int32_t* etypes, int32_t len, const Csr& P, int32_t k) {
+    if (!g) fail(SMCM_EINVAL, "null graph");
+    if (k < 0 || k >= len) fail(SMCM_EINVAL, "bad prefix length");
+    L = len - k;
+    want_products = ctx->plan_cost == 1;
+    vts.assign(1, vtypes[0]);
+    vts.insert(vts.end(), vtypes + k + 1, vtypes + len + 1);
+    ets.assign(1, -1);
+    ets.insert(ets.end(), etypes + k + 1, etypes + len);
+    mats.resize(L);
+    mats[0] = P;
+    for (int32_t i = 1; i < L; ++i) mats[i] = typed(g, vts[i], ets[i], vts[i + 1]);
+    subset = false;
+  }
+
   void estimate(double* c_out = nullptr, double* p_out = nullptr) {
     DevAlloc& al = ctx->al;
     const double t0 = now_s();
@@ -416,6 +442,7 @@ struct Chain {
         smcm::spgemm_blocked(a.m, b.m, est, L - i, block_bytes, *consumer, ctx->opts, &ss, al, tm);
       else
         smcm::spgemm(a.m, b.m, est, L - i, C, ctx->opts, &ss, al, true, tm);
+      if (on_product && !(root && consumer)) on_product(i, j, C);
       if (stats) {
         stats->products += ss.products;
         for (int t = 0; t < 4; ++t) stats->rows_tier[t] += ss.rows[t];

@@ -32,7 +32,8 @@ EXPORTS = [
     "smcm_result_nrows", "smcm_result_nnz", "smcm_result_copy", "smcm_result_copy_rows", "smcm_result_row_ids",
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
     "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth", "smcm_chain_products_estimate",
-    "smcm_batch_run",
+    "smcm_batch_run", "smcm_server_create", "smcm_server_destroy", "smcm_server_submit", "smcm_server_poll",
+    "smcm_server_stats", "smcm_server_replay",
 ]
 
 
@@ -51,6 +52,18 @@ class BatchStats(C.Structure):
                 ("rerun_jobs", C.c_int32), ("groups", C.c_int32), ("nodes", C.c_int64),
                 ("tiles", C.c_int64), ("products_est", C.c_int64), ("plan_s", C.c_double), ("exec_s", C.c_double),
                 ("large_s", C.c_double), ("total_s", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("plan_cost", C.c_int32), ("budget_bytes", C.c_uint64),
+                ("cache_bytes", C.c_uint64), ("cache_entry_max", C.c_uint64), ("block_bytes", C.c_uint64),
+                ("large_slots", C.c_int32), ("large_cu_percent", C.c_int32), ("large_bytes", C.c_uint64)]
+
+
+class QueryOut(C.Structure):
+    _fields_ = [("nnz", C.c_int64), ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("predicted_bytes", C.c_uint64), ("status", C.c_int32), ("cache_k", C.c_int32),
+                ("blocks", C.c_int32), ("pad", C.c_int32), ("error", C.c_char * 160)]
 
 
 class SmcmError(RuntimeError):
@@ -86,6 +99,12 @@ def load() -> C.CDLL:
             "smcm_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_chain_products_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_batch_run": [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            "smcm_server_create": [vp, vp, i32, vp, vp],
+            "smcm_server_destroy": [vp],
+            "smcm_server_submit": [vp, i32, vp, vp, i32, vp, i64, vp, vp],
+            "smcm_server_poll": [vp, i64, i32, vp],
+            "smcm_server_stats": [vp, vp, vp, vp],
+            "smcm_server_replay": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "smcm_estimator_rows": [vp, vp, vp, vp, i32, vp, i64, i32, vp, i64],
             "smcm_result_nrows": [vp],
             "smcm_result_nnz": [vp],
@@ -170,7 +189,9 @@ class Context:
 
     def close(self) -> None:
         if self._h:
-            for ch in list(self._children):
+            kids = list(self._children)
+            # servers first (they use the graphs), then results and graphs
+            for ch in sorted(kids, key=lambda c: 0 if isinstance(c, Server) else 1):
                 ch.free() if isinstance(ch, Result) else ch.close()
             _check(load().smcm_ctx_destroy(self._h))
             self._h = C.c_void_p()
@@ -398,3 +419,73 @@ def build_pair(graph: Graph, path: MetaPath, src_rows=None) -> tuple[Result, Pai
     product and the four timings."""
     r, t = graph.run(path, src_rows)
     return r, PairTimes(es_time=t["est"], dp_time=t["dp"], spgemm_time=t["spgemm"], smcm_time=t["total"])
+
+
+class Server:
+    """Serving form: worker slots over graphs resident in HBM (see smcm_server_create)."""
+
+    def __init__(self, ctx: Context, graphs: list, slots: int = 4, plan_cost: int = 1, budget: int = 200 << 30,
+                 cache: int = 16 << 30, cache_entry_max: int = 2 << 30, block: int = 32 << 30,
+                 large_slots: int | None = None, large_bytes: int = 4 << 30, large_cu_percent: int = 75):
+        self.ctx = ctx
+        self.graphs = list(graphs)
+        arr = (C.c_void_p * len(graphs))(*[G._h for G in graphs])
+        if large_slots is None:
+            large_slots = slots // 2 if slots > 1 else 0
+        o = ServerOpts(slots, plan_cost, budget, cache, cache_entry_max, block, large_slots, large_cu_percent,
+                       large_bytes)
+        h = C.c_void_p()
+        _check(load().smcm_server_create(ctx._h, arr, len(graphs), C.byref(o), C.byref(h)))
+        self._h = h
+        ctx._children.add(self)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(load().smcm_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> dict:
+        hits, ins, byts = C.c_int64(), C.c_int64(), C.c_uint64()
+        _check(load().smcm_server_stats(self._h, C.byref(hits), C.byref(ins), C.byref(byts)))
+        return {"cache_hits": hits.value, "cache_inserts": ins.value, "cache_bytes": byts.value}
+
+    def replay(self, queries, arrivals):
+        """queries: list of (graph index, MetaPath, global source rows or None); arrivals: seconds.
+        Returns (list of per-query dicts, list of per-row digest arrays)."""
+        n = len(queries)
+        gi = np.array([q[0] for q in queries], np.int32)
+        plen = np.array([q[1].pathLen for q in queries], np.int32)
+        vts = np.concatenate([np.asarray(q[1].vertexTypes, np.int32) for q in queries])
+        ets = np.concatenate([np.asarray(q[1].edgeTypes, np.int32) for q in queries])
+        if any(q[2] is None for q in queries):
+            if not all(q[2] is None for q in queries):
+                raise ValueError("either every query has source rows or none")
+            soff, srows = None, None
+            nrows = [int(self.graphs[q[0]].hg.dimensions[q[1].vertexTypes[0]]) for q in queries]
+        else:
+            parts = [np.ascontiguousarray(q[2], np.int32) for q in queries]
+            nrows = [len(p) for p in parts]
+            soff = np.zeros(n + 1, np.int64)
+            soff[1:] = np.cumsum(nrows)
+            srows = np.concatenate(parts) if soff[-1] > 0 else np.zeros(1, np.int32)
+        arr = np.ascontiguousarray(arrivals, np.float64)
+        dig = np.zeros(max(int(sum(nrows)), 1), np.uint64)
+        outs = (QueryOut * max(n, 1))()
+        _check(load().smcm_server_replay(self._h, n, _ptr(gi), _ptr(plen), _ptr(vts), _ptr(ets),
+                                         None if soff is None else _ptr(soff), None if srows is None else _ptr(srows),
+                                         _ptr(arr), _ptr(dig), outs))
+        res, digs, o = [], [], 0
+        for i in range(n):
+            q = outs[i]
+            d = {k: getattr(q, k) for k, _ in QueryOut._fields_}
+            d["error"] = d["error"].decode(errors="replace")
+            res.append(d)
+            digs.append(dig[o:o + nrows[i]])
+            o += nrows[i]
+        return res, digs
