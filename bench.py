@@ -13,6 +13,14 @@ value = intermediate products per second (sum over the plan's SpGEMMs; the plan 
 reference DP's, which the tests check).  roofline = algorithmic bytes / HIP-event duration of the
 dominant kernel over the timed region; cpu_baseline = the oracle (CPU restatement of the
 reference path, OpenMP) on a fixed sample of source rows of the same chains.
+
+Beside it, on the same GPU in the same run:
+  config1  the 20k/30k/200 graph's A-P-V-P-A chain: SpGEMM kernels' algorithmic bytes / time;
+  batched  4096 chains (L 2-6, log-uniform sources 16..65536) over that graph with T=20k:
+           smcm_batch_run vs the per-chain path, queries/s, every result's digests compared;
+  served   a Poisson stream over that graph and the 100k-user location graph (L 2-10, sources
+           log-uniform 1..100k, 70% sharing a prefix): the server vs one-at-a-time execution,
+           queries/s and p50/p99 latency, every result's digests compared.
 """
 from __future__ import annotations
 
@@ -90,6 +98,112 @@ def run_plan(O, mats, pairs):
     return stack[0], total
 
 
+def bench_config1(hg1, reps: int = 20) -> dict:
+    """A-P-V-P-A over all 20k authors of the workload-1 graph: SpGEMM kernels' algorithmic bytes
+    per second of kernel time (HIP events)."""
+    ctx = smcm.Context(0)
+    G = smcm.Graph(ctx, hg1)
+    mp = configs.metapath_from_letters("A-P-V-P-A", "bib")
+    for _ in range(3):
+        r, _ = G.run(mp)
+        r.free()
+    ctx.set_option("kernel_timing", 1)
+    ctx.kernel_stats_reset()
+    t0 = time.perf_counter()
+    prods = 0
+    for _ in range(reps):
+        r, _ = G.run(mp)
+        prods += ctx.last_stats()["products"]
+        r.free()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ks = ctx.kernel_stats()
+    ms = sum(v[0] for k, v in ks.items() if k.startswith("spgemm."))
+    byts = sum(v[2] for k, v in ks.items() if k.startswith("spgemm."))
+    G.close()
+    ctx.close()
+    gbps = byts / (ms * 1e-3) / 1e9 if ms else 0.0
+    return {"workload": "A-P-V-P-A over A=20k P=30k V=200 (seed 0x5EED0001), all rows",
+            "chain_ms": round(dt / reps * 1e3, 3), "spgemm_kernel_ms": round(ms / reps, 3),
+            "products_per_chain": prods // reps, "spgemm_GBps": round(gbps, 1),
+            "spgemm_frac_of_peak": round(gbps / PEAK_HBM_GBPS, 4)}
+
+
+def bench_batched(hg1, n_chains: int = 4096) -> dict:
+    """Workload 4: the batched form against the per-chain path on the same chains."""
+    ctx = smcm.Context(0)
+    G = smcm.Graph(ctx, hg1)
+    batch = configs.cfg4_batch(hg1.dimensions, n_chains)
+    off = hg1.type_offset
+    paths = [q.path for q in batch]
+    srcs = [(q.src_rows + off[q.path.vertexTypes[0]]).astype(np.int32) for q in batch]
+    G.run_batch(paths[:32], srcs[:32])  # typed matrices, code objects
+    ctx.sync()
+    t0 = time.perf_counter()
+    nnz, dig, _, st = G.run_batch(paths, srcs, digests=True)
+    tb = time.perf_counter() - t0
+    print(f"[bench] batched: {n_chains} chains in {tb:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    bad = 0
+    for i, mp in enumerate(paths):
+        r, _ = G.run(mp, srcs[i])
+        d = r.row_digest()
+        if r.nnz != nnz[i] or not np.array_equal(d, dig[i]):
+            bad += 1
+        r.free()
+        if i % 1024 == 1023:
+            print(f"[bench] per-chain path: {i + 1} chains, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+    tp = time.perf_counter() - t0
+    G.close()
+    ctx.close()
+    return {"workload": f"{n_chains} chains, L 2-6 over A/P/V/T, sources log-uniform 16..65536 "
+                        f"(seed 0x5EED0004), graph A=20k P=30k V=200 T=20k",
+            "queries_per_s": round(n_chains / tb, 2), "plain_queries_per_s": round(n_chains / tp, 2),
+            "speedup": round(tp / tb, 2), "seconds": round(tb, 3), "plain_seconds": round(tp, 3),
+            "mismatching_chains": bad, "jobs": st["jobs"], "executor_jobs": st["small_jobs"],
+            "result_nnz": int(nnz.sum())}
+
+
+def bench_served(hg1, n_queries: int = 128, qps: float = 5.0) -> dict:
+    """Workload 5: the server against one-at-a-time execution on the same Poisson stream."""
+    hg3 = configs.CFG3.build()
+    ctx = smcm.Context(0)
+    G = [smcm.Graph(ctx, hg1), smcm.Graph(ctx, hg3)]
+    hgs = [hg1, hg3]
+    stream = configs.cfg5_stream(hg1.dimensions, hg3.dimensions, n_queries=n_queries, qps=qps)
+    queries = [(q.graph, q.path, (q.src_rows + hgs[q.graph].type_offset[q.path.vertexTypes[0]]).astype(np.int32))
+               for q in stream]
+    arr = np.array([q.arrival for q in stream])
+    res = {}
+    for name, kw in (("served", dict(slots=4, plan_cost=1)), ("plain", dict(slots=1, plan_cost=0, cache=0))):
+        srv = smcm.Server(ctx, G, **kw)
+        outs, digs = srv.replay(queries, arr)
+        stats = srv.stats()
+        srv.close()
+        lat = np.array([o["t_done"] - a for o, a in zip(outs, arr)])
+        span = max(o["t_done"] for o in outs) - arr[0]
+        res[name] = (outs, digs, lat, span, stats)
+        print(f"[bench] {name}: {n_queries} queries, span {span:.1f} s", file=sys.stderr, flush=True)
+    (o1, d1, l1, s1, st1), (o0, d0, l0, s0, _) = res["served"], res["plain"]
+    bad = sum(1 for i in range(n_queries) if o1[i]["status"] or o0[i]["status"] or o1[i]["nnz"] != o0[i]["nnz"]
+              or not np.array_equal(d1[i], d0[i]))
+    for G_ in G:
+        G_.close()
+    ctx.close()
+    return {"workload": f"{n_queries} queries, Poisson {qps} /s, L 2-10, sources log-uniform 1..100k, 70% sharing "
+                        "a prefix (seed 0x5EED0005); graphs A=20k P=30k V=200 T=20k and U=100k L=200k C=500 Y=1k "
+                        "S=168 resident",
+            "queries_per_s": round(n_queries / s1, 3), "p50_ms": round(float(np.percentile(l1, 50)) * 1e3, 1),
+            "p99_ms": round(float(np.percentile(l1, 99)) * 1e3, 1),
+            "plain_queries_per_s": round(n_queries / s0, 3),
+            "plain_p50_ms": round(float(np.percentile(l0, 50)) * 1e3, 1),
+            "plain_p99_ms": round(float(np.percentile(l0, 99)) * 1e3, 1),
+            "mismatching_queries": bad, "cache_hits": st1["cache_hits"],
+            "server": "4 slots (2 for large queries), products plan, 200 GiB admission budget, 16 GiB prefix cache",
+            "plain": "one query at a time, reference DP plan, no cache"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +213,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=float, default=0.02, help="fraction of source rows for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains", default=",".join(CHAINS))
+    ap.add_argument("--no-side", action="store_true", help="skip the config-1, batched and served measurements")
+    ap.add_argument("--batch-chains", type=int, default=4096)
+    ap.add_argument("--serve-queries", type=int, default=128)
+    ap.add_argument("--serve-qps", type=float, default=5.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -163,6 +281,15 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hg, args.cpu_sample)
+    G.close()
+    ctx.close()
+
+    side = {}
+    if not args.no_side:
+        hg1 = configs.CFG1.build()
+        side["config1"] = bench_config1(hg1)
+        side["batched"] = bench_batched(hg1, args.batch_chains)
+        side["served"] = bench_served(hg1, args.serve_queries, args.serve_qps)
 
     value = prods_step * args.steps / dt
     out = {
@@ -186,10 +313,9 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        **side,
     }
     print(json.dumps(out))
-    G.close()
-    ctx.close()
     return 0
 
 
