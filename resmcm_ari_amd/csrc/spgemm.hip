@@ -54,11 +54,12 @@ __global__ void entry_len_kernel(const int32_t* __restrict__ acol, int64_t nnz, 
 
 enum Tier { kEmpty = 0, kWaveTier = 1, kSortTier = 2, kBitmapTier = 3, kHeavyTier = 4, kNumBins = 5 };
 
+
 __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __restrict__ aptr, int32_t r0, int32_t r1,
                                                             int32_t m, const int64_t* __restrict__ S,
                                                             const double* __restrict__ est_r, int est_stride,
                                                             double ncols, int s_max, int m_max, int dense_div,
-                                                            int64_t heavy_ub, int force_tier,
+                                                            int64_t heavy_ub, int force_tier, int wave_bitmap_cap,
                                                             int32_t* __restrict__ bins, int32_t* __restrict__ counts,
                                                             unsigned long long* __restrict__ sums) {
   const int32_t u = r0 + (int32_t)(blockIdx.x * kThreads + threadIdx.x);
@@ -72,9 +73,11 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
     } else if (force_tier >= 0) {
       tier = force_tier == 0 ? kWaveTier : force_tier == 1 ? kSortTier : kBitmapTier;
       if (tier == kWaveTier && ub > 64) tier = kSortTier;
-      if (tier == kSortTier && ub > m_max) tier = kBitmapTier;
+      if (tier == kSortTier && ub > (wave_bitmap_cap ? wave_bitmap_cap : m_max)) tier = kBitmapTier;
     } else if (ub <= s_max) {
       tier = kWaveTier;
+    } else if (wave_bitmap_cap) {
+      tier = ub <= wave_bitmap_cap ? kSortTier : kBitmapTier;
     } else {
       bool dense = false;
       if (est_r) dense = est_r[(size_t)u * est_stride] * ncols * (double)dense_div >= ncols;
@@ -463,7 +466,178 @@ __global__ void __launch_bounds__(kThreads) wave_tier_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
-// sort tier: one workgroup per row, products sorted in LDS.  Dynamic LDS: key[P] (+ val[P]).
+// mid tier for products with few columns (wave bitmap; rows with 64 < ub <= wave_bitmap_cap):
+// one wave per row, four rows per block.
+// The wave's own LDS region holds the row's column bitmap (nw 32-bit words), the exclusive
+// popcount prefix of every word (16-bit) and value slots (cap <= 65535 entries).  Symbolic:
+// mark the columns, count the bits.  Numeric: mark, rank every word, accumulate each product at
+// the rank of its column, list every column at its rank, then write the row in column order
+// with consecutive lanes writing consecutive entries.  No
+// workgroup barrier: the four waves of a block work on four rows independently.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct WaveBitmapLayout {
+  int32_t nw = 0;        // bitmap words (ncols / 32, rounded up)
+  int32_t cap = 0;       // value (and column) slots, >= every routed row's ub
+  size_t rank_off = 0, val_off = 0, col_off = 0, per_wave = 0;
+  static size_t bytes(int64_t nw, int64_t cap, bool numeric) {
+    if (!numeric) return (size_t)nw * 4;
+    return ((size_t)nw * 6 + 7) / 8 * 8 + (size_t)cap * 12;
+  }
+  static WaveBitmapLayout make(int32_t ncols, int32_t cap, bool numeric) {
+    WaveBitmapLayout l;
+    l.nw = (ncols + 31) / 32;
+    l.cap = numeric ? cap : 0;
+    l.rank_off = (size_t)l.nw * 4;
+    l.val_off = (l.rank_off + (numeric ? (size_t)l.nw * 2 : 0) + 7) / 8 * 8;
+    l.col_off = l.val_off + (size_t)l.cap * 8;
+    l.per_wave = bytes(l.nw, cap, numeric);
+    return l;
+  }
+};
+
+// the mid tier (64 < ub <= wave_bitmap_cap) of a product with at most wave_bitmap_cols columns runs one
+// wave per row over an LDS column bitmap; four waves' regions must fit the LDS of a workgroup
+inline bool wave_bitmap_for(const SpgemmOpts& o, int32_t ncols) {
+  if (o.wave_bitmap_cols <= 0 || ncols > o.wave_bitmap_cols || o.wave_bitmap_cap < 64 || o.wave_bitmap_cap > 65535)
+    return false;
+  const int64_t nw = (ncols + 31) / 32;
+  return (int64_t)WaveBitmapLayout::bytes(nw, o.wave_bitmap_cap, true) * (kThreads / kWave) <= o.lds_bytes;
+}
+
+// f(col, val) for every product of row [rs, re) of A, one wave: lanes load 64 entries at a time;
+// a B segment of at most 8 products is walked by its lane (all its loads issued first), longer
+// ones by the whole wave with four loads in flight per lane
+template <bool AV, bool BV, class F>
+__device__ __forceinline__ void wave_products(int64_t rs, int64_t re, const int32_t* __restrict__ acol,
+                                              const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
+                                              const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
+                                              F&& f) {
+  constexpr int kLane = 8;
+  const int lane = lane_id();
+  for (int64_t cb = rs; cb < re; cb += kWave) {
+    const int64_t e = cb + lane;
+    int64_t b0 = 0, b1 = 0;
+    uint64_t a = 1ull;
+    if (e < re) {
+      const int32_t k = acol[e];
+      b0 = bptr[k];
+      b1 = bptr[k + 1];
+      if constexpr (AV) a = aval[e];
+    }
+    const int64_t len = b1 - b0;
+    const bool longseg = len > kLane;
+    if (!longseg && len > 0) {
+      int32_t c[kLane];
+      uint64_t v[kLane];
+#pragma unroll
+      for (int x = 0; x < kLane; ++x) {
+        if (x < len) {
+          c[x] = bcol[b0 + x];
+          v[x] = BV ? bval[b0 + x] : 1ull;
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < kLane; ++x)
+        if (x < len) f(c[x], BV ? a * v[x] : a);
+    }
+    uint64_t mask = __ballot(longseg);
+    while (mask) {
+      const int src = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const int64_t s0 = __shfl(b0, src), s1 = __shfl(b1, src);
+      const uint64_t sa = __shfl(a, src);
+      for (int64_t b = s0 + lane; b < s1; b += 4 * kWave) {
+        int32_t c[4];
+        uint64_t v[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          if (b + x * kWave < s1) {
+            c[x] = bcol[b + x * kWave];
+            v[x] = BV ? bval[b + x * kWave] : 1ull;
+          }
+        }
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+          if (b + x * kWave < s1) f(c[x], BV ? sa * v[x] : sa);
+      }
+    }
+  }
+}
+
+template <bool NUMERIC, bool AV, bool BV>
+__global__ void __launch_bounds__(kThreads) wave_bitmap_kernel(
+    const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
+    const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
+    const int32_t* __restrict__ rows, int32_t nrows, WaveBitmapLayout lay, int64_t* __restrict__ row_nnz,
+    const int64_t* __restrict__ cptr, int32_t* __restrict__ ccol, uint64_t* __restrict__ cval) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int32_t ri = blockIdx.x * (kThreads / kWave) + wid;
+  if (ri >= nrows) return;  // whole wave exits together
+  unsigned char* base = lds + (size_t)wid * lay.per_wave;
+  uint32_t* bits = (uint32_t*)base;
+  uint16_t* rank = (uint16_t*)(base + lay.rank_off);
+  uint64_t* vals = (uint64_t*)(base + lay.val_off);
+  int32_t* cols = (int32_t*)(base + lay.col_off);
+  const int32_t nw = lay.nw;
+  const int32_t u = rows[ri];
+  const int64_t rs = aptr[u], re = aptr[u + 1];
+  for (int32_t w = lane; w < nw; w += kWave) bits[w] = 0u;
+  wave_lds_sync();
+  wave_products<false, false>(rs, re, acol, aval, bptr, bcol, bval,
+                              [&](int32_t c, uint64_t) { atomicOr(&bits[c >> 5], 1u << (c & 31)); });
+  wave_lds_sync();
+  const int32_t wpl = (nw + kWave - 1) / kWave;
+  const int32_t w0 = lane * wpl, w1 = min(w0 + wpl, nw);
+  int32_t cnt = 0;
+  for (int32_t w = w0; w < w1; ++w) cnt += __popc(bits[w]);
+  int32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  const int32_t total = __shfl(incl, kWave - 1);
+  if constexpr (!NUMERIC) {
+    if (lane == 0) row_nnz[u] = total;
+  } else {
+    int32_t run = incl - cnt;
+    for (int32_t w = w0; w < w1; ++w) {
+      rank[w] = (uint16_t)run;
+      run += __popc(bits[w]);
+    }
+    for (int32_t i = lane; i < total; i += kWave) vals[i] = 0ull;
+    wave_lds_sync();
+    wave_products<AV, BV>(rs, re, acol, aval, bptr, bcol, bval, [&](int32_t c, uint64_t v) {
+      const int32_t w = c >> 5;
+      const int32_t r = rank[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
+      atomicAdd((unsigned long long*)&vals[r], (unsigned long long)v);
+    });
+    // every lane lists the columns of its own words at their ranks, then the row leaves in
+    // column order with consecutive lanes writing consecutive entries
+    for (int32_t w = w0; w < w1; ++w) {
+      uint32_t x = bits[w];
+      int32_t r = rank[w];
+      while (x) {
+        cols[r++] = (w << 5) | (__ffs(x) - 1);
+        x &= x - 1u;
+      }
+    }
+    wave_lds_sync();
+    const int64_t o = cptr[u];
+    for (int32_t p = lane; p < total; p += kWave) {
+      ccol[o + p] = cols[p];
+      cval[o + p] = vals[p];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 template <bool NUMERIC, bool AV, bool BV>
 __global__ void __launch_bounds__(kThreads) sort_tier_kernel(
     const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
@@ -929,7 +1103,15 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     SMCM_LAUNCH_CHECK();
     T.stop(tok, st, tier_bytes(NUMERIC, tc, kWaveTier, AV, BV));
   }
-  if (nS > 0) {
+  if (nS > 0 && wave_bitmap_for(o, B.ncols)) {
+    const WaveBitmapLayout wl = WaveBitmapLayout::make(B.ncols, (int32_t)o.wave_bitmap_cap, NUMERIC);
+    const unsigned g = (unsigned)ceil_div(nS, kThreads / kWave);
+    const int tok = T.start(NUMERIC ? "spgemm.numeric.wavebitmap" : "spgemm.symbolic.wavebitmap", st);
+    wave_bitmap_kernel<NUMERIC, AV, BV><<<g, kThreads, wl.per_wave * (kThreads / kWave), st>>>(
+        A.ptr, A.col, A.val, B.ptr, B.col, B.val, bins + (size_t)kSortTier * m, nS, wl, row_nnz, cptr, ccol, cval);
+    SMCM_LAUNCH_CHECK();
+    T.stop(tok, st, tier_bytes(NUMERIC, tc, kSortTier, AV, BV));
+  } else if (nS > 0) {
     int cap = 64;
     while (cap < o.m_max) cap <<= 1;
     const size_t lds = (size_t)cap * (NUMERIC ? 12 : 4);
@@ -1149,7 +1331,7 @@ struct SpgemmCore {
     if (r1 > r0) {
       bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, kThreads), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
-          o.force_tier, bins.p, counts.p, sums.p);
+          o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_cap : 0, bins.p, counts.p, sums.p);
       SMCM_LAUNCH_CHECK();
     }
     unsigned long long sh[2 * kNumBins];

@@ -28,7 +28,8 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("tier", [-1, 0, 1, 2])
+# tier "1s": the mid tier through the workgroup LDS sort instead of the wave bitmap
+@pytest.mark.parametrize("tier", [-1, 0, 1, "1s", 2])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("vals", ["none", "A", "B", "AB"])
 def test_spgemm_tiers(ctx, case, tier, vals):
@@ -37,12 +38,41 @@ def test_spgemm_tiers(ctx, case, tier, vals):
     A = random_csr(rng, m, k, row_len=la, with_vals=vals in ("A", "AB"))
     B = random_csr(rng, k, n, row_len=lb, with_vals=vals in ("B", "AB"))
     ref = O.spgemm(A, B)
-    ctx.set_option("force_tier", tier)
+    wb = ctx.get_option("wave_bitmap_cols")
+    ctx.set_option("force_tier", 1 if tier == "1s" else tier)
+    if tier == "1s":
+        ctx.set_option("wave_bitmap_cols", 0)
     try:
         ptr, col, val = _run(ctx, A, B)
     finally:
         ctx.set_option("force_tier", -1)
+        ctx.set_option("wave_bitmap_cols", wb)
     assert_same_csr(ptr, col, val, ref, what=f"{name} tier={tier} vals={vals}")
+
+
+@pytest.mark.parametrize("cap", [64, 1024, 2048, 4096])
+@pytest.mark.parametrize("n", [1, 31, 33, 4096, 32768])
+@pytest.mark.parametrize("shape", ["long", "short"])
+def test_spgemm_wave_bitmap_shapes(ctx, cap, n, shape):
+    """Wave-bitmap mid tier at word boundaries and LDS limits; A rows of few long B segments and
+    of many short ones (lane-walked and wave-walked segments)."""
+    rng = np.random.default_rng(cap * 7 + n + (shape == "long"))
+    if shape == "long":
+        A = random_csr(rng, 300, 120, row_len=lambda r: r.integers(0, 12), with_vals=True)
+        B = random_csr(rng, 120, n, row_len=lambda r: r.integers(0, min(n, 400) + 1), with_vals=True)
+    else:
+        A = random_csr(rng, 300, 2000, row_len=lambda r: r.integers(0, 400), with_vals=True)
+        B = random_csr(rng, 2000, n, row_len=lambda r: r.integers(0, min(n, 9) + 1), with_vals=True)
+    ref = O.spgemm(A, B)
+    saved = ctx.get_option("wave_bitmap_cap")
+    ctx.set_option("wave_bitmap_cap", cap)
+    ctx.set_option("force_tier", 1)
+    try:
+        ptr, col, val = _run(ctx, A, B)
+    finally:
+        ctx.set_option("force_tier", -1)
+        ctx.set_option("wave_bitmap_cap", saved)
+    assert_same_csr(ptr, col, val, ref, what=f"wave bitmap n={n} cap={cap} {shape}")
 
 
 def test_spgemm_duplicates_in_A(ctx):
