@@ -16,6 +16,8 @@ reference path, OpenMP) on a fixed sample of source rows of the same chains.
 
 Beside it, on the same GPU in the same run:
   config1  the 20k/30k/200 graph's A-P-V-P-A chain: SpGEMM kernels' algorithmic bytes / time;
+  config3  the long location meta-paths over all 100k users (results of 1e10 entries streamed in
+           row blocks), reference plan and product plan;
   batched  4096 chains (L 2-6, log-uniform sources 16..65536) over that graph with T=20k:
            smcm_batch_run vs the per-chain path, queries/s, every result's digests compared;
   served   a Poisson stream over that graph and the 100k-user location graph (L 2-10, sources
@@ -120,13 +122,55 @@ def bench_config1(hg1, reps: int = 20) -> dict:
     ks = ctx.kernel_stats()
     ms = sum(v[0] for k, v in ks.items() if k.startswith("spgemm."))
     byts = sum(v[2] for k, v in ks.items() if k.startswith("spgemm."))
+    ctx.set_option("kernel_timing", 0)
+    copy_gbps = ctx.copy_bandwidth(1 << 32, 2)  # also the counter calibration of a profiled run
     G.close()
     ctx.close()
     gbps = byts / (ms * 1e-3) / 1e9 if ms else 0.0
     return {"workload": "A-P-V-P-A over A=20k P=30k V=200 (seed 0x5EED0001), all rows",
             "chain_ms": round(dt / reps * 1e3, 3), "spgemm_kernel_ms": round(ms / reps, 3),
             "products_per_chain": prods // reps, "spgemm_GBps": round(gbps, 1),
-            "spgemm_frac_of_peak": round(gbps / PEAK_HBM_GBPS, 4)}
+            "spgemm_frac_of_peak": round(gbps / PEAK_HBM_GBPS, 4), "measured_copy_GBps": round(copy_gbps, 1),
+            "kernels": {k: {"ms": round(v[0] / reps, 4), "launches": v[1] // reps,
+                            "GBps": round(v[2] / v[0] / 1e6, 1) if v[0] else 0.0,
+                            "bytes_per_launch": round(v[2] / v[1]) if v[1] else 0}
+                        for k, v in sorted(ks.items(), key=lambda kv: -kv[1][0])}}
+
+
+def bench_config3(block_gb: float = 32.0) -> dict:
+    """Workload 3: the long location meta-paths over all 100k users, roots streamed in row blocks;
+    the reference plan (drop-in) and the product plan, same results."""
+    hg3 = configs.CFG3.build()
+    ctx = smcm.Context(0)
+    G = smcm.Graph(ctx, hg3)
+    out = {"workload": "U=100k L=200k C=500 Y=1k S=168 (seed 0x5EED0003), all users, roots in "
+                       f"{block_gb:g} GB row blocks", "chains": []}
+    tot = {0: [0.0, 0], 1: [0.0, 0]}
+    for p in configs.CFG3_PATHS:
+        mp = configs.metapath_from_letters(p, "loc")
+        row = {"path": p}
+        digs = []
+        for pc in (0, 1):
+            ctx.set_option("plan_cost", pc)
+            t0 = time.perf_counter()
+            n, d, _ = G.run_blocked(mp, int(block_gb * (1 << 30)), digest=True)
+            dt = time.perf_counter() - t0
+            st = ctx.last_stats()
+            digs.append(d)
+            key = "reference_plan" if pc == 0 else "product_plan"
+            row[key] = {"seconds": round(dt, 3), "products": st["products"], "nnz": n}
+            tot[pc][0] += dt
+            tot[pc][1] += st["products"]
+            print(f"[bench] config 3 {p} plan_cost={pc}: {dt:.2f} s", file=sys.stderr, flush=True)
+        row["same_result"] = bool(np.array_equal(digs[0], digs[1]))
+        out["chains"].append(row)
+    ctx.set_option("plan_cost", 0)
+    out["reference_plan_products_per_s"] = round(tot[0][1] / tot[0][0], 1)
+    out["product_plan_seconds"] = round(tot[1][0], 3)
+    out["reference_plan_seconds"] = round(tot[0][0], 3)
+    G.close()
+    ctx.close()
+    return out
 
 
 def bench_batched(hg1, n_chains: int = 4096) -> dict:
@@ -217,11 +261,20 @@ def main() -> int:
     ap.add_argument("--batch-chains", type=int, default=4096)
     ap.add_argument("--serve-queries", type=int, default=128)
     ap.add_argument("--serve-qps", type=float, default=5.0)
+    ap.add_argument("--only", choices=["config1", "config3", "batched", "served"],
+                    help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by source rows, see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.only:
+        hg1 = configs.CFG1.build() if args.only != "config3" else None
+        res = {"config1": lambda: bench_config1(hg1), "config3": lambda: bench_config3(),
+               "batched": lambda: bench_batched(hg1, args.batch_chains),
+               "served": lambda: bench_served(hg1, args.serve_queries, args.serve_qps)}[args.only]()
+        print(json.dumps({args.only: res}))
+        return 0
     chains = args.chains.split(",")
     block_bytes = int(args.block_gb * (1 << 30))
 
@@ -288,6 +341,7 @@ def main() -> int:
     if not args.no_side:
         hg1 = configs.CFG1.build()
         side["config1"] = bench_config1(hg1)
+        side["config3"] = bench_config3()
         side["batched"] = bench_batched(hg1, args.batch_chains)
         side["served"] = bench_served(hg1, args.serve_queries, args.serve_qps)
 

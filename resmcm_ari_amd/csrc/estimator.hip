@@ -9,7 +9,7 @@
 // built with -ffp-contract=off), so per-row densities are bit-identical to a one-thread run of
 // the reference.  The product for one row is a dependent chain and cannot be reassociated; rows
 // of degree <= kLightDeg use one lane each, longer rows one wave each (the wave gathers 64
-// factors in parallel and every lane multiplies them in order through readlane broadcasts).
+// factors in parallel into LDS and lane j multiplies chain j through them in order).
 // Once all running products are exactly 0 they stay 0, so a row stops early - this is exact.
 //
 // c_ij is reduced in a fixed tree order (deterministic run to run); exact_sum instead sums in
@@ -57,22 +57,24 @@ __global__ void __launch_bounds__(kThreads) est_light_kernel(const int64_t* __re
   }
 }
 
-// one wave per heavy row
+// one wave per heavy row: lanes compute the factors (1 - r) of 64 entries at a time and stage
+// them in LDS; lane j then multiplies chain j through them in row order (the reference's order),
+// reading its factors from LDS (loads issued ahead of the dependent multiplies)
 template <int W>
 __global__ void __launch_bounds__(kThreads) est_heavy_kernel(const int64_t* __restrict__ ptr,
                                                              const int32_t* __restrict__ col,
                                                              const double* __restrict__ Rn, double* __restrict__ Ri,
                                                              const int32_t* __restrict__ heavy,
                                                              const int32_t* __restrict__ nheavy) {
+  __shared__ double F[kThreads / kWave][kWave * (W > 0 ? W : 1)];
   const int lane = lane_id();
   const int32_t nh = *nheavy;
   const int32_t wpb = kThreads / kWave;
+  double* fw = F[threadIdx.x / kWave];
   for (int32_t h = blockIdx.x * wpb + threadIdx.x / kWave; h < nh; h += gridDim.x * wpb) {
     const int32_t u = heavy[h];
     const int64_t p0 = ptr[u], p1 = ptr[u + 1];
-    double acc[W];
-#pragma unroll
-    for (int j = 0; j < W; ++j) acc[j] = 1;
+    double acc = 1.0;  // lane j < W: the running product of chain j
     double f[W];
     {
       const int64_t x = p0 + lane;
@@ -81,35 +83,38 @@ __global__ void __launch_bounds__(kThreads) est_heavy_kernel(const int64_t* __re
       for (int j = 0; j < W; ++j) f[j] = x < p1 ? (1 - r[j]) : 1.0;
     }
     for (int64_t b = p0; b < p1; b += kWave) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) fw[lane * W + j] = f[j];
       // prefetch the next chunk's factors before the dependent multiply chain
-      double fn[W];
       {
         const int64_t x = b + kWave + lane;
         const double* r = Rn + (size_t)(x < p1 ? col[x] : 0) * W;
 #pragma unroll
-        for (int j = 0; j < W; ++j) fn[j] = x < p1 ? (1 - r[j]) : 1.0;
+        for (int j = 0; j < W; ++j) f[j] = x < p1 ? (1 - r[j]) : 1.0;
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const int cnt = (int)((p1 - b) < kWave ? (p1 - b) : kWave);
-      for (int k = 0; k < cnt; ++k) {
-#pragma unroll
-        for (int j = 0; j < W; ++j) acc[j] *= readlane_f64(f[j], k);
+      if (lane < W) {
+        if (cnt == kWave) {
+#pragma unroll 16
+          for (int k = 0; k < kWave; ++k) acc *= fw[k * W + lane];
+        } else {
+          for (int k = 0; k < cnt; ++k) acc *= fw[k * W + lane];
+        }
       }
-      bool all_zero = true;
-#pragma unroll
-      for (int j = 0; j < W; ++j) all_zero &= (acc[j] == 0.0);
-      if (all_zero) break;
-#pragma unroll
-      for (int j = 0; j < W; ++j) f[j] = fn[j];
+      // every chain exactly 0: it stays 0 (exact early exit)
+      const bool live = lane < W && acc != 0.0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (__ballot(live) == 0) break;
     }
-    if (lane == 0) {
-      double* out = Ri + (size_t)u * (W + 1);
-#pragma unroll
-      for (int j = 0; j < W; ++j) out[1 + j] = 1 - acc[j];
-    }
+    if (lane < W) Ri[(size_t)u * (W + 1) + 1 + lane] = 1 - acc;
   }
 }
 
-// per-block partial sums of r_ij[u] * dim_j in a fixed tree order
 struct DimJ {
   double d[kMaxW > 0 ? kMaxW : 1];
 };

@@ -31,6 +31,9 @@ TIMED = {
     "spgemm.numeric.window": r"window_tier_kernel<",
     "spgemm.numeric.bitmap": r"bitmap_tier_kernel<true",
     "spgemm.symbolic.bitmap": r"bitmap_tier_kernel<false",
+    "spgemm.numeric.wavebitmap": r"wave_bitmap_kernel<true",
+    "spgemm.symbolic.wavebitmap": r"wave_bitmap_kernel<false",
+    "batch.executor": r"batch_kernel<",
     "spgemm.numeric.sort": r"sort_tier_kernel<true",
     "spgemm.symbolic.sort": r"sort_tier_kernel<false",
     "spgemm.numeric.wave": r"wave_tier_kernel<true",
@@ -111,9 +114,11 @@ def main(argv=None) -> int:
         with open(a.bench) as f:
             for line in f:
                 line = line.strip()
-                if line.startswith("{") and '"metric"' in line:
-                    bench = json.loads(line)
-    kernel = a.kernel or (bench["roofline"]["kernel"] if bench else "spgemm.numeric.window")
+                if line.startswith("{"):
+                    d = json.loads(line)
+                    # the contract line, or one side measurement (bench.py --only X)
+                    bench = d if "metric" in d else next(iter(d.values()))
+    kernel = a.kernel or (bench["roofline"]["kernel"] if bench and "roofline" in bench else "spgemm.numeric.window")
 
     out = [f"# {a.title}", ""]
     if a.command:
@@ -136,7 +141,7 @@ def main(argv=None) -> int:
             names = match(pat, [r[0] for r in ks]) if pat else []
             calls = sum(r[1] for r in ks if r[0] in names)
             tr = sum(r[2] for r in ks if r[0] in names) / calls / 1e6 if calls else float("nan")
-            ev = v["ms"] / v["launches"] if v["launches"] else float("nan")
+            ev = v["ms"] / v["launches"] if v["launches"] else float("nan")  # per launch
             gb = v.get("bytes_per_launch", 0) / 1e9
             out.append(f"| {tname} | {v['launches']} | {ev:.3f} | {tr:.3f} | {gb:.3f} |")
         out += ["", "The trace also covers the warm-up step; estimator.pass spans several kernels (light and heavy "
