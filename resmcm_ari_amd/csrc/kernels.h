@@ -137,9 +137,10 @@ struct SpgemmOpts {
   int64_t lds_bytes = 65536;   // LDS available per workgroup
   int cus = 256;           // compute units (persistent grid size)
   int force_tier = -1;     // testing: 0 = wave, 1 = LDS sort, 2 = bitmap for every non-empty row
-  int64_t wave_bitmap_cols = 32768;  // rows of products with at most this many columns and
-  int64_t wave_bitmap_cap = 2048;    // 64 < ub <= wave_bitmap_cap: one wave per row over an LDS
-                                     // column bitmap (cols 0 = never)
+  int64_t wave_bitmap_cols = 32768;    // rows of products with at most this many columns and
+  int64_t wave_bitmap_max_ub = 2048;   // 64 < ub <= wave_bitmap_max_ub: one wave per row over an LDS
+                                       // column bitmap (cols 0 = never)
+  int64_t wave_bitmap_cap = 2048;      // LDS value slots per wave; wider rows accumulate in HBM
 };
 
 struct SpgemmStats {

@@ -137,6 +137,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "wave_bitmap_cap") {
       if (v < 64 || v > 65535) fail(SMCM_EINVAL, "wave_bitmap_cap must be in [64, 65535]");
       o.wave_bitmap_cap = v;
+    } else if (k == "wave_bitmap_max_ub") {
+      if (v < 64 || v > 65535) fail(SMCM_EINVAL, "wave_bitmap_max_ub must be in [64, 65535]");
+      o.wave_bitmap_max_ub = v;
     } else if (k == "plan_cost") {
       if (v != 0 && v != 1) fail(SMCM_EINVAL, "plan_cost must be 0 (reference) or 1 (products)");
       ctx->plan_cost = (int)v;
@@ -190,6 +193,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "plan_cost") *v = ctx->plan_cost;
     else if (k == "wave_bitmap_cols") *v = o.wave_bitmap_cols;
     else if (k == "wave_bitmap_cap") *v = o.wave_bitmap_cap;
+    else if (k == "wave_bitmap_max_ub") *v = o.wave_bitmap_max_ub;
     else if (k == "batch_small_products") *v = ctx->batch.small_products;
     else if (k == "batch_threads") *v = ctx->batch.threads;
     else if (k == "batch_sort_cap") *v = ctx->batch.sort_cap;

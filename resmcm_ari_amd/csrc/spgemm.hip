@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
                                                             int32_t m, const int64_t* __restrict__ S,
                                                             const double* __restrict__ est_r, int est_stride,
                                                             double ncols, int s_max, int m_max, int dense_div,
-                                                            int64_t heavy_ub, int force_tier, int wave_bitmap_cap,
+                                                            int64_t heavy_ub, int force_tier, int wave_bitmap_max_ub,
                                                             int32_t* __restrict__ bins, int32_t* __restrict__ counts,
                                                             unsigned long long* __restrict__ sums) {
   const int32_t u = r0 + (int32_t)(blockIdx.x * kThreads + threadIdx.x);
@@ -73,11 +73,11 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
     } else if (force_tier >= 0) {
       tier = force_tier == 0 ? kWaveTier : force_tier == 1 ? kSortTier : kBitmapTier;
       if (tier == kWaveTier && ub > 64) tier = kSortTier;
-      if (tier == kSortTier && ub > (wave_bitmap_cap ? wave_bitmap_cap : m_max)) tier = kBitmapTier;
+      if (tier == kSortTier && ub > (wave_bitmap_max_ub ? wave_bitmap_max_ub : m_max)) tier = kBitmapTier;
     } else if (ub <= s_max) {
       tier = kWaveTier;
-    } else if (wave_bitmap_cap) {
-      tier = ub <= wave_bitmap_cap ? kSortTier : kBitmapTier;
+    } else if (wave_bitmap_max_ub) {
+      tier = ub <= wave_bitmap_max_ub ? kSortTier : kBitmapTier;
     } else {
       bool dense = false;
       if (est_r) dense = est_r[(size_t)u * est_stride] * ncols * (double)dense_div >= ncols;
@@ -466,13 +466,14 @@ __global__ void __launch_bounds__(kThreads) wave_tier_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
-// mid tier for products with few columns (wave bitmap; rows with 64 < ub <= wave_bitmap_cap):
+// mid tier for products with few columns (wave bitmap; rows with 64 < ub <= wave_bitmap_max_ub):
 // one wave per row, four rows per block.
 // The wave's own LDS region holds the row's column bitmap (nw 32-bit words), the exclusive
-// popcount prefix of every word (16-bit) and value slots (cap <= 65535 entries).  Symbolic:
+// popcount prefix of every word (16-bit, widths <= 65535) and cap value / column slots.  Symbolic:
 // mark the columns, count the bits.  Numeric: mark, rank every word, accumulate each product at
 // the rank of its column, list every column at its rank, then write the row in column order
-// with consecutive lanes writing consecutive entries.  No
+// with consecutive lanes writing consecutive entries; a row wider than cap accumulates its values
+// with device atomics at their final positions instead.  No
 // workgroup barrier: the four waves of a block work on four rows independently.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -500,10 +501,11 @@ struct WaveBitmapLayout {
   }
 };
 
-// the mid tier (64 < ub <= wave_bitmap_cap) of a product with at most wave_bitmap_cols columns runs one
+// the mid tier (64 < ub <= wave_bitmap_max_ub) of a product with at most wave_bitmap_cols columns runs one
 // wave per row over an LDS column bitmap; four waves' regions must fit the LDS of a workgroup
 inline bool wave_bitmap_for(const SpgemmOpts& o, int32_t ncols) {
-  if (o.wave_bitmap_cols <= 0 || ncols > o.wave_bitmap_cols || o.wave_bitmap_cap < 64 || o.wave_bitmap_cap > 65535)
+  if (o.wave_bitmap_cols <= 0 || ncols > o.wave_bitmap_cols || o.wave_bitmap_cap < 64 || o.wave_bitmap_cap > 65535 ||
+      o.wave_bitmap_max_ub < 64 || o.wave_bitmap_max_ub > 65535)
     return false;
   const int64_t nw = (ncols + 31) / 32;
   return (int64_t)WaveBitmapLayout::bytes(nw, o.wave_bitmap_cap, true) * (kThreads / kWave) <= o.lds_bytes;
@@ -611,33 +613,55 @@ __global__ void __launch_bounds__(kThreads) wave_bitmap_kernel(
       rank[w] = (uint16_t)run;
       run += __popc(bits[w]);
     }
-    for (int32_t i = lane; i < total; i += kWave) vals[i] = 0ull;
-    wave_lds_sync();
-    wave_products<AV, BV>(rs, re, acol, aval, bptr, bcol, bval, [&](int32_t c, uint64_t v) {
-      const int32_t w = c >> 5;
-      const int32_t r = rank[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
-      atomicAdd((unsigned long long*)&vals[r], (unsigned long long)v);
-    });
-    // every lane lists the columns of its own words at their ranks, then the row leaves in
-    // column order with consecutive lanes writing consecutive entries
-    for (int32_t w = w0; w < w1; ++w) {
-      uint32_t x = bits[w];
-      int32_t r = rank[w];
-      while (x) {
-        cols[r++] = (w << 5) | (__ffs(x) - 1);
-        x &= x - 1u;
-      }
-    }
-    wave_lds_sync();
     const int64_t o = cptr[u];
-    for (int32_t p = lane; p < total; p += kWave) {
-      ccol[o + p] = cols[p];
-      cval[o + p] = vals[p];
+    if (total <= lay.cap) {
+      for (int32_t i = lane; i < total; i += kWave) vals[i] = 0ull;
+      wave_lds_sync();
+      wave_products<AV, BV>(rs, re, acol, aval, bptr, bcol, bval, [&](int32_t c, uint64_t v) {
+        const int32_t w = c >> 5;
+        const int32_t r = rank[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
+        atomicAdd((unsigned long long*)&vals[r], (unsigned long long)v);
+      });
+      // every lane lists the columns of its own words at their ranks, then the row leaves in
+      // column order with consecutive lanes writing consecutive entries
+      for (int32_t w = w0; w < w1; ++w) {
+        uint32_t x = bits[w];
+        int32_t r = rank[w];
+        while (x) {
+          cols[r++] = (w << 5) | (__ffs(x) - 1);
+          x &= x - 1u;
+        }
+      }
+      wave_lds_sync();
+      for (int32_t p = lane; p < total; p += kWave) {
+        ccol[o + p] = cols[p];
+        cval[o + p] = vals[p];
+      }
+    } else {
+      // wider than the LDS value slots: columns written from the bitmap, values accumulated
+      // with device atomics at their final positions (zeroed first)
+      wave_lds_sync();
+      for (int32_t w = w0; w < w1; ++w) {
+        uint32_t x = bits[w];
+        int64_t r = o + rank[w];
+        while (x) {
+          ccol[r++] = (w << 5) | (__ffs(x) - 1);
+          x &= x - 1u;
+        }
+      }
+      for (int32_t i = lane; i < total; i += kWave) cval[o + i] = 0ull;
+      __threadfence();
+      wave_products<AV, BV>(rs, re, acol, aval, bptr, bcol, bval, [&](int32_t c, uint64_t v) {
+        const int32_t w = c >> 5;
+        const int32_t r = rank[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
+        atomicAdd((unsigned long long*)&cval[o + r], (unsigned long long)v);
+      });
     }
   }
 }
 
 // ------------------------------------------------------------------------------------------
+// sort tier: one workgroup per row, products sorted in LDS.  Dynamic LDS: key[P] (+ val[P]).
 template <bool NUMERIC, bool AV, bool BV>
 __global__ void __launch_bounds__(kThreads) sort_tier_kernel(
     const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
@@ -1037,6 +1061,12 @@ __global__ void __launch_bounds__(kThreads) split_wide_kernel(const int32_t* __r
     cls = w > vcap ? 1 : 0;
   }
   const int lane = lane_id();
+  {
+    int32_t wmax = cls == 0 ? (int32_t)w : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+    if (lane == 0 && wmax > 0) atomicMax(&counts[2], wmax);
+  }
   for (int c = 0; c < 2; ++c) {
     const uint64_t mask = __ballot(cls == c);
     if (!mask) continue;
@@ -1068,7 +1098,7 @@ struct TierCounts {
 struct SplitScratch {
   int32_t* narrow = nullptr;
   int32_t* wide = nullptr;
-  int32_t* counts = nullptr;          // 2
+  int32_t* counts = nullptr;          // 3: narrow rows, wide rows, widest narrow row
   unsigned long long* sums = nullptr;  // 6
   int32_t* bnd = nullptr;             // window bound table of B (built on first use)
   WindowLayout wl;
@@ -1126,7 +1156,7 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
   if (nB + nH > 0 && !NUMERIC) {
     fill_i32(queue, 1, 0, st);
     const size_t lds = lay.bytes_sym();
-    const int per_cu = std::max<int64_t>(1, std::min<int64_t>(4, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
+    const int per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / o.d_threads, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
     const unsigned g = (unsigned)std::min<int64_t>(nB + nH, (int64_t)o.cus * per_cu);
     TierCounts both;
     both.rows[0] = nB + nH;
@@ -1141,13 +1171,13 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
   }
   if (nB + nH > 0 && NUMERIC) {
     // split by exact width: narrow rows keep the LDS bitmap + rank kernel, wide rows use windows
-    fill_i32(split->counts, 2, 0, st);
+    fill_i32(split->counts, 3, 0, st);
     fill_i64((int64_t*)split->sums, 6, 0, st);
     split_wide_kernel<<<(unsigned)ceil_div(nB + nH, kThreads), kThreads, 0, st>>>(
         bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m, nB, A.ptr, S, cptr, lay.vcap,
         split->narrow, split->wide, split->counts, split->sums);
     SMCM_LAUNCH_CHECK();
-    int32_t cnt[2];
+    int32_t cnt[3];
     unsigned long long sums[6];
     SMCM_HIP(hipMemcpyAsync(cnt, split->counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipMemcpyAsync(sums, split->sums, sizeof(sums), hipMemcpyDeviceToHost, st));
@@ -1161,13 +1191,16 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     }
     if (cnt[0] > 0) {
       fill_i32(queue, 1, 0, st);
-      const size_t lds = lay.bytes_num();
-      const int per_cu = std::max<int64_t>(1, std::min<int64_t>(4, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
+      // value slots for the widest narrow row only: a smaller LDS footprint, more workgroups per CU
+      BitmapLayout nl = lay;
+      nl.vcap = std::min<int64_t>(lay.vcap, std::max<int64_t>(64, ((int64_t)cnt[2] + 63) / 64 * 64));
+      const size_t lds = nl.bytes_num();
+      const int per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / o.d_threads, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
       const unsigned g = (unsigned)std::min<int64_t>(cnt[0], (int64_t)o.cus * per_cu);
       const int tok = T.start("spgemm.numeric.bitmap", st);
       bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val,
                                                                        nullptr, 0, split->narrow, cnt[0], B.ncols,
-                                                                       lay.nw, lay.vcap, queue, row_nnz, cptr, ccol,
+                                                                       nl.nw, nl.vcap, queue, row_nnz, cptr, ccol,
                                                                        cval);
       SMCM_LAUNCH_CHECK();
       T.stop(tok, st, tier_bytes(true, cls, 0, AV, BV));
@@ -1313,7 +1346,7 @@ struct SpgemmCore {
     split.wl = choose_window(B.ncols, o);
     set_all_lds_attrs(std::max({lay.bytes_num(), lay.bytes_sym(), split.wl.bytes()}));
     split_rows = DevBuf<int32_t>(al, 2 * (int64_t)(A.nrows > 0 ? A.nrows : 1));
-    split_counts = DevBuf<int32_t>(al, 2);
+    split_counts = DevBuf<int32_t>(al, 3);
     split_sums = DevBuf<unsigned long long>(al, 6);
     split.narrow = split_rows.p;
     split.wide = split_rows.p + (A.nrows > 0 ? A.nrows : 1);
@@ -1331,7 +1364,7 @@ struct SpgemmCore {
     if (r1 > r0) {
       bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, kThreads), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
-          o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_cap : 0, bins.p, counts.p, sums.p);
+          o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_max_ub : 0, bins.p, counts.p, sums.p);
       SMCM_LAUNCH_CHECK();
     }
     unsigned long long sh[2 * kNumBins];

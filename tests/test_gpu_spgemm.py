@@ -55,7 +55,8 @@ def test_spgemm_tiers(ctx, case, tier, vals):
 @pytest.mark.parametrize("shape", ["long", "short"])
 def test_spgemm_wave_bitmap_shapes(ctx, cap, n, shape):
     """Wave-bitmap mid tier at word boundaries and LDS limits; A rows of few long B segments and
-    of many short ones (lane-walked and wave-walked segments)."""
+    of many short ones (lane-walked and wave-walked segments); rows wider than cap accumulate in
+    device memory."""
     rng = np.random.default_rng(cap * 7 + n + (shape == "long"))
     if shape == "long":
         A = random_csr(rng, 300, 120, row_len=lambda r: r.integers(0, 12), with_vals=True)
@@ -64,14 +65,16 @@ def test_spgemm_wave_bitmap_shapes(ctx, cap, n, shape):
         A = random_csr(rng, 300, 2000, row_len=lambda r: r.integers(0, 400), with_vals=True)
         B = random_csr(rng, 2000, n, row_len=lambda r: r.integers(0, min(n, 9) + 1), with_vals=True)
     ref = O.spgemm(A, B)
-    saved = ctx.get_option("wave_bitmap_cap")
+    saved = ctx.get_option("wave_bitmap_cap"), ctx.get_option("wave_bitmap_max_ub")
     ctx.set_option("wave_bitmap_cap", cap)
+    ctx.set_option("wave_bitmap_max_ub", 32768)
     ctx.set_option("force_tier", 1)
     try:
         ptr, col, val = _run(ctx, A, B)
     finally:
         ctx.set_option("force_tier", -1)
-        ctx.set_option("wave_bitmap_cap", saved)
+        ctx.set_option("wave_bitmap_cap", saved[0])
+        ctx.set_option("wave_bitmap_max_ub", saved[1])
     assert_same_csr(ptr, col, val, ref, what=f"wave bitmap n={n} cap={cap} {shape}")
 
 
