@@ -81,6 +81,14 @@ void hin_free(hin_graph* g);
  * Writes up to cap vertex types and cap-1 edge types; *L receives the number of matrices.
  * Returns -EINVAL when the counts do not satisfy #vtypes == #etypes + 1. */
 int hin_parse_metapath(const char* s, int32_t* vtypes, int32_t* etypes, int32_t cap, int32_t* L);
+/* Every meta-path with L edges over a schema, as tool/generate_DBLP.cpp:286-304 (and
+ * generate_FourSquare.cpp / generate_IMDB.cpp) enumerate them: depth-first from each start type in
+ * starts[] order, neighbours of a type in the order of the relations (rel_src[r] --rel_et[r]-->
+ * rel_dst[r]) listed for it.  Writes at most cap meta-paths (out_vts: cap*(L+1), out_ets: cap*L);
+ * *count receives the total. */
+int hin_enumerate_metapaths(const int32_t* rel_src, const int32_t* rel_et, const int32_t* rel_dst,
+                            int32_t nrel, const int32_t* starts, int32_t nstarts, int32_t L,
+                            int32_t* out_vts, int32_t* out_ets, int64_t cap, int64_t* count);
 /* "vt-et-vt-...-vt" as MetaPath::toString; returns the length written (excluding NUL). */
 int hin_metapath_to_string(const int32_t* vtypes, const int32_t* etypes, int32_t L, char* buf,
                            int32_t cap);

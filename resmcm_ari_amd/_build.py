@@ -85,6 +85,9 @@ def build_cli() -> str:
     srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES]
     _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, *srcs, "-o", out,
           "-L" + LIBDIR, "-lsmcm", "-lsmcm_hin", "-Wl,-rpath,$ORIGIN"])
+    # meta-path enumeration (host only)
+    _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(CSRC, "cli_metapaths.cpp"), "-o",
+          os.path.join(LIBDIR, "smcm_metapaths"), "-L" + LIBDIR, "-lsmcm_hin", "-Wl,-rpath,$ORIGIN"])
     return out
 
 

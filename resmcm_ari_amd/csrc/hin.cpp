@@ -9,6 +9,7 @@
 #include "smcm_hin.h"
 
 #include <algorithm>
+#include <functional>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -442,6 +443,41 @@ int hin_parse_metapath(const char* s, int32_t* vtypes, int32_t* etypes, int32_t 
   for (size_t k = 0; k < vt.size(); ++k) if (vtypes) vtypes[k] = vt[k];
   for (size_t k = 0; k < et.size(); ++k) if (etypes) etypes[k] = et[k];
   *L = nmat;
+  return 0;
+}
+
+int hin_enumerate_metapaths(const int32_t* rel_src, const int32_t* rel_et, const int32_t* rel_dst,
+                            int32_t nrel, const int32_t* starts, int32_t nstarts, int32_t L,
+                            int32_t* out_vts, int32_t* out_ets, int64_t cap, int64_t* count) {
+  if (!count || nrel < 0 || nstarts < 0 || L < 0 || (nrel > 0 && (!rel_src || !rel_et || !rel_dst)) ||
+      (nstarts > 0 && !starts) || (cap > 0 && (!out_vts || (L > 0 && !out_ets)))) {
+    g_err = "bad arguments";
+    return -22;
+  }
+  int64_t n = 0;
+  std::vector<int32_t> vts(L + 1), ets(L > 0 ? L : 1);
+  // depth-first, neighbours in relation order (recursion depth is L)
+  std::function<void(int32_t)> dfs = [&](int32_t depth) {
+    if (depth == L) {
+      if (n < cap) {
+        std::copy(vts.begin(), vts.end(), out_vts + n * (L + 1));
+        if (L > 0) std::copy(ets.begin(), ets.begin() + L, out_ets + n * L);
+      }
+      ++n;
+      return;
+    }
+    for (int32_t r = 0; r < nrel; ++r) {
+      if (rel_src[r] != vts[depth]) continue;
+      ets[depth] = rel_et[r];
+      vts[depth + 1] = rel_dst[r];
+      dfs(depth + 1);
+    }
+  };
+  for (int32_t i = 0; i < nstarts; ++i) {
+    vts[0] = starts[i];
+    dfs(0);
+  }
+  *count = n;
   return 0;
 }
 

@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
         L.hin_free.restype = None
         L.hin_parse_metapath.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
         L.hin_metapath_to_string.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
+        L.hin_enumerate_metapaths.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                              C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
         L.hin_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -192,3 +194,34 @@ def read_metapathesN(file: str) -> list[MetaPath]:
             except ValueError:
                 break
     return paths
+
+
+# schemas of the reference's meta-path generators: start types in order, then relations
+# (src type, edge type, dst type) in each type's neighbour order (tool/generate_DBLP.cpp:325-358,
+# tool/generate_FourSquare.cpp:335-366, tool/generate_IMDB.cpp:304-328)
+SCHEMAS = {
+    "DBLP": ([0, 1, 2, 3], [(0, 0, 1), (0, 1, 2), (0, 2, 3), (1, 3, 0), (2, 4, 0), (3, 5, 0)]),
+    "FourSquare": ([0, 1, 2, 3, 4], [(0, 0, 1), (0, 2, 2), (0, 4, 3), (0, 6, 4), (1, 1, 0), (2, 3, 0), (3, 5, 0),
+                                     (4, 7, 0)]),
+    "IMDB": ([0, 1, 2, 3], [(0, 0, 1), (0, 2, 2), (0, 4, 3), (1, 1, 0), (2, 3, 0), (3, 5, 0)]),
+}
+
+
+def enumerate_metapaths(schema, L: int) -> list[MetaPath]:
+    """Every meta-path with L edges over a schema (name in SCHEMAS or (starts, relations)), in the
+    order of the reference's generators."""
+    starts, rels = SCHEMAS[schema] if isinstance(schema, str) else schema
+    src = np.array([r[0] for r in rels], np.int32)
+    et = np.array([r[1] for r in rels], np.int32)
+    dst = np.array([r[2] for r in rels], np.int32)
+    st = np.array(starts, np.int32)
+    cnt = C.c_int64()
+    _check(lib().hin_enumerate_metapaths(src.ctypes.data, et.ctypes.data, dst.ctypes.data, len(rels), st.ctypes.data,
+                                         len(st), L, None, None, 0, C.byref(cnt)))
+    n = cnt.value
+    vts = np.zeros(max(n * (L + 1), 1), np.int32)
+    ets = np.zeros(max(n * L, 1), np.int32)
+    _check(lib().hin_enumerate_metapaths(src.ctypes.data, et.ctypes.data, dst.ctypes.data, len(rels), st.ctypes.data,
+                                         len(st), L, vts.ctypes.data, ets.ctypes.data, n, C.byref(cnt)))
+    return [MetaPath(vertexTypes=vts[i * (L + 1):(i + 1) * (L + 1)].tolist(), edgeTypes=ets[i * L:(i + 1) * L].tolist())
+            for i in range(n)]

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from resmcm_ari_amd import configs
+from resmcm_ari_amd import hin
 from resmcm_ari_amd.hin import HeteroGraph, MetaPath, read_metapathesN
 
 
@@ -95,3 +96,30 @@ def test_random_walk_paths_are_schema_valid():
         for i in range(L):
             assert (mp.vertexTypes[i], mp.vertexTypes[i + 1]) in configs.BIB_EDGE
             assert configs.BIB_EDGE[(mp.vertexTypes[i], mp.vertexTypes[i + 1])] == mp.edgeTypes[i]
+
+
+@pytest.mark.parametrize("ds", ["DBLP", "FourSquare", "IMDB"])
+@pytest.mark.parametrize("L", [2, 3, 4, 5, 6, 7])
+def test_enumerate_metapaths_matches_reference_files(ds, L):
+    """The enumerator reproduces the reference's meta-path files line for line (the files of
+    materials/input/metapath/, kept as fixtures)."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "metapath", f"{ds}-l{L}.txt")
+    if not os.path.exists(path):
+        pytest.skip(f"{ds}-l{L} has no reference file")
+    with open(path) as f:
+        want = [line.strip() for line in f if line.strip()]
+    got = [mp.toString() for mp in hin.enumerate_metapaths(ds, L)]
+    assert got == want
+
+
+def test_metapaths_program(tmp_path):
+    """The drop-in enumerator program writes the reference's file for FourSquare, L = 3."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(__file__)), "resmcm_ari_amd", "lib", "smcm_metapaths")
+    if not os.path.exists(exe):
+        pytest.skip("smcm_metapaths not built")
+    out = tmp_path / "fs3.txt"
+    subprocess.run([exe, "3", "FourSquare", str(out)], check=True, capture_output=True)
+    ref = os.path.join(os.path.dirname(__file__), "golden", "metapath", "FourSquare-l3.txt")
+    assert out.read_text() == open(ref).read()
