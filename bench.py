@@ -18,6 +18,7 @@ Beside it, on the same GPU in the same run:
   config1  the 20k/30k/200 graph's A-P-V-P-A chain: SpGEMM kernels' algorithmic bytes / time;
   config3  the long location meta-paths over all 100k users (results of 1e10 entries streamed in
            row blocks), reference plan and product plan;
+  vendor   rocSPARSE's SpGEMM (binary64 values) on the final products of A-P-V-P-A, configs 1 and 2;
   batched  4096 chains (L 2-6, log-uniform sources 16..65536) over that graph with T=20k:
            smcm_batch_run vs the per-chain path, queries/s, every result's digests compared;
   served   a Poisson stream over that graph and the 100k-user location graph (L 2-10, sources
@@ -173,6 +174,65 @@ def bench_config3(block_gb: float = 32.0) -> dict:
     return out
 
 
+def bench_vendor(hg1, hg2=None) -> dict:
+    """The final products of A-P-V-P-A (config 1, and config 2 when given): this library's SpGEMM
+    kernels (HIP-event time, uint64 counts) against rocSPARSE's generic SpGEMM (binary64 values),
+    same operands; the structures of both results are compared."""
+    from resmcm_ari_amd import vendor
+
+    out = {"note": "rocSPARSE times its nnz and compute stages; ours the analysis, symbolic and numeric kernels "
+                   "(allocations excluded in both)", "cases": []}
+    ctx = smcm.Context(0)
+    for name, hg, reps in (("config1", hg1, 5), ("config2", hg2, 2)):
+        if hg is None:
+            continue
+        G = smcm.Graph(ctx, hg)
+        # operands: X = A-P-V (left) and Y = V-P-A (right), as the DP's plan builds them
+        X, _ = G.run(configs.metapath_from_letters("A-P-V", "bib"))
+        Y, _ = G.run(configs.metapath_from_letters("V-P-A", "bib"))
+        xp, xc, xv = X.to_host()
+        yp, yc, yv = Y.to_host()
+        off = hg.type_offset
+        xc = (xc - off[configs.V]).astype(np.int32)  # type-local columns
+        yc = (yc - off[configs.A]).astype(np.int32)
+        X.free()
+        Y.free()
+        m, k, n = len(xp) - 1, len(yp) - 1, int(hg.dimensions[configs.A])
+        ctx.set_option("kernel_timing", 1)
+        ms_ours = []
+        ours = None
+        for r in range(reps + 1):
+            ctx.kernel_stats_reset()
+            res = ctx.spgemm_csr(m, k, n, xp, xc, xv, yp, yc, yv)
+            ks = ctx.kernel_stats()
+            if r > 0:
+                ms_ours.append(sum(v[0] for kk, v in ks.items() if kk.startswith("spgemm.")))
+            if r == reps and name == "config1":
+                ours = res.to_host()
+            nnz_ours = res.nnz
+            res.free()
+        ctx.set_option("kernel_timing", 0)
+        ms_v, nnz_v, vres = vendor.spgemm_f64(m, k, n, xp, xc, xv.astype(np.float64), yp, yc,
+                                              yv.astype(np.float64), reps=reps, want_result=(name == "config1"))
+        same = nnz_v == nnz_ours
+        if vres is not None and ours is not None:
+            vp, vc, vv = vres
+            gp, gc, gv = ours
+            # rocSPARSE rows may come unsorted: compare each row's column set
+            same = same and np.array_equal(vp, gp)
+            if same:
+                order = np.lexsort((vc, np.repeat(np.arange(m), np.diff(vp))))
+                same = np.array_equal(vc[order], gc.astype(np.int32))  # both type-local
+        t = float(np.median(ms_ours))
+        out["cases"].append({"case": f"{name} A-P-V * V-P-A ({m}x{k} * {k}x{n})", "nnz": nnz_ours,
+                             "ours_ms": round(t, 3), "rocsparse_ms": round(ms_v, 3),
+                             "speedup": round(ms_v / t, 2) if t else None, "same_structure": bool(same)})
+        print(f"[bench] vendor {name}: ours {t:.3f} ms, rocSPARSE {ms_v:.3f} ms", file=sys.stderr, flush=True)
+        G.close()
+    ctx.close()
+    return out
+
+
 def bench_batched(hg1, n_chains: int = 4096) -> dict:
     """Workload 4: the batched form against the per-chain path on the same chains."""
     ctx = smcm.Context(0)
@@ -261,7 +321,7 @@ def main() -> int:
     ap.add_argument("--batch-chains", type=int, default=4096)
     ap.add_argument("--serve-queries", type=int, default=128)
     ap.add_argument("--serve-qps", type=float, default=5.0)
-    ap.add_argument("--only", choices=["config1", "config3", "batched", "served"],
+    ap.add_argument("--only", choices=["config1", "config3", "batched", "served", "vendor"],
                     help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -272,6 +332,7 @@ def main() -> int:
         hg1 = configs.CFG1.build() if args.only != "config3" else None
         res = {"config1": lambda: bench_config1(hg1), "config3": lambda: bench_config3(),
                "batched": lambda: bench_batched(hg1, args.batch_chains),
+               "vendor": lambda: bench_vendor(hg1, configs.CFG2.build()),
                "served": lambda: bench_served(hg1, args.serve_queries, args.serve_qps)}[args.only]()
         print(json.dumps({args.only: res}))
         return 0
@@ -342,6 +403,7 @@ def main() -> int:
         hg1 = configs.CFG1.build()
         side["config1"] = bench_config1(hg1)
         side["config3"] = bench_config3()
+        side["vendor"] = bench_vendor(hg1, hg)
         side["batched"] = bench_batched(hg1, args.batch_chains)
         side["served"] = bench_served(hg1, args.serve_queries, args.serve_qps)
 

@@ -91,6 +91,14 @@ def build_cli() -> str:
     return out
 
 
+def build_vendor() -> str:
+    """rocSPARSE baseline library (timing comparisons only; the product path never loads it)."""
+    out = os.path.join(LIBDIR, "libsmcm_vendor.so")
+    _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", os.path.join(CSRC, "vendor_spgemm.cpp"), "-o", out,
+          "-lrocsparse"])
+    return out
+
+
 def build_oracle(with_reference: bool | None = None) -> None:
     """Compile the oracle's C restatement; also the reference's own programs from their
     sources when the reference tree is present (only in the build container)."""

@@ -1,0 +1,143 @@
+// vendor_spgemm.cpp - rocSPARSE's generic SpGEMM on the same operands, as a timing baseline.
+//
+// Built into its own library (libsmcm_vendor.so); the product path never uses it.  rocSPARSE has
+// no integer value types, so values are binary64 (path counts are exact below 2^53) and the
+// structure of C is the comparison that is exact.  Times: the nnz and compute stages (buffer and
+// output allocations made once, outside the timed region), HIP events on one stream.
+#include <hip/hip_runtime.h>
+#include <rocsparse/rocsparse.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+extern "C" {
+
+// status 0 ok; negative: a HIP or rocSPARSE step failed (message in err, cap bytes)
+int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, const int32_t* a_col,
+                       const double* a_val, const int64_t* b_ptr, const int32_t* b_col, const double* b_val,
+                       int32_t reps, double* ms_per_product, int64_t* c_nnz, int64_t* c_ptr_out,
+                       int32_t* c_col_out, double* c_val_out, char* err, int32_t cap);
+}
+
+namespace {
+
+struct Fail {
+  std::string msg;
+};
+
+#define V_HIP(x)                                                         \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) throw Fail{std::string(#x) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+#define V_SP(x)                                                          \
+  do {                                                                   \
+    rocsparse_status s_ = (x);                                           \
+    if (s_ != rocsparse_status_success) throw Fail{std::string(#x) + ": status " + std::to_string((int)s_)}; \
+  } while (0)
+
+template <class T>
+T* dev_copy(const T* h, size_t n) {
+  T* d = nullptr;
+  V_HIP(hipMalloc(&d, sizeof(T) * (n ? n : 1)));
+  if (n) V_HIP(hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+  return d;
+}
+
+}  // namespace
+
+int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, const int32_t* a_col,
+                       const double* a_val, const int64_t* b_ptr, const int32_t* b_col, const double* b_val,
+                       int32_t reps, double* ms_per_product, int64_t* c_nnz, int64_t* c_ptr_out,
+                       int32_t* c_col_out, double* c_val_out, char* err, int32_t cap) {
+  std::vector<void*> dev;
+  rocsparse_handle h = nullptr;
+  rocsparse_spmat_descr A = nullptr, B = nullptr, C = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  try {
+    if (m < 0 || k < 0 || n < 0 || reps < 1) throw Fail{"bad arguments"};
+    const int64_t annz = a_ptr[m], bnnz = b_ptr[k];
+    auto keep = [&](void* p) {
+      dev.push_back(p);
+      return p;
+    };
+    void* dap = keep(dev_copy(a_ptr, (size_t)m + 1));
+    void* dac = keep(dev_copy(a_col, (size_t)annz));
+    void* dav = keep(dev_copy(a_val, (size_t)annz));
+    void* dbp = keep(dev_copy(b_ptr, (size_t)k + 1));
+    void* dbc = keep(dev_copy(b_col, (size_t)bnnz));
+    void* dbv = keep(dev_copy(b_val, (size_t)bnnz));
+    V_HIP(hipStreamCreate(&st));
+    V_SP(rocsparse_create_handle(&h));
+    V_SP(rocsparse_set_stream(h, st));
+    const auto i64 = rocsparse_indextype_i64, i32 = rocsparse_indextype_i32;
+    const auto f64 = rocsparse_datatype_f64_r;
+    V_SP(rocsparse_create_csr_descr(&A, m, k, annz, dap, dac, dav, i64, i32, rocsparse_index_base_zero, f64));
+    V_SP(rocsparse_create_csr_descr(&B, k, n, bnnz, dbp, dbc, dbv, i64, i32, rocsparse_index_base_zero, f64));
+    void* dcp = keep(nullptr);
+    V_HIP(hipMalloc(&dcp, sizeof(int64_t) * ((size_t)m + 1)));
+    dev.back() = dcp;
+    V_SP(rocsparse_create_csr_descr(&C, m, n, 0, dcp, nullptr, nullptr, i64, i32, rocsparse_index_base_zero, f64));
+    const double alpha = 1.0;
+    size_t bsize = 0;
+    V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, nullptr, nullptr, C,
+                          f64, rocsparse_spgemm_alg_default, rocsparse_spgemm_stage_buffer_size, &bsize, nullptr));
+    void* buf = keep(nullptr);
+    V_HIP(hipMalloc(&buf, bsize ? bsize : 1));
+    dev.back() = buf;
+    V_HIP(hipEventCreate(&e0));
+    V_HIP(hipEventCreate(&e1));
+    void* dcc = nullptr;
+    void* dcv = nullptr;
+    int64_t nnz = 0;
+    float total_ms = 0;
+    for (int32_t r = 0; r <= reps; ++r) {  // r = 0: warm-up (and the output allocation)
+      V_HIP(hipEventRecord(e0, st));
+      V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, nullptr, nullptr, C,
+                            f64, rocsparse_spgemm_alg_default, rocsparse_spgemm_stage_nnz, &bsize, buf));
+      if (r == 0) {
+        int64_t rr, cc;
+        V_HIP(hipStreamSynchronize(st));
+        V_SP(rocsparse_spmat_get_size(C, &rr, &cc, &nnz));
+        V_HIP(hipMalloc(&dcc, sizeof(int32_t) * (nnz ? nnz : 1)));
+        dev.push_back(dcc);
+        V_HIP(hipMalloc(&dcv, sizeof(double) * (nnz ? nnz : 1)));
+        dev.push_back(dcv);
+        V_SP(rocsparse_csr_set_pointers(C, dcp, dcc, dcv));
+        V_HIP(hipEventRecord(e0, st));
+      }
+      V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, nullptr, nullptr, C,
+                            f64, rocsparse_spgemm_alg_default, rocsparse_spgemm_stage_compute, &bsize, buf));
+      V_HIP(hipEventRecord(e1, st));
+      V_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      V_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) total_ms += ms;
+    }
+    *ms_per_product = total_ms / reps;
+    *c_nnz = nnz;
+    if (c_ptr_out) V_HIP(hipMemcpy(c_ptr_out, dcp, sizeof(int64_t) * ((size_t)m + 1), hipMemcpyDeviceToHost));
+    if (c_col_out && nnz) V_HIP(hipMemcpy(c_col_out, dcc, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    if (c_val_out && nnz) V_HIP(hipMemcpy(c_val_out, dcv, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+  } catch (const Fail& f) {
+    if (err && cap > 0) {
+      std::strncpy(err, f.msg.c_str(), (size_t)cap - 1);
+      err[cap - 1] = 0;
+    }
+    rc = -5;
+  }
+  if (A) rocsparse_destroy_spmat_descr(A);
+  if (B) rocsparse_destroy_spmat_descr(B);
+  if (C) rocsparse_destroy_spmat_descr(C);
+  if (h) rocsparse_destroy_handle(h);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
+}
