@@ -552,7 +552,7 @@ int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_
     B.val = b_val ? (uint64_t*)up(b_val, sizeof(uint64_t) * bnnz) : nullptr;
     smcm::SpgemmStats ss;
     try {
-      smcm::spgemm(A, B, nullptr, 0, C, ctx->opts, &ss, al, true);
+      smcm::spgemm(A, B, nullptr, 0, C, ctx->opts, &ss, al, true, ctx->timer.enabled ? &ctx->timer : nullptr);
     } catch (...) {
       free_csr(al, A);
       free_csr(al, B);

@@ -54,7 +54,7 @@ int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, co
                        int32_t* c_col_out, double* c_val_out, char* err, int32_t cap) {
   std::vector<void*> dev;
   rocsparse_handle h = nullptr;
-  rocsparse_spmat_descr A = nullptr, B = nullptr, C = nullptr;
+  rocsparse_spmat_descr A = nullptr, B = nullptr, C = nullptr, D = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
@@ -82,9 +82,15 @@ int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, co
     V_HIP(hipMalloc(&dcp, sizeof(int64_t) * ((size_t)m + 1)));
     dev.back() = dcp;
     V_SP(rocsparse_create_csr_descr(&C, m, n, 0, dcp, nullptr, nullptr, i64, i32, rocsparse_index_base_zero, f64));
-    const double alpha = 1.0;
+    // D: an empty m x n matrix (beta = 0)
+    std::vector<int64_t> zeros((size_t)m + 1, 0);
+    void* ddp = keep(dev_copy(zeros.data(), zeros.size()));
+    void* ddc = keep(dev_copy(a_col, 1));
+    void* ddv = keep(dev_copy(a_val, 1));
+    V_SP(rocsparse_create_csr_descr(&D, m, n, 0, ddp, ddc, ddv, i64, i32, rocsparse_index_base_zero, f64));
+    const double alpha = 1.0, beta = 0.0;
     size_t bsize = 0;
-    V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, nullptr, nullptr, C,
+    V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, &beta, D, C,
                           f64, rocsparse_spgemm_alg_default, rocsparse_spgemm_stage_buffer_size, &bsize, nullptr));
     void* buf = keep(nullptr);
     V_HIP(hipMalloc(&buf, bsize ? bsize : 1));
@@ -97,7 +103,7 @@ int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, co
     float total_ms = 0;
     for (int32_t r = 0; r <= reps; ++r) {  // r = 0: warm-up (and the output allocation)
       V_HIP(hipEventRecord(e0, st));
-      V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, nullptr, nullptr, C,
+      V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, &beta, D, C,
                             f64, rocsparse_spgemm_alg_default, rocsparse_spgemm_stage_nnz, &bsize, buf));
       if (r == 0) {
         int64_t rr, cc;
@@ -110,7 +116,7 @@ int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, co
         V_SP(rocsparse_csr_set_pointers(C, dcp, dcc, dcv));
         V_HIP(hipEventRecord(e0, st));
       }
-      V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, nullptr, nullptr, C,
+      V_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, &beta, D, C,
                             f64, rocsparse_spgemm_alg_default, rocsparse_spgemm_stage_compute, &bsize, buf));
       V_HIP(hipEventRecord(e1, st));
       V_HIP(hipEventSynchronize(e1));
@@ -133,6 +139,7 @@ int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, co
   if (A) rocsparse_destroy_spmat_descr(A);
   if (B) rocsparse_destroy_spmat_descr(B);
   if (C) rocsparse_destroy_spmat_descr(C);
+  if (D) rocsparse_destroy_spmat_descr(D);
   if (h) rocsparse_destroy_handle(h);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
