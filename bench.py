@@ -242,6 +242,8 @@ def bench_batched(hg1, n_chains: int = 4096) -> dict:
     paths = [q.path for q in batch]
     srcs = [(q.src_rows + off[q.path.vertexTypes[0]]).astype(np.int32) for q in batch]
     G.run_batch(paths[:32], srcs[:32])  # typed matrices, code objects
+    tune = G.autotune_batch(paths[:256], srcs[:256])  # on a sample, before the timed run
+    print(f"[bench] batched: autotuned in {tune['trials']} trials", file=sys.stderr, flush=True)
     ctx.sync()
     t0 = time.perf_counter()
     nnz, dig, _, st = G.run_batch(paths, srcs, digests=True)
@@ -266,6 +268,7 @@ def bench_batched(hg1, n_chains: int = 4096) -> dict:
             "queries_per_s": round(n_chains / tb, 2), "plain_queries_per_s": round(n_chains / tp, 2),
             "speedup": round(tp / tb, 2), "seconds": round(tb, 3), "plain_seconds": round(tp, 3),
             "mismatching_chains": bad, "jobs": st["jobs"], "executor_jobs": st["small_jobs"],
+            "autotuned": {k: tune[k] for k in ("small_products", "threads", "sort_cap", "tile_rows", "trials")},
             "result_nnz": int(nnz.sum())}
 
 

@@ -163,6 +163,19 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
                    const int32_t* src_rows, int64_t* nnz, uint64_t* row_digest,
                    smcm_result** results, smcm_batch_stats* stats);
 
+typedef struct smcm_batch_tuning {
+  double baseline_s, best_s;     /* sample time with the settings on entry / the chosen ones */
+  int64_t small_products;        /* chosen options (set on the context) */
+  int32_t threads, sort_cap, tile_rows, trials;
+} smcm_batch_tuning;
+/* Autotuner of the batched form: runs smcm_batch_run on the given chains (a sample of the
+ * workload), best of reps runs per setting: first the executor threshold (batch_small_products),
+ * then the executor shape (batch_threads, batch_sort_cap, batch_tile_rows).  Leaves the fastest
+ * settings on ctx and reports them. */
+int smcm_batch_autotune(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t* path_len,
+                        const int32_t* vtypes, const int32_t* etypes, const int64_t* src_off,
+                        const int32_t* src_rows, int32_t reps, smcm_batch_tuning* out);
+
 /* ---- serving form (no counterpart in the reference) --------------------------------------
  * A request loop over graphs resident in HBM.  Worker threads (slots), each with its own HIP
  * stream and allocator, take submitted queries as they arrive; each query is planned (estimator

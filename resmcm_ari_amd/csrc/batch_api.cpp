@@ -582,3 +582,74 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Autotuner of the batched form: times smcm_batch_run on the given (sample) chains for the
+// small-job threshold and then the executor shapes, keeping the fastest setting of each stage.
+int smcm_batch_autotune(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t* path_len,
+                        const int32_t* vtypes, const int32_t* etypes, const int64_t* src_off,
+                        const int32_t* src_rows, int32_t reps, smcm_batch_tuning* out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (reps < 1) reps = 1;
+    auto saved = ctx->batch;
+    auto time_it = [&]() {
+      double best = 1e300;
+      for (int32_t r = 0; r < reps; ++r) {
+        const double t0 = now_s();
+        const int rc = smcm_batch_run(ctx, g, n_chains, path_len, vtypes, etypes, src_off, src_rows, nullptr, nullptr,
+                                      nullptr, nullptr);
+        if (rc != SMCM_OK) return -1.0;  // this setting does not run (e.g. LDS)
+        best = std::min(best, now_s() - t0);
+      }
+      return best;
+    };
+    smcm_batch_tuning t{};
+    t.trials = 0;
+    // 1. which jobs go to the executor
+    const int64_t thresholds[] = {0, 1 << 20, 1 << 22, 1 << 24, 1 << 25, 1 << 27};
+    double best = time_it();
+    t.baseline_s = best;
+    int64_t best_thr = ctx->batch.small_products;
+    for (int64_t thr : thresholds) {
+      ctx->batch.small_products = thr;
+      const double s = time_it();
+      t.trials++;
+      if (s >= 0 && s < best) {
+        best = s;
+        best_thr = thr;
+      }
+    }
+    ctx->batch.small_products = best_thr;
+    // 2. executor shape (only matters when jobs use the executor)
+    if (best_thr > 0) {
+      const int shapes[][2] = {{256, 512}, {256, 1024}, {256, 2048}, {512, 512}, {512, 1024}};
+      const int tiles[] = {8, 16, 32, 64};
+      auto bestb = ctx->batch;
+      for (auto& sh : shapes) {
+        for (int tr : tiles) {
+          ctx->batch.threads = sh[0];
+          ctx->batch.sort_cap = sh[1];
+          ctx->batch.tile_rows = tr;
+          const double s = time_it();
+          t.trials++;
+          if (s >= 0 && s < best) {
+            best = s;
+            bestb = ctx->batch;
+          }
+        }
+      }
+      ctx->batch = bestb;
+    }
+    (void)saved;
+    t.best_s = best;
+    t.small_products = ctx->batch.small_products;
+    t.threads = ctx->batch.threads;
+    t.sort_cap = ctx->batch.sort_cap;
+    t.tile_rows = ctx->batch.tile_rows;
+    if (out) *out = t;
+  });
+}
+
+}  // extern "C"

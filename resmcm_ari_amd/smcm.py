@@ -33,7 +33,7 @@ EXPORTS = [
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
     "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth", "smcm_chain_products_estimate",
     "smcm_batch_run", "smcm_server_create", "smcm_server_destroy", "smcm_server_submit", "smcm_server_poll",
-    "smcm_server_stats", "smcm_server_replay",
+    "smcm_server_stats", "smcm_server_replay", "smcm_batch_autotune",
 ]
 
 
@@ -52,6 +52,11 @@ class BatchStats(C.Structure):
                 ("rerun_jobs", C.c_int32), ("groups", C.c_int32), ("nodes", C.c_int64),
                 ("tiles", C.c_int64), ("products_est", C.c_int64), ("plan_s", C.c_double), ("exec_s", C.c_double),
                 ("large_s", C.c_double), ("total_s", C.c_double)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("baseline_s", C.c_double), ("best_s", C.c_double), ("small_products", C.c_int64),
+                ("threads", C.c_int32), ("sort_cap", C.c_int32), ("tile_rows", C.c_int32), ("trials", C.c_int32)]
 
 
 class ServerOpts(C.Structure):
@@ -99,6 +104,7 @@ def load() -> C.CDLL:
             "smcm_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_chain_products_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_batch_run": [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            "smcm_batch_autotune": [vp, vp, i32, vp, vp, vp, vp, vp, i32, vp],
             "smcm_server_create": [vp, vp, i32, vp, vp],
             "smcm_server_destroy": [vp],
             "smcm_server_submit": [vp, i32, vp, vp, i32, vp, i64, vp, vp],
@@ -341,6 +347,24 @@ class Graph:
         res = [Result(C.c_void_p(hs[i]), self.ctx) for i in range(n)] if results else None
         stats = {k: getattr(st, k) for k, _ in BatchStats._fields_}
         return nnz[:n], out_dig, res, stats
+
+    def autotune_batch(self, paths, src_rows=None, reps: int = 1) -> dict:
+        """Autotuner of the batched form on a sample of chains; leaves the best settings on the
+        context and returns them."""
+        n = len(paths)
+        plen = np.array([mp.pathLen for mp in paths], np.int32)
+        vts = np.concatenate([np.asarray(mp.vertexTypes, np.int32) for mp in paths])
+        ets = np.concatenate([np.asarray(mp.edgeTypes, np.int32) for mp in paths])
+        soff = srows = None
+        if src_rows is not None:
+            parts = [np.ascontiguousarray(r, np.int32) for r in src_rows]
+            soff = np.zeros(n + 1, np.int64)
+            soff[1:] = np.cumsum([len(p) for p in parts])
+            srows = np.concatenate(parts) if soff[-1] > 0 else np.zeros(1, np.int32)
+        t = BatchTuning()
+        _check(load().smcm_batch_autotune(self.ctx._h, self._h, n, _ptr(plen), _ptr(vts), _ptr(ets), _ptr(soff),
+                                          _ptr(srows), reps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in BatchTuning._fields_}
 
     def estimator_rows(self, mp: MetaPath, i: int, src_rows=None) -> np.ndarray:
         L = mp.pathLen

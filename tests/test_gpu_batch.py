@@ -158,3 +158,15 @@ def test_batch_shared_paths(ctx, graphs, batch_opts, small):
     ctx.set_option("batch_small_products", (1 << 40) if small else 0)
     st = check(ctx, G, paths, srcs, ref)
     assert st["jobs"] == len(letters), st
+
+
+def test_batch_autotune(ctx, graphs, batch_opts):
+    """The autotuner leaves a valid setting on the context and results stay identical."""
+    hg, G = graphs["mid"]
+    paths, srcs = make_batch(hg, 30, 41, smax=1000)
+    ref = reference(G, paths, srcs)
+    t = G.autotune_batch(paths[:20], srcs[:20])
+    assert t["trials"] > 0 and t["best_s"] <= t["baseline_s"] + 1e-9, t
+    assert ctx.get_option("batch_small_products") == t["small_products"]
+    assert ctx.get_option("batch_tile_rows") == t["tile_rows"]
+    check(ctx, G, paths, srcs, ref)
