@@ -243,15 +243,17 @@ __device__ __forceinline__ int64_t block_exscan_i64(int64_t x, int64_t* red, int
 // classed by length and compacted (one packed block scan):
 //   cnt <= kTiny        walked by the thread that owns the entry;
 //   cnt <= kQuarter     a 16-lane quarter of a wave per segment, four segments per wave step;
-//   longer              cut into pieces of kPiece products taken round-robin by the waves,
-//                       lanes reading consecutive elements (four loads in flight per lane).
+//   longer              cut into pieces of kPiece = 64 products taken round-robin by the waves,
+//                       kPieceStep = 8 pieces per wave step, lanes reading consecutive elements
+//                       (eight loads in flight per lane).
 // All lanes of a step read consecutive elements of one B row (coalesced), a single very long B
 // row (a hub) is shared by the whole workgroup, and there is no per-product search: a piece's
 // segment is found once per piece (wave-uniform LDS reads).
 constexpr int kChunk = 512;  // A entries per traversal round
 constexpr int kTiny = 2;
 constexpr int kQuarter = 16;
-constexpr int kPiece = 4 * kWave;
+constexpr int kPiece = kWave;
+constexpr int kPieceStep = 8;
 constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
 
 struct TravLds {
@@ -318,14 +320,15 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
     for (int h = 0; h < 2; ++h)
       if (ok[h]) f(c[h], BV ? av2[h] * v[h] : av2[h]);
   }
-  // pieces of the long segments, two per wave step (eight loads in flight per lane)
+  // pieces of the long segments (kPiece = one element per lane), kPieceStep pieces per wave step:
+  // kPieceStep loads in flight per lane
   int t = 0;
-  for (int32_t i = wid; i < npieces; i += 2 * nwaves) {
-    int32_t n[2];
-    int64_t b[2];
-    uint64_t av1[2];
+  for (int32_t i = wid; i < npieces; i += kPieceStep * nwaves) {
+    int32_t n[kPieceStep];
+    int64_t b[kPieceStep];
+    uint64_t av1[kPieceStep];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kPieceStep; ++h) {
       const int32_t ii = i + h * nwaves;
       n[h] = 0;
       if (ii < npieces) {
@@ -337,25 +340,18 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
         av1[h] = L.av[sidx];
       }
     }
-    int32_t c[2][4];
-    uint64_t v[2][4];
+    int32_t c[kPieceStep];
+    uint64_t v[kPieceStep];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int xx = lane + u * kWave;
-        if (xx < n[h]) {
-          c[h][u] = bcol[b[h] + xx];
-          v[h][u] = BV ? bval[b[h] + xx] : 1ull;
-        }
+    for (int h = 0; h < kPieceStep; ++h) {
+      if (lane < n[h]) {
+        c[h] = bcol[b[h] + lane];
+        v[h] = BV ? bval[b[h] + lane] : 1ull;
       }
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (lane + u * kWave < n[h]) f(c[h][u], BV ? av1[h] * v[h][u] : av1[h]);
-    }
+    for (int h = 0; h < kPieceStep; ++h)
+      if (lane < n[h]) f(c[h], BV ? av1[h] * v[h] : av1[h]);
   }
   __syncthreads();
 }
