@@ -243,17 +243,18 @@ __device__ __forceinline__ int64_t block_exscan_i64(int64_t x, int64_t* red, int
 // classed by length and compacted (one packed block scan):
 //   cnt <= kTiny        walked by the thread that owns the entry;
 //   cnt <= kQuarter     a 16-lane quarter of a wave per segment, four segments per wave step;
-//   longer              cut into pieces of kPiece = 64 products taken round-robin by the waves,
-//                       kPieceStep = 8 pieces per wave step, lanes reading consecutive elements
-//                       (eight loads in flight per lane).
+//   longer              cut into pieces of 64·E products taken round-robin by the waves, STEP
+//                       pieces per wave step, lanes reading consecutive elements (E·STEP loads
+//                       in flight per lane).  The numeric window kernel uses E = 1, STEP = 8
+//                       (short pieces fill the lanes); the bitmap tier E = 4, STEP = 2 (fewer
+//                       pieces to look up for its cheaper per-product work).
 // All lanes of a step read consecutive elements of one B row (coalesced), a single very long B
 // row (a hub) is shared by the whole workgroup, and there is no per-product search: a piece's
 // segment is found once per piece (wave-uniform LDS reads).
 constexpr int kChunk = 512;  // A entries per traversal round
 constexpr int kTiny = 2;
 constexpr int kQuarter = 16;
-constexpr int kPiece = kWave;
-constexpr int kPieceStep = 8;
+
 constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
 
 struct TravLds {
@@ -266,9 +267,10 @@ struct TravLds {
 
 // One traversal round: every participating thread brings at most one segment (st, cnt) of B
 // with the A value a of its entry (cnt = 0: none).  Ends with a barrier.
-template <bool BV, class F>
+template <bool BV, int E = 1, int STEP = 8, class F>
 __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, const int32_t* __restrict__ bcol,
                                            const uint64_t* __restrict__ bval, TravLds& L, F&& f) {
+  constexpr int kPieceLen = E * kWave;
   const int nwaves = blockDim.x / kWave;
   const int wid = threadIdx.x / kWave, lane = lane_id();
   int64_t packed = 0;  // pieces << 32 | long << 16 | quarter
@@ -278,7 +280,7 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
     } else if (cnt <= kQuarter) {
       packed = 1;
     } else {
-      packed = ((int64_t)((cnt + kPiece - 1) / kPiece) << 32) | (1 << 16);
+      packed = ((int64_t)((cnt + kPieceLen - 1) / kPieceLen) << 32) | (1 << 16);
     }
   }
   int64_t total;
@@ -320,43 +322,50 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
     for (int h = 0; h < 2; ++h)
       if (ok[h]) f(c[h], BV ? av2[h] * v[h] : av2[h]);
   }
-  // pieces of the long segments (kPiece = one element per lane), kPieceStep pieces per wave step:
-  // kPieceStep loads in flight per lane
+  // pieces of the long segments (E elements per lane), STEP pieces per wave step: E * STEP loads
+  // in flight per lane
   int t = 0;
-  for (int32_t i = wid; i < npieces; i += kPieceStep * nwaves) {
-    int32_t n[kPieceStep];
-    int64_t b[kPieceStep];
-    uint64_t av1[kPieceStep];
+  for (int32_t i = wid; i < npieces; i += STEP * nwaves) {
+    int32_t n[STEP];
+    int64_t b[STEP];
+    uint64_t av1[STEP];
 #pragma unroll
-    for (int h = 0; h < kPieceStep; ++h) {
+    for (int h = 0; h < STEP; ++h) {
       const int32_t ii = i + h * nwaves;
       n[h] = 0;
       if (ii < npieces) {
         while (L.Q[t + 1] <= ii) ++t;  // wave-uniform; segments are visited in order
         const int32_t j = ii - L.Q[t];
         const int32_t sidx = nquarter + t;
-        n[h] = min(kPiece, L.len[sidx] - j * kPiece);
-        b[h] = L.start[sidx] + (int64_t)j * kPiece;
+        n[h] = min(kPieceLen, L.len[sidx] - j * kPieceLen);
+        b[h] = L.start[sidx] + (int64_t)j * kPieceLen;
         av1[h] = L.av[sidx];
       }
     }
-    int32_t c[kPieceStep];
-    uint64_t v[kPieceStep];
+    int32_t c[STEP][E];
+    uint64_t v[STEP][E];
 #pragma unroll
-    for (int h = 0; h < kPieceStep; ++h) {
-      if (lane < n[h]) {
-        c[h] = bcol[b[h] + lane];
-        v[h] = BV ? bval[b[h] + lane] : 1ull;
+    for (int h = 0; h < STEP; ++h) {
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int xx = lane + u * kWave;
+        if (xx < n[h]) {
+          c[h][u] = bcol[b[h] + xx];
+          v[h][u] = BV ? bval[b[h] + xx] : 1ull;
+        }
       }
     }
 #pragma unroll
-    for (int h = 0; h < kPieceStep; ++h)
-      if (lane < n[h]) f(c[h], BV ? av1[h] * v[h] : av1[h]);
+    for (int h = 0; h < STEP; ++h) {
+#pragma unroll
+      for (int u = 0; u < E; ++u)
+        if (lane + u * kWave < n[h]) f(c[h][u], BV ? av1[h] * v[h][u] : av1[h]);
+    }
   }
   __syncthreads();
 }
 
-template <bool AV, bool BV, class SegF, class F>
+template <bool AV, bool BV, int E = 1, int STEP = 8, class SegF, class F>
 __device__ __forceinline__ void traverse(const int64_t rs, const int64_t re, const int32_t* __restrict__ acol,
                                          const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
                                          const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
@@ -377,7 +386,7 @@ __device__ __forceinline__ void traverse(const int64_t rs, const int64_t re, con
         st = b0 + off;
       }
     }
-    trav_round<BV>(st, cnt, a, bcol, bval, L, f);
+    trav_round<BV, E, STEP>(st, cnt, a, bcol, bval, L, f);
   }
 }
 
@@ -796,7 +805,7 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
       const int64_t nwords64 = (wlen + 63) / 64;
       for (int64_t i = threadIdx.x; i < nwords64 * 2; i += nthreads) bm[i] = 0u;
       __syncthreads();
-      traverse<false, false>(rs, re, acol, nullptr, bptr, bcol, nullptr, L, whole_row,
+      traverse<false, false, 4, 2>(rs, re, acol, nullptr, bptr, bcol, nullptr, L, whole_row,
                              [&](int32_t c, uint64_t) {
                                const int64_t x = (int64_t)c - w0;
                                if (x >= 0 && x < wlen) atomicOr(&bm[x >> 5], 1u << (x & 31));
@@ -834,7 +843,7 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
           __threadfence();
         }
         __syncthreads();
-        traverse<AV, BV>(rs, re, acol, aval, bptr, bcol, bval, L, whole_row, [&](int32_t c, uint64_t v) {
+        traverse<AV, BV, 4, 2>(rs, re, acol, aval, bptr, bcol, bval, L, whole_row, [&](int32_t c, uint64_t v) {
                                           const int64_t x = (int64_t)c - w0;
                                           if (x >= 0 && x < wlen) {
                                             const int64_t wd = x >> 6;
