@@ -59,6 +59,16 @@ SMCM_DEV double readlane_f64(double x, int lane) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// A value the whole wave holds (read from LDS at a wave-uniform address, say) moved into scalar
+// registers: the compiler cannot prove such a value uniform and would keep it in a VGPR per lane.
+SMCM_DEV int32_t uniform_i32(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+SMCM_DEV int64_t uniform_i64(int64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uint64_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+SMCM_DEV uint64_t uniform_u64(uint64_t x) { return (uint64_t)uniform_i64((int64_t)x); }
+
 SMCM_DEV int64_t wave_sum_i64(int64_t x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);

@@ -130,8 +130,9 @@ struct SpgemmOpts {
   int dense_div = 64;      // estimated width >= ncols/dense_div sends the row to the bitmap tier
   int64_t heavy_ub = 1 << 16;  // bitmap-tier rows with ub above this are scheduled first
   int d_threads = 512;     // bitmap-tier workgroup size
-  int w_threads = 1024;    // window-tier workgroup size
-  int64_t window_cols = 0; // window width cap (0 = largest that fits in LDS)
+  int w_threads = 512;     // window-tier workgroup size
+  int64_t window_cols = 8192;  // window width cap (0 = largest that fits in LDS); 8192 columns
+                               // (67 KB of LDS) fit two 512-thread workgroups per CU
   int64_t bitmap_vcap = -1; // cap of the bitmap tier's LDS value slots (-1 = as many as fit);
                             // rows wider than this use the window kernel
   int64_t lds_bytes = 65536;   // LDS available per workgroup
@@ -141,6 +142,11 @@ struct SpgemmOpts {
   int64_t wave_bitmap_max_ub = 2048;   // 64 < ub <= wave_bitmap_max_ub: one wave per row over an LDS
                                        // column bitmap (cols 0 = never)
   int64_t wave_bitmap_cap = 2048;      // LDS value slots per wave; wider rows accumulate in HBM
+  int window_major = 1;     // window tier in window-major task order (0 = one row at a time)
+  int64_t window_min_products = 128;   // rows wider than the bitmap tier's LDS value slots use the
+                                       // window kernel when they average at least this many
+                                       // products per (row, window); otherwise the bitmap kernel
+                                       // accumulates their values in HBM (device atomics)
 };
 
 struct SpgemmStats {

@@ -123,6 +123,12 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "window_cols") {
       if (v != 0 && (v < 64 || (v & (v - 1)) != 0)) fail(SMCM_EINVAL, "window_cols must be 0 or a power of two >= 64");
       o.window_cols = v;
+    } else if (k == "window_major") {
+      if (v != 0 && v != 1) fail(SMCM_EINVAL, "window_major must be 0 or 1");
+      o.window_major = (int)v;
+    } else if (k == "window_min_products") {
+      if (v < 0) fail(SMCM_EINVAL, "window_min_products must be >= 0");
+      o.window_min_products = v;
     } else if (k == "bitmap_vcap") {
       if (v < -1) fail(SMCM_EINVAL, "bitmap_vcap must be -1 or >= 0");
       o.bitmap_vcap = v;
@@ -188,6 +194,8 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "w_threads") *v = o.w_threads;
     else if (k == "window_cols") *v = o.window_cols;
     else if (k == "bitmap_vcap") *v = o.bitmap_vcap;
+    else if (k == "window_major") *v = o.window_major;
+    else if (k == "window_min_products") *v = o.window_min_products;
     else if (k == "force_tier") *v = o.force_tier;
     else if (k == "exact_cost_sum") *v = ctx->exact_cost_sum;
     else if (k == "plan_cost") *v = ctx->plan_cost;

@@ -256,6 +256,7 @@ constexpr int kTiny = 2;
 constexpr int kQuarter = 16;
 
 constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
+constexpr int64_t kWindowCountBytes = (int64_t)1 << 30;  // cap of the per-(row, window) count table
 
 struct TravLds {
   int64_t start[kChunk];  // first B index of a compacted segment (quarter segments, then long)
@@ -334,12 +335,12 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
       const int32_t ii = i + h * nwaves;
       n[h] = 0;
       if (ii < npieces) {
-        while (L.Q[t + 1] <= ii) ++t;  // wave-uniform; segments are visited in order
-        const int32_t j = ii - L.Q[t];
+        while (uniform_i32(L.Q[t + 1]) <= ii) ++t;  // wave-uniform; segments are visited in order
+        const int32_t j = ii - uniform_i32(L.Q[t]);
         const int32_t sidx = nquarter + t;
-        n[h] = min(kPieceLen, L.len[sidx] - j * kPieceLen);
-        b[h] = L.start[sidx] + (int64_t)j * kPieceLen;
-        av1[h] = L.av[sidx];
+        n[h] = min(kPieceLen, uniform_i32(L.len[sidx]) - j * kPieceLen);
+        b[h] = uniform_i64(L.start[sidx]) + (int64_t)j * kPieceLen;
+        av1[h] = uniform_u64(L.av[sidx]);
       }
     }
     int32_t c[STEP][E];
@@ -763,7 +764,8 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
                                    const int32_t* __restrict__ rows0, int32_t n0, const int32_t* __restrict__ rows1,
                                    int32_t n1, int32_t ncols, int64_t nw, int64_t vcap, int32_t* __restrict__ queue,
                                    int64_t* __restrict__ row_nnz, const int64_t* __restrict__ cptr,
-                                   int32_t* __restrict__ ccol, uint64_t* __restrict__ cval) {
+                                   int32_t* __restrict__ ccol, uint64_t* __restrict__ cval, int log2w = 0,
+                                   int32_t nwin = 0, int32_t* __restrict__ wcnt = nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* bm = (uint32_t*)smem;
   const uint64_t* bm64 = (const uint64_t*)smem;
@@ -814,6 +816,20 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
         int64_t local = 0;
         for (int64_t i = threadIdx.x; i < nwords32; i += nthreads) local += __popc(bm[i]);
         cnt += block_sum_i64(local, red64);
+        if (wcnt) {
+          // columns per window of the window tier (2^log2w columns, aligned; w0 is a multiple
+          // of 128 and 2^log2w >= 64, so no 64-bit word straddles two windows): a wave per window
+          const int64_t xa = w0 >> log2w, xb = (w0 + wlen - 1) >> log2w;
+          for (int64_t x = xa + wid; x <= xb; x += nwaves) {
+            const int64_t ja = max((x << log2w) - w0, (int64_t)0) >> 6;
+            const int64_t jb = min(((x + 1) << log2w) - w0, wlen + 63) >> 6;
+            int64_t c = 0;
+            for (int64_t j = ja + lane; j < jb; j += kWave) c += __popcll(bm64[j]);
+            c = wave_sum_i64(c);
+            if (lane == 0 && c) atomicAdd(&wcnt[(size_t)u * nwin + x], (int32_t)c);
+          }
+          __syncthreads();  // the next column window clears the bitmap
+        }
       } else {
         // prefix count per 64-bit word: each thread scans a contiguous range of words
         const int64_t per = (nwords64 + nthreads - 1) / nthreads;
@@ -885,8 +901,10 @@ struct WindowLayout {
   size_t bytes() const { return (size_t)w * 8 + (size_t)w / 8 + (size_t)w / 16; }
 };
 
-// bnd[k * (nwin + 1) + w] = offset (within row k of B) of the first element with column >= w*W;
-// bnd[k * (nwin + 1) + nwin] = length of row k.  One wave per row; each entry written once.
+// Window-major bound table: bnd[w * nrows + k] = offset (within row k of B) of the first element
+// with column >= w*W; bnd[nwin * nrows + k] = length of row k.  Window-major so that the two rows
+// of the table a window reads (w and w + 1) are contiguous.  One wave per row of B; each entry
+// written once.
 __global__ void __launch_bounds__(kThreads) window_bounds_kernel(const int64_t* __restrict__ bptr,
                                                                  const int32_t* __restrict__ bcol, int32_t nrows,
                                                                  int log2w, int32_t nwin, int32_t* __restrict__ bnd) {
@@ -895,14 +913,14 @@ __global__ void __launch_bounds__(kThreads) window_bounds_kernel(const int64_t* 
   for (int64_t k = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; k < nrows; k += (int64_t)gridDim.x * wpb) {
     const int64_t b0 = bptr[k];
     const int32_t len = (int32_t)(bptr[k + 1] - b0);
-    int32_t* out = bnd + (size_t)k * (nwin + 1);
+    int32_t* out = bnd + k;
     for (int32_t x = lane; x < len; x += kWave) {
       const int32_t w = bcol[b0 + x] >> log2w;
       const int32_t wp = x == 0 ? -1 : (bcol[b0 + x - 1] >> log2w);
-      for (int32_t ww = wp + 1; ww <= w; ++ww) out[ww] = x;
+      for (int32_t ww = wp + 1; ww <= w; ++ww) out[(size_t)ww * nrows] = x;
     }
     const int32_t wl = len > 0 ? (bcol[b0 + len - 1] >> log2w) : -1;
-    for (int32_t ww = wl + 1 + lane; ww <= nwin; ww += kWave) out[ww] = len;
+    for (int32_t ww = wl + 1 + lane; ww <= nwin; ww += kWave) out[(size_t)ww * nrows] = len;
   }
 }
 
@@ -911,9 +929,9 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
                                    const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
                                    const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
                                    const int32_t* __restrict__ rows, int32_t nrows, int32_t ncols, int log2w,
-                                   int32_t nwin, const int32_t* __restrict__ bnd, int32_t* __restrict__ queue,
-                                   const int64_t* __restrict__ cptr, int32_t* __restrict__ ccol,
-                                   uint64_t* __restrict__ cval) {
+                                   int32_t nwin, const int32_t* __restrict__ bnd, int32_t nbrows,
+                                   int32_t* __restrict__ queue, const int64_t* __restrict__ cptr,
+                                   int32_t* __restrict__ ccol, uint64_t* __restrict__ cval) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int32_t W = 1 << log2w;
   uint64_t* vals = (uint64_t*)smem;
@@ -982,7 +1000,7 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
         if (fast) {
           int64_t st = 0, cnt = 0;
           if (rs + (int64_t)threadIdx.x < re) {
-            const int32_t s1 = bnd[(size_t)my_k * (nwin + 1) + w + 1];
+            const int32_t s1 = bnd[(size_t)(w + 1) * nbrows + my_k];
             st = my_b0 + my_s;
             cnt = s1 - my_s;
             my_s = s1;
@@ -993,9 +1011,9 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
           traverse<AV, BV>(
               rs, re, acol, aval, bptr, bcol, bval, L,
               [&](int64_t, int32_t k, int64_t, int64_t, int64_t& off, int64_t& cnt) {
-                const int32_t* bk = bnd + (size_t)k * (nwin + 1);
-                off = bk[wcur];
-                cnt = bk[wcur + 1] - bk[wcur];
+                const int32_t* bk = bnd + (size_t)wcur * nbrows + k;
+                off = bk[0];
+                cnt = bk[nbrows] - off;
               },
               acc);
         }
@@ -1040,6 +1058,163 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
       }
       __syncthreads();
       if (done == width) break;
+    }
+  }
+}
+
+// Window-major window tier.  Task t = w * nrows + ri is the part of the ri-th wide row that falls
+// in window w; tasks are claimed in that order, so all workgroups work on the same window (or two
+// neighbours) at a time and the B elements they gather - the window's slice of every B row - stay
+// in L2 instead of being fetched again for every row.  The symbolic pass counted each row's
+// columns per window, which gives every task its output offset and size (empty tasks end at
+// once).  Pass 0 accumulates values only and takes the nonzero slots as the window's columns;
+// when that finds fewer columns than counted (a sum that wrapped to 0 mod 2^64) the window is
+// redone with presence bits before anything is written.
+struct WinRow {
+  int64_t rs, re;  // extent of the row in A
+  int64_t base;    // absolute output offset of the row
+};
+
+// Task metadata of one launch: info[ri] for the ri-th wide row and the exclusive prefix of its
+// per-window counts, pre[ri * (nwin + 1) + w] = columns in windows < w (pre[.. + nwin] = width).
+// One wave per row.
+__global__ void __launch_bounds__(kThreads) window_prep_kernel(const int32_t* __restrict__ rows, int32_t nrows,
+                                                               const int64_t* __restrict__ aptr,
+                                                               const int64_t* __restrict__ cptr,
+                                                               const int32_t* __restrict__ wcnt, int32_t nwin,
+                                                               WinRow* __restrict__ info, int32_t* __restrict__ pre) {
+  const int lane = lane_id();
+  const int64_t wpb = kThreads / kWave;
+  for (int64_t ri = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; ri < nrows; ri += (int64_t)gridDim.x * wpb) {
+    const int32_t u = rows[ri];
+    if (lane == 0) info[ri] = WinRow{aptr[u], aptr[u + 1], cptr[u]};
+    const int32_t* wc = wcnt + (size_t)u * nwin;
+    int32_t* out = pre + (size_t)ri * (nwin + 1);
+    if (lane == 0) out[0] = 0;
+    int32_t carry = 0;
+    for (int32_t x0 = 0; x0 < nwin; x0 += kWave) {
+      int32_t v = x0 + lane < nwin ? wc[x0 + lane] : 0;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int32_t y = __shfl_up(v, o);
+        if (lane >= o) v += y;
+      }
+      if (x0 + lane < nwin) out[x0 + lane + 1] = carry + v;
+      carry += __shfl(v, kWave - 1);
+    }
+  }
+}
+
+template <bool AV, bool BV>
+__global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
+                                    const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+                                    const uint64_t* __restrict__ bval, const WinRow* __restrict__ info,
+                                    const int32_t* __restrict__ pre, int32_t nrows, int32_t ncols, int log2w,
+                                    int32_t nwin, const int32_t* __restrict__ bnd, int32_t nbrows,
+                                    int32_t* __restrict__ queue, int32_t* __restrict__ ccol,
+                                    uint64_t* __restrict__ cval) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int32_t W = 1 << log2w;
+  uint64_t* vals = (uint64_t*)smem;
+  uint32_t* pres = (uint32_t*)(smem + (size_t)W * 8);
+  uint64_t* pres64 = (uint64_t*)pres;
+  __shared__ TravLds L;
+  __shared__ int32_t claim;
+  __shared__ int64_t meta[4];  // A extent, output offset and column count of the current task
+  __shared__ int32_t wcw[1024 / kWave];
+  const int nthreads = blockDim.x, nwaves = nthreads / kWave;
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int chunk = min(kChunk, nthreads);
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int64_t ntasks = (int64_t)nrows * nwin;
+  for (int i = threadIdx.x; i < W; i += nthreads) vals[i] = 0ull;
+  for (int i = threadIdx.x; i < W / 32; i += nthreads) pres[i] = 0u;
+  if (threadIdx.x == 0) claim = atomicAdd(queue, 1);
+  __syncthreads();
+  while (true) {
+    const int32_t t = uniform_i32(claim);
+    if (t >= ntasks) break;
+    const int32_t w = t / nrows;
+    __syncthreads();  // every thread has read `claim`
+    if (threadIdx.x == 0) {
+      // the next task is claimed now, so that its round trip overlaps this task's metadata loads
+      const int32_t nx = atomicAdd(queue, 1);
+      const int32_t ri = t - w * nrows;
+      const WinRow r = info[ri];
+      const int32_t* p = pre + (size_t)ri * (nwin + 1) + w;
+      const int32_t p0 = p[0], p1 = p[1];
+      meta[0] = r.rs;
+      meta[1] = r.re;
+      meta[2] = r.base + p0;
+      meta[3] = p1 - p0;
+      claim = nx;
+    }
+    __syncthreads();
+    const int64_t rs = uniform_i64(meta[0]), re = uniform_i64(meta[1]), rowbase = uniform_i64(meta[2]);
+    const int32_t target = (int32_t)uniform_i64(meta[3]);
+    if (target == 0) continue;
+    const int64_t w0 = (int64_t)w << log2w;
+    const int32_t* b_lo = bnd + (size_t)w * nbrows;
+    const int32_t* b_hi = b_lo + nbrows;
+    for (int pass = 0; pass < 2; ++pass) {
+      const bool with_pres = pass == 1;
+      auto acc = [&](int32_t c, uint64_t v) {
+        const int32_t x = (int32_t)(c - w0);
+        atomicAdd((unsigned long long*)&vals[x], (unsigned long long)v);
+        if (with_pres) atomicOr(&pres[x >> 5], 1u << (x & 31));
+      };
+      for (int64_t base = rs; base < re; base += chunk) {
+        int64_t st = 0, cnt = 0;
+        uint64_t a = 1ull;
+        const int64_t e = base + threadIdx.x;
+        if ((int)threadIdx.x < chunk && e < re) {
+          const int32_t k = acol[e];
+          const int32_t s0 = b_lo[k];
+          cnt = b_hi[k] - s0;
+          st = bptr[k] + s0;
+          a = AV ? aval[e] : 1ull;
+        }
+        trav_round<BV>(st, cnt, a, bcol, bval, L, acc);
+      }
+      // compaction in column order: each wave owns a contiguous run of 64-column words
+      const int64_t wend = min(w0 + (int64_t)W, (int64_t)ncols);
+      const int32_t nwords = (int32_t)((wend - w0 + 63) / 64);
+      const int32_t per = (nwords + nwaves - 1) / nwaves;
+      const int32_t ja = min(wid * per, nwords), jb = min(ja + per, nwords);
+      int32_t mine = 0;
+      for (int32_t j = ja; j < jb; ++j) {
+        const uint64_t m = with_pres ? pres64[j] : __ballot(vals[j * 64 + lane] != 0ull);
+        mine += __popcll(m);
+      }
+      if (lane == 0) wcw[wid] = mine;
+      __syncthreads();
+      int32_t woff = 0, wtot = 0;
+      for (int v2 = 0; v2 < nwaves; ++v2) {
+        const int32_t cw = wcw[v2];
+        woff += v2 < wid ? cw : 0;
+        wtot += cw;
+      }
+      const bool write = with_pres || wtot == target;
+      int64_t pos0 = woff;
+      for (int32_t j = ja; j < jb; ++j) {
+        const int32_t xi = j * 64 + lane;
+        const uint64_t v = vals[xi];
+        const uint64_t m = with_pres ? pres64[j] : __ballot(v != 0ull);
+        if (write && ((m >> lane) & 1ull)) {
+          const int64_t slot = pos0 + __popcll(m & below);
+          if (slot < target) {
+            ccol[rowbase + slot] = (int32_t)(w0 + xi);
+            cval[rowbase + slot] = v;
+          }
+        }
+        if (m) vals[xi] = 0ull;
+        if (with_pres && lane == 0) pres64[j] = 0ull;
+        pos0 += __popcll(m);
+      }
+      // the next task's first accumulations (inline tiny segments) precede its first barrier:
+      // every wave must be done with this window's slots before any wave moves on
+      __syncthreads();
+      if (write) break;
     }
   }
 }
@@ -1106,8 +1281,11 @@ struct SplitScratch {
   int32_t* counts = nullptr;          // 3: narrow rows, wide rows, widest narrow row
   unsigned long long* sums = nullptr;  // 6
   int32_t* bnd = nullptr;             // window bound table of B (built on first use)
+  int32_t* wcnt = nullptr;            // columns per (row, window) from the symbolic pass, or null
+                                      // (then the window tier runs one row at a time)
   WindowLayout wl;
   struct SpgemmCore* owner = nullptr;
+  DevAlloc* al = nullptr;             // the owner's allocator (per-launch task metadata)
   void build_bounds(const Csr& B, hipStream_t st);
 };
 
@@ -1170,7 +1348,8 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     const int tok = T.start("spgemm.symbolic.bitmap", st);
     bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(
         A.ptr, A.col, A.val, B.ptr, B.col, B.val, bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m,
-        nB, B.ncols, lay.nw, lay.vcap, queue, row_nnz, cptr, ccol, cval);
+        nB, B.ncols, lay.nw, lay.vcap, queue, row_nnz, cptr, ccol, cval, split->wl.log2w, (int32_t)split->wl.nwin,
+        split->wcnt);
     SMCM_LAUNCH_CHECK();
     T.stop(tok, st, tier_bytes(false, both, 0, AV, BV));
   }
@@ -1210,7 +1389,24 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
       SMCM_LAUNCH_CHECK();
       T.stop(tok, st, tier_bytes(true, cls, 0, AV, BV));
     }
-    if (cnt[1] > 0) {
+    // wide rows with few products per window: one pass over a column bitmap of the whole row
+    // (values accumulated in the output by device atomics) beats many nearly empty windows
+    const double per_task = cnt[1] > 0 ? (double)cls.products[1] / ((double)cnt[1] * (double)split->wl.nwin) : 0.0;
+    if (cnt[1] > 0 && per_task < (double)o.window_min_products) {
+      fill_i32(queue, 1, 0, st);
+      BitmapLayout gl = lay;
+      gl.vcap = 0;
+      const size_t lds = gl.bytes_num();
+      const int per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / o.d_threads, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
+      const unsigned g = (unsigned)std::min<int64_t>(cnt[1], (int64_t)o.cus * per_cu);
+      const int tok = T.start("spgemm.numeric.bitmap.hbm", st);
+      bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val,
+                                                                       nullptr, 0, split->wide, cnt[1], B.ncols,
+                                                                       gl.nw, gl.vcap, queue, row_nnz, cptr, ccol,
+                                                                       cval);
+      SMCM_LAUNCH_CHECK();
+      T.stop(tok, st, tier_bytes(true, cls, 1, AV, BV));
+    } else if (cnt[1] > 0) {
       if (!split->bnd) split->build_bounds(B, st);
       fill_i32(queue, 1, 0, st);
       const size_t lds = split->wl.bytes();
@@ -1220,10 +1416,24 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
       const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(by_lds, by_waves));
       const unsigned g = (unsigned)std::min<int64_t>(cnt[1], (int64_t)o.cus * per_cu);
       const int tok = T.start("spgemm.numeric.window", st);
-      window_tier_kernel<AV, BV><<<g, o.w_threads, lds, st>>>(A.ptr, A.col, A.val, B.ptr, B.col, B.val, split->wide,
-                                                              cnt[1], B.ncols, split->wl.log2w,
-                                                              (int32_t)split->wl.nwin, split->bnd, queue, cptr, ccol,
-                                                              cval);
+      if (split->wcnt) {
+        const int32_t nwin = (int32_t)split->wl.nwin;
+        DevBuf<WinRow> info(*split->al, cnt[1]);
+        DevBuf<int32_t> pre(*split->al, (int64_t)cnt[1] * (nwin + 1));
+        window_prep_kernel<<<(unsigned)std::min<int64_t>(ceil_div(cnt[1], kThreads / kWave), 65536), kThreads, 0,
+                             st>>>(split->wide, cnt[1], A.ptr, cptr, split->wcnt, nwin, info.p, pre.p);
+        SMCM_LAUNCH_CHECK();
+        auto kern = window_major_kernel<AV, BV>;
+        int nb = 0;
+        SMCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, o.w_threads, lds));
+        const unsigned gm = (unsigned)std::max<int64_t>(1, (int64_t)o.cus * std::max(nb, 1));
+        kern<<<gm, o.w_threads, lds, st>>>(A.col, A.val, B.ptr, B.col, B.val, info.p, pre.p, cnt[1], B.ncols,
+                                           split->wl.log2w, nwin, split->bnd, B.nrows, queue, ccol, cval);
+      } else {
+        window_tier_kernel<AV, BV><<<g, o.w_threads, lds, st>>>(
+            A.ptr, A.col, A.val, B.ptr, B.col, B.val, split->wide, cnt[1], B.ncols, split->wl.log2w,
+            (int32_t)split->wl.nwin, split->bnd, B.nrows, queue, cptr, ccol, cval);
+      }
       SMCM_LAUNCH_CHECK();
       T.stop(tok, st, tier_bytes(true, cls, 1, AV, BV));
     }
@@ -1250,9 +1460,12 @@ template <bool NUMERIC, bool AV, bool BV>
 void set_lds_attr(size_t bytes) {
   (void)hipFuncSetAttribute((const void*)bitmap_tier_kernel<NUMERIC, AV, BV>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if constexpr (NUMERIC)
+  if constexpr (NUMERIC) {
     (void)hipFuncSetAttribute((const void*)window_tier_kernel<AV, BV>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)bytes);
+    (void)hipFuncSetAttribute((const void*)window_major_kernel<AV, BV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
+  }
 }
 
 void set_all_lds_attrs(size_t bytes) {
@@ -1322,7 +1535,7 @@ struct SpgemmCore {
   DevBuf<int64_t> S;
   DevBuf<int32_t> bins, counts, queue;
   DevBuf<unsigned long long> sums;
-  DevBuf<int32_t> split_rows, split_counts, bnd;
+  DevBuf<int32_t> split_rows, split_counts, bnd, wcnt;
   DevBuf<unsigned long long> split_sums;
   SplitScratch split;
   TierCounts tc;
@@ -1358,6 +1571,14 @@ struct SpgemmCore {
     split.counts = split_counts.p;
     split.sums = split_sums.p;
     split.owner = this;
+    split.al = &al;
+    // per-(row, window) column counts for the window-major window tier; the task index
+    // (rows x windows) must stay an int32 queue value
+    const int64_t cells = (int64_t)A.nrows * split.wl.nwin;
+    if (o.window_major && A.nrows > 0 && cells * 4 <= kWindowCountBytes && cells < (int64_t)INT32_MAX - (1 << 24)) {
+      wcnt = DevBuf<int32_t>(al, cells);
+      split.wcnt = wcnt.p;
+    }
     SMCM_HIP(hipMemcpyAsync(&products, S.p + A.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   }
 
@@ -1386,6 +1607,7 @@ struct SpgemmCore {
   // widths of the binned rows into row_nnz (other entries untouched); fills tc.c_nnz
   void symbolic(int64_t* row_nnz) {
     hipStream_t st = al.stream;
+    if (split.wcnt) fill_i32(split.wcnt, (int64_t)A.nrows * split.wl.nwin, 0, st);
     dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, &split, queue.p, st, tm);
     unsigned long long* tn = sums.p + 2 * kNumBins;
     fill_i64((int64_t*)tn, kNumBins, 0, st);
@@ -1413,7 +1635,7 @@ struct SpgemmCore {
 };
 
 void SplitScratch::build_bounds(const Csr& B, hipStream_t st) {
-  owner->bnd = DevBuf<int32_t>(owner->al, (int64_t)B.nrows * (wl.nwin + 1));
+  owner->bnd = DevBuf<int32_t>(owner->al, (int64_t)std::max(B.nrows, 1) * (wl.nwin + 1));
   bnd = owner->bnd.p;
   if (B.nrows > 0) {
     window_bounds_kernel<<<(unsigned)std::min<int64_t>(ceil_div(B.nrows, kThreads / kWave), 65536), kThreads, 0, st>>>(

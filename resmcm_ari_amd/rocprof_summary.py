@@ -28,8 +28,9 @@ from collections import defaultdict
 
 # timed-kernel name -> pattern on the HIP kernel name (template arguments included)
 TIMED = {
-    "spgemm.numeric.window": r"window_tier_kernel<",
+    "spgemm.numeric.window": r"window_(tier|major)_kernel<",
     "spgemm.numeric.bitmap": r"bitmap_tier_kernel<true",
+    "spgemm.numeric.bitmap.hbm": None,  # the same kernel as spgemm.numeric.bitmap (values in HBM)
     "spgemm.symbolic.bitmap": r"bitmap_tier_kernel<false",
     "spgemm.numeric.wavebitmap": r"wave_bitmap_kernel<true",
     "spgemm.symbolic.wavebitmap": r"wave_bitmap_kernel<false",

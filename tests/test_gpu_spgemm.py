@@ -1,4 +1,5 @@
 """SpGEMM tiers on the GPU vs the oracle (bit-exact rowptr, sorted columns, uint64 values)."""
+import contextlib
 import zlib
 
 import numpy as np
@@ -8,6 +9,27 @@ import oracle as O
 from helpers import assert_same_csr, random_csr
 
 pytestmark = pytest.mark.gpu
+
+
+# window-tier routes for rows wider than the bitmap tier's LDS value slots
+WIDE_MODES = {
+    "window_major": {"window_major": 1, "window_min_products": 0},
+    "row_major": {"window_major": 0, "window_min_products": 0},
+    "bitmap_hbm": {"window_min_products": 1 << 62},
+}
+
+
+@contextlib.contextmanager
+def options(ctx, **kv):
+    """Set context options for the block, restoring the previous values afterwards."""
+    old = {k: ctx.get_option(k) for k in kv}
+    try:
+        for k, v in kv.items():
+            ctx.set_option(k, v)
+        yield
+    finally:
+        for k, v in old.items():
+            ctx.set_option(k, v)
 
 
 def _run(ctx, A, B):
@@ -117,46 +139,49 @@ def test_spgemm_wraparound(ctx):
         assert_same_csr(ptr, col, val, ref, what=f"wrap tier={tier}")
 
 
+@pytest.mark.parametrize("mode", list(WIDE_MODES))
 @pytest.mark.parametrize("window", [64, 256, 0])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("vals", ["none", "AB"])
-def test_spgemm_window_tier(ctx, case, window, vals):
+def test_spgemm_window_tier(ctx, case, window, vals, mode):
     """Every non-empty row through the bitmap tier with no LDS value slots: the numeric pass
-    runs in the window kernel (cursor-driven dense LDS windows)."""
+    runs in the window kernel (dense LDS windows) in window-major task order (per-(row, window)
+    counts from the symbolic pass) or one row at a time, or in the bitmap kernel with values
+    accumulated in HBM."""
     name, m, k, n, la, lb = case
     rng = np.random.default_rng(zlib.crc32(f"win/{name}/{window}/{vals}".encode()))
     A = random_csr(rng, m, k, row_len=la, with_vals=vals == "AB")
     B = random_csr(rng, k, n, row_len=lb, with_vals=vals == "AB")
     ref = O.spgemm(A, B)
-    ctx.set_option("force_tier", 2)
-    ctx.set_option("bitmap_vcap", 0)
-    ctx.set_option("window_cols", window)
-    try:
+    with options(ctx, force_tier=2, bitmap_vcap=0, window_cols=window, **WIDE_MODES[mode]):
         ptr, col, val = _run(ctx, A, B)
-    finally:
-        ctx.set_option("force_tier", -1)
-        ctx.set_option("bitmap_vcap", -1)
-        ctx.set_option("window_cols", 0)
-    assert_same_csr(ptr, col, val, ref, what=f"{name} window={window} vals={vals}")
+    assert_same_csr(ptr, col, val, ref, what=f"{name} window={window} vals={vals} mode={mode}")
 
 
 def test_spgemm_window_many_entries(ctx):
-    """Rows with more A entries than the LDS cursor slots use the global cursor array."""
+    """Rows with more A entries than one traversal round (several rounds per window)."""
     rng = np.random.default_rng(77)
     A = random_csr(rng, 6, 5000, row_len=lambda r: r.integers(2500, 4000))
     B = random_csr(rng, 5000, 3000, row_len=lambda r: r.integers(0, 40), with_vals=True)
     ref = O.spgemm(A, B)
-    for window in (64, 1024, 0):
-        ctx.set_option("force_tier", 2)
-        ctx.set_option("bitmap_vcap", 0)
-        ctx.set_option("window_cols", window)
-        try:
+    for mode, kv in WIDE_MODES.items():
+        for window in (64, 1024, 0):
+            with options(ctx, force_tier=2, bitmap_vcap=0, window_cols=window, **kv):
+                ptr, col, val = _run(ctx, A, B)
+            assert_same_csr(ptr, col, val, ref, what=f"many entries window={window} mode={mode}")
+
+
+def test_window_major_symbolic_windows(ctx):
+    """Per-(row, window) counts when the symbolic bitmap itself runs over several column windows
+    (LDS capped so the symbolic bitmap covers fewer columns than B has)."""
+    rng = np.random.default_rng(91)
+    A = random_csr(rng, 30, 2000, row_len=lambda r: r.integers(50, 400), with_vals=True)
+    B = random_csr(rng, 2000, 300000, row_len=lambda r: int(r.zipf(1.3)) % 3000, with_vals=True)
+    ref = O.spgemm(A, B)
+    for mode, kv in WIDE_MODES.items():
+        with options(ctx, force_tier=2, bitmap_vcap=0, lds_bytes=65536, window_cols=4096, **kv):
             ptr, col, val = _run(ctx, A, B)
-        finally:
-            ctx.set_option("force_tier", -1)
-            ctx.set_option("bitmap_vcap", -1)
-            ctx.set_option("window_cols", 0)
-        assert_same_csr(ptr, col, val, ref, what=f"many entries window={window}")
+        assert_same_csr(ptr, col, val, ref, what=f"symbolic windows mode={mode}")
 
 
 @pytest.mark.parametrize("threads", [256, 512, 1024])
@@ -167,21 +192,12 @@ def test_spgemm_workgroup_sizes(ctx, threads):
     A = random_csr(rng, 40, 3000, row_len=lambda r: r.integers(200, 1500), with_vals=True)
     B = random_csr(rng, 3000, 2500, row_len=lambda r: int(r.zipf(1.4)) % 900, with_vals=True)
     ref = O.spgemm(A, B)
-    for vcap in (-1, 0):
-        ctx.set_option("force_tier", 2)
-        ctx.set_option("bitmap_vcap", vcap)
-        ctx.set_option("d_threads", threads)
-        ctx.set_option("w_threads", threads)
-        ctx.set_option("window_cols", 256)
-        try:
+    modes = [(-1, "window_major")] + [(0, m) for m in WIDE_MODES]
+    for vcap, mode in modes:
+        with options(ctx, force_tier=2, bitmap_vcap=vcap, d_threads=threads, w_threads=threads, window_cols=256,
+                     **WIDE_MODES[mode]):
             ptr, col, val = _run(ctx, A, B)
-        finally:
-            ctx.set_option("force_tier", -1)
-            ctx.set_option("bitmap_vcap", -1)
-            ctx.set_option("d_threads", 512)
-            ctx.set_option("w_threads", 1024)
-            ctx.set_option("window_cols", 0)
-        assert_same_csr(ptr, col, val, ref, what=f"threads={threads} vcap={vcap}")
+        assert_same_csr(ptr, col, val, ref, what=f"threads={threads} vcap={vcap} mode={mode}")
 
 
 def test_window_tier_wraparound(ctx):
@@ -193,12 +209,26 @@ def test_window_tier_wraparound(ctx):
               np.array([big, big, 5], np.uint64))
     B = O.CSR(2, 3, np.array([0, 2, 3], np.int64), np.array([0, 2, 2], np.int32), np.array([1, 1, 1], np.uint64))
     ref = O.spgemm(A, B)
-    ctx.set_option("force_tier", 2)
-    ctx.set_option("bitmap_vcap", 0)
-    try:
-        ptr, col, val = _run(ctx, A, B)
-    finally:
-        ctx.set_option("force_tier", -1)
-        ctx.set_option("bitmap_vcap", -1)
-    assert list(col) == [0, 2, 2] and list(val) == [int(big), 0, 5]
-    assert_same_csr(ptr, col, val, ref, what="window wrap")
+    for mode, kv in WIDE_MODES.items():
+        with options(ctx, force_tier=2, bitmap_vcap=0, **kv):
+            ptr, col, val = _run(ctx, A, B)
+        assert list(col) == [0, 2, 2] and list(val) == [int(big), 0, 5]
+        assert_same_csr(ptr, col, val, ref, what=f"window wrap mode={mode}")
+
+
+def test_window_major_wraparound_later_window(ctx):
+    """A wrapped count in a window after the first: that window is redone with presence bits
+    and its entries land after the earlier windows' (64-column windows)."""
+    big = np.uint64(2**63)
+    # row 0: columns 5 (value 1), 70 (2^63 + 2^63 = 0), 200 (3); row 1: column 70 only
+    A = O.CSR(2, 3, np.array([0, 3, 4], np.int64), np.array([0, 1, 2, 2], np.int32),
+              np.array([1, big, big, 7], np.uint64))
+    B = O.CSR(3, 260, np.array([0, 2, 3, 4], np.int64), np.array([5, 200, 70, 70], np.int32),
+              np.array([1, 3, 1, 1], np.uint64))
+    ref = O.spgemm(A, B)
+    for mode, kv in WIDE_MODES.items():
+        with options(ctx, force_tier=2, bitmap_vcap=0, window_cols=64, **kv):
+            ptr, col, val = _run(ctx, A, B)
+        assert list(col) == [5, 70, 200, 70]
+        assert list(val) == [1, 0, 3, 7]
+        assert_same_csr(ptr, col, val, ref, what=f"later-window wrap mode={mode}")
