@@ -70,7 +70,10 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
 /* Tuning / mode keys: "s_max", "m_max", "dense_div", "heavy_ub", "d_threads", "force_tier",
  * "exact_cost_sum" (1 = sum c_ij in row order on one lane: bit-exact with the reference at one
  * thread), "kernel_timing" (1 = time every kernel with HIP events on the context's stream),
- * "lds_bytes". */
+ * "lds_bytes", "plan_cost" (the DP's cost: 0 the reference's estimated sizes, 1 estimated
+ * products, 2-5 the reference's other providers MetaAC / MNC / DMap / LGraph, see
+ * smcm_cost_estimate), "lgraph_seed", "window_major", "window_min_products", "window_cols",
+ * "w_threads". */
 int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);
 int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* value);
 const char* smcm_last_error(void);
@@ -134,6 +137,19 @@ int smcm_chain_products_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vt
 /* Estimated nnz of the whole chain, c_[0][L-1] (accuracy/res.cpp:494-523, 564-600). */
 int smcm_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
                   int32_t L, const int32_t* src_rows, int64_t n_src, double* est_nnz);
+/* The reference's other DP cost providers over the whole chain (no source subset), selected by
+ * `provider`: 2 MetaAC (efficient/meta.cpp:448-566), 3 MNC sketches (efficient/mnc.cpp:313-800),
+ * 4 DMap density maps (efficient/dm.cpp:52-620), 5 LGraph rank propagation
+ * (efficient/lg.cpp:52-740; ranks from the seeded generator of option lgraph_seed, where the
+ * reference seeds mt19937 from random_device).  accuracy = 0: the planner's tables (the same
+ * plan the option plan_cost = provider makes smcm_chain_run execute); accuracy = 1: the
+ * estimate-only programs (accuracy/{meta,mnc,dm,lg}.cpp).  *estimate = the number those
+ * programs print for the meta-path (MetaAC and DMap: density * n * n; MNC and LGraph: nnz).
+ * c_table (the provider's density or size per interval), m_table (DP costs): L*L doubles;
+ * s_table: L*L.  Any table may be NULL. */
+int smcm_cost_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
+                       int32_t L, int32_t provider, int32_t accuracy, double* estimate,
+                       double* c_table, double* m_table, int32_t* s_table);
 /* Row densities r_ij of chain position i for j = i..L-1: out[u*(L-i) + (j-i)], u over the rows
  * of M_i (cap = number of doubles available in out). */
 int smcm_estimator_rows(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,

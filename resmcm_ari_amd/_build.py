@@ -30,6 +30,7 @@ HIP_SOURCES = [
     "spgemm.hip",
     "scan.hip",
     "batch.hip",
+    "providers.hip",
     "smcm_api.cpp",
     "batch_api.cpp",
     "server.cpp",

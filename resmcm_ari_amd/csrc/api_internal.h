@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "providers.h"
 #include "smcm.h"
 
 using smcm::Csr;
@@ -78,7 +79,9 @@ struct smcm_ctx {
   uint64_t budget = 0;
   smcm::SpgemmOpts opts;
   bool exact_cost_sum = false;
-  int plan_cost = 0;  // 0: the reference DP over estimated sizes; 1: estimated products + sizes
+  int plan_cost = 0;  // 0: the reference DP over estimated sizes; 1: estimated products + sizes;
+                      // 2-5: the reference's other cost providers (MetaAC, MNC, DMap, LGraph)
+  uint64_t lgraph_seed = 0x5EED0000ull;  // LGraph ranks
   struct {
     int64_t small_products = 1 << 25;      // chains with fewer estimated products use the executor
     int threads = 512, sort_cap = 1024;    // executor workgroup shape
@@ -182,6 +185,7 @@ struct Chain {
   std::vector<double> c;           // L*L
   std::vector<double> ptab;        // L*L*L estimated products per split (plan_cost 1)
   bool want_products = false;
+  int provider = 0;                // 2-5: plan with one of the reference's other cost providers
   std::vector<double> mtab;
   std::vector<int32_t> stab, pairs;
   double es_time = 0, dp_time = 0, spgemm_time = 0, total = 0;
@@ -246,9 +250,68 @@ struct Chain {
     subset = false;
   }
 
+  // The reference's other cost providers (efficient/{meta,mnc,dm,lg}.cpp; accuracy = the
+  // estimate-only programs accuracy/*.cpp): their c, m and s tables over the global index space.
+  void provider_tables(int provider, bool accuracy, smcm::ProviderOut& out) {
+    DevAlloc& al = ctx->al;
+    smcm::ProviderChain pc;
+    pc.L = L;
+    pc.mats = mats.data();
+    pc.n = g->n;
+    pc.seed = ctx->lgraph_seed;
+    DevBuf<int32_t> src_g;
+    for (int32_t i = 0; i < L; ++i) {
+      pc.rowg.push_back(g->bins[vts[i]]);
+      pc.colg.push_back(g->bins[vts[i + 1]]);
+      pc.same_type.push_back(vts[i] == vts[i + 1]);
+    }
+    if (subset) {
+      // rows of M_0 are the sources: their global ids
+      std::vector<int32_t> h(src_local.size());
+      std::vector<int32_t> ids;
+      for (int32_t v = 0; v < g->n; ++v)
+        if (g->h_vtype[v] == vts[0]) ids.push_back(v);
+      for (size_t r = 0; r < h.size(); ++r) h[r] = ids[src_local[r]];
+      src_g = DevBuf<int32_t>(al, (int64_t)h.size());
+      if (!h.empty())
+        SMCM_HIP(hipMemcpyAsync(src_g.p, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, al.stream));
+      pc.rowg[0] = src_g.p;
+    }
+    std::vector<int32_t> lo(g->ntypes, INT32_MAX), hi(g->ntypes, -1);
+    for (int32_t v = 0; v < g->n; ++v) {
+      const int32_t t = g->h_vtype[v];
+      lo[t] = std::min(lo[t], v);
+      hi[t] = std::max(hi[t], v);
+    }
+    for (int32_t q = 0; q <= L; ++q) {
+      const int32_t t = vts[q];
+      pc.blo.push_back(hi[t] < 0 ? 0 : lo[t] / 256);
+      pc.bhi.push_back(hi[t] < 0 ? -1 : hi[t] / 256);
+    }
+    if (provider == smcm::kProviderLGraph) {
+      for (int32_t i = 0; i < L; ++i) {
+        std::vector<int32_t> ids;
+        for (int32_t v = 0; v < g->n; ++v)
+          if (g->h_vtype[v] == vts[i + 1]) ids.push_back(v);
+        pc.col_type_ids.push_back(std::move(ids));
+      }
+    }
+    smcm::run_provider(provider, accuracy, pc, out, al);
+  }
+
   void estimate(double* c_out = nullptr, double* p_out = nullptr) {
     DevAlloc& al = ctx->al;
     const double t0 = now_s();
+    if (provider >= smcm::kProviderMetaAC) {
+      if (c_out || p_out) fail(SMCM_EINVAL, "cost providers 2-5 run on the host-planned path only");
+      smcm::ProviderOut po;
+      provider_tables(provider, false, po);
+      c = po.c;
+      mtab = po.m;
+      stab = po.s;
+      es_time = now_s() - t0;
+      return;
+    }
     c.assign((size_t)L * L, 0.0);
     R.clear();
     R.resize(L);
@@ -314,6 +377,13 @@ struct Chain {
   // resmcm.cpp:519-538 and :563-573
   void dp() {
     const double t0 = now_s();
+    if (provider >= smcm::kProviderMetaAC) {
+      // the provider's DP ran with its estimates (provider_tables)
+      pairs.clear();
+      if (L > 1) plan_rec(0, L - 1);
+      dp_time = now_s() - t0;
+      return;
+    }
     mtab.assign((size_t)L * L, 0.0);
     stab.assign((size_t)L * L, 0);
     if (want_products && !ptab.empty()) {
@@ -436,7 +506,8 @@ struct Chain {
       Csr C;
       smcm::SpgemmStats ss;
       const int32_t i = a.i, j = b.j;
-      const double* est = R[i].p + (j - i);
+      // the estimator's row densities guide row binning (absent when a cost provider planned)
+      const double* est = i < (int32_t)R.size() && R[i].p ? R[i].p + (j - i) : nullptr;
       const bool root = ++done_products == nprod;
       if (root && consumer)
         smcm::spgemm_blocked(a.m, b.m, est, L - i, block_bytes, *consumer, ctx->opts, &ss, al, tm);

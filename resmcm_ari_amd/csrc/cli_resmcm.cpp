@@ -4,7 +4,10 @@
 // and the same appended output line per meta-path (:883):
 //   "<meta-path> <nnz> <total> <spgemm> <dp> <est>"
 // with times averaged over the iterations.  THREADS is accepted for compatibility (the
-// products run on the GPU).  An optional 8th argument selects the device.
+// products run on the GPU).  An optional 8th argument selects the device.  The environment
+// variable SMCM_PLAN_COST (meta, mnc, dm, lg or 0-5) selects the DP's cost provider, which makes
+// the program a drop-in for the reference's efficient/{meta,mnc,dm,lg} programs as well (same
+// arguments and output line).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -48,6 +51,15 @@ int main(int argc, char** argv) {
       smcm_graph_upload(ctx, hg->n, hg->vtype, hg->adj_ptr, hg->adj_nbr, hg->adj_et, &g) != 0) {
     std::cerr << smcm_last_error() << std::endl;
     return 1;
+  }
+
+  if (const char* pc = std::getenv("SMCM_PLAN_COST")) {
+    const std::string v = pc;
+    const int64_t code = v == "meta" ? 2 : v == "mnc" ? 3 : v == "dm" ? 4 : v == "lg" ? 5 : std::atoll(pc);
+    if (smcm_set_option(ctx, "plan_cost", code) != 0) {
+      std::cerr << smcm_last_error() << std::endl;
+      return 1;
+    }
   }
 
   // read_metapathesN (:339-357): stop at the first malformed line

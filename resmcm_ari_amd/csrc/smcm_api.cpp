@@ -30,6 +30,7 @@ int run_chain(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t
     *out = nullptr;
     Chain ch{ctx, g};
     ch.prepare(vtypes, etypes, L, src_rows, n_src);
+    ch.provider = ctx->plan_cost >= smcm::kProviderMetaAC ? ctx->plan_cost : 0;
     Csr C;
     run_core(ch, forced_pairs, &C, nullptr, 0, times);
     try {
@@ -147,8 +148,11 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
       if (v < 64 || v > 65535) fail(SMCM_EINVAL, "wave_bitmap_max_ub must be in [64, 65535]");
       o.wave_bitmap_max_ub = v;
     } else if (k == "plan_cost") {
-      if (v != 0 && v != 1) fail(SMCM_EINVAL, "plan_cost must be 0 (reference) or 1 (products)");
+      if (v < 0 || v > 5)
+        fail(SMCM_EINVAL, "plan_cost must be 0 (reference), 1 (products), 2 (MetaAC), 3 (MNC), 4 (DMap) or 5 (LGraph)");
       ctx->plan_cost = (int)v;
+    } else if (k == "lgraph_seed") {
+      ctx->lgraph_seed = (uint64_t)v;
     } else if (k == "batch_small_products") {
       if (v < 0) fail(SMCM_EINVAL, "batch_small_products must be >= 0");
       ctx->batch.small_products = v;
@@ -199,6 +203,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "force_tier") *v = o.force_tier;
     else if (k == "exact_cost_sum") *v = ctx->exact_cost_sum;
     else if (k == "plan_cost") *v = ctx->plan_cost;
+    else if (k == "lgraph_seed") *v = (int64_t)ctx->lgraph_seed;
     else if (k == "wave_bitmap_cols") *v = o.wave_bitmap_cols;
     else if (k == "wave_bitmap_cap") *v = o.wave_bitmap_cap;
     else if (k == "wave_bitmap_max_ub") *v = o.wave_bitmap_max_ub;
@@ -319,6 +324,7 @@ int smcm_chain_run_blocked(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, 
     if (block_bytes < 12) fail(SMCM_EINVAL, "block_bytes must hold at least one entry");
     Chain ch{ctx, g};
     ch.prepare(vtypes, etypes, L, src_rows, n_src);
+    ch.provider = ctx->plan_cost >= smcm::kProviderMetaAC ? ctx->plan_cost : 0;
     DevAlloc& al = ctx->al;
     const int32_t nrows = ch.mats[0].nrows;
     DevBuf<uint64_t> dig(al, row_digest ? nrows : 1);
@@ -397,6 +403,7 @@ int smcm_chain_plan(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const i
     check_ctx(ctx);
     Chain ch{ctx, g};
     ch.prepare(vtypes, etypes, L, src_rows, n_src);
+    ch.provider = ctx->plan_cost >= smcm::kProviderMetaAC ? ctx->plan_cost : 0;
     ch.estimate();
     ch.dp();
     const size_t LL = (size_t)L * L;
@@ -431,6 +438,26 @@ int smcm_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int
     ch.prepare(vtypes, etypes, L, src_rows, n_src);
     ch.estimate();
     *est_nnz = ch.c[L - 1];
+  });
+}
+
+int smcm_cost_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes, int32_t L,
+                       int32_t provider, int32_t accuracy, double* estimate, double* c_table, double* m_table,
+                       int32_t* s_table) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!estimate) fail(SMCM_EINVAL, "null output");
+    if (provider < smcm::kProviderMetaAC || provider > smcm::kProviderLGraph)
+      fail(SMCM_EINVAL, "provider must be 2 (MetaAC), 3 (MNC), 4 (DMap) or 5 (LGraph)");
+    Chain ch{ctx, g};
+    ch.prepare(vtypes, etypes, L, nullptr, 0);
+    smcm::ProviderOut po;
+    ch.provider_tables(provider, accuracy != 0, po);
+    *estimate = po.estimate;
+    const size_t LL = (size_t)L * L;
+    if (c_table) std::memcpy(c_table, po.c.data(), sizeof(double) * LL);
+    if (m_table) std::memcpy(m_table, po.m.data(), sizeof(double) * LL);
+    if (s_table) std::memcpy(s_table, po.s.data(), sizeof(int32_t) * LL);
   });
 }
 

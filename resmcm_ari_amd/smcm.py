@@ -33,8 +33,11 @@ EXPORTS = [
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
     "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth", "smcm_chain_products_estimate",
     "smcm_batch_run", "smcm_server_create", "smcm_server_destroy", "smcm_server_submit", "smcm_server_poll",
-    "smcm_server_stats", "smcm_server_replay", "smcm_batch_autotune",
+    "smcm_server_stats", "smcm_server_replay", "smcm_batch_autotune", "smcm_cost_estimate",
 ]
+
+# DP cost providers (option plan_cost and Graph.cost_estimate)
+PROVIDERS = {"metaac": 2, "mnc": 3, "dmap": 4, "lgraph": 5}
 
 
 class Times(C.Structure):
@@ -102,6 +105,7 @@ def load() -> C.CDLL:
             "smcm_chain_run_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp],
             "smcm_chain_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp, vp],
             "smcm_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
+            "smcm_cost_estimate": [vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp],
             "smcm_chain_products_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_batch_run": [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "smcm_batch_autotune": [vp, vp, i32, vp, vp, vp, vp, vp, i32, vp],
@@ -303,6 +307,21 @@ class Graph:
         x = C.c_double()
         _check(load().smcm_estimate(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen, _ptr(s), ns, C.byref(x)))
         return float(x.value)
+
+    def cost_estimate(self, mp: MetaPath, provider, accuracy: bool = False):
+        """One of the reference's other DP cost providers ("metaac", "mnc", "dmap", "lgraph" or
+        2-5) over the whole chain: (estimate, c, m, s) - the number the reference's accuracy
+        program prints (accuracy=True) or the planner's tables (accuracy=False)."""
+        L = mp.pathLen
+        vt, et = self._path(mp)
+        pid = PROVIDERS[provider] if isinstance(provider, str) else int(provider)
+        est = C.c_double()
+        c = np.zeros(L * L, np.float64)
+        m = np.zeros(L * L, np.float64)
+        st = np.zeros(L * L, np.int32)
+        _check(load().smcm_cost_estimate(self.ctx._h, self._h, _ptr(vt), _ptr(et), L, pid, 1 if accuracy else 0,
+                                         C.byref(est), _ptr(c), _ptr(m), _ptr(st)))
+        return float(est.value), c.reshape(L, L), m.reshape(L, L), st.reshape(L, L)
 
     def products_estimate(self, mp: MetaPath, src_rows=None) -> np.ndarray:
         """Estimated product count of every split: P[i, k, j] for X_ik * X_{k+1,j}."""

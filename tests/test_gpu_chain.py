@@ -193,3 +193,29 @@ def test_cli_matches_reference_lines(tmp_path):
         for g, w in zip(got, want):
             assert len(g) == 6
             assert g[0] == w[0] and g[1] == w[1], (name, g, w)
+
+
+@pytest.mark.parametrize("prov", ["meta", "mnc", "dm", "lg"])
+def test_cli_cost_providers(tmp_path, prov):
+    """SMCM_PLAN_COST makes the drop-in program plan with one of the reference's other cost
+    providers (efficient/{meta,mnc,dm,lg}): same argv and line format, and the nnz column is
+    the reference program's (every plan gives the same result)."""
+    import os
+    import subprocess
+
+    from resmcm_ari_amd import _build
+    import test_oracle_golden as TG
+
+    name = "bib_tiny"
+    z, hg = TG.load(name)
+    files = [str(tmp_path / f"{x}.txt") for x in ("graph", "vertex", "edge")]
+    hg.write_text(*files)
+    mpf = tmp_path / "mp.txt"
+    mpf.write_text("".join(str(p) + "\n" for p in z["paths"]))
+    out = tmp_path / "out.txt"
+    env = dict(os.environ, SMCM_PLAN_COST=prov)
+    subprocess.run([os.path.join(_build.LIBDIR, "smcm_resmcm"), "8", "1", *files, str(mpf), str(out)],
+                   check=True, stdout=subprocess.DEVNULL, env=env)
+    got = [line.split() for line in out.read_text().splitlines()]
+    want = TG.ref_lines(name, "resmcm")
+    assert [(g[0], g[1]) for g in got] == [(w[0], w[1]) for w in want]
