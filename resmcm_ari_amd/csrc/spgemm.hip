@@ -258,6 +258,7 @@ constexpr int kQuarter = 16;
 constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
 constexpr int64_t kWindowCountBytes = (int64_t)1 << 30;  // cap of the per-(row, window) count table
 
+
 struct TravLds {
   int64_t start[kChunk];  // first B index of a compacted segment (quarter segments, then long)
   uint64_t av[kChunk];    // A value of its entry
@@ -268,16 +269,27 @@ struct TravLds {
 
 // One traversal round: every participating thread brings at most one segment (st, cnt) of B
 // with the A value a of its entry (cnt = 0: none).  Ends with a barrier.
-template <bool BV, int E = 1, int STEP = 8, class F>
+template <bool BV, int E = 1, int STEP = 8, int QH = 2, class F>
 __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, const int32_t* __restrict__ bcol,
                                            const uint64_t* __restrict__ bval, TravLds& L, F&& f) {
   constexpr int kPieceLen = E * kWave;
   const int nwaves = blockDim.x / kWave;
   const int wid = threadIdx.x / kWave, lane = lane_id();
   int64_t packed = 0;  // pieces << 32 | long << 16 | quarter
+  // tiny segments: the owning thread's loads are issued before the scan's barriers and
+  // accumulated after them
+  int32_t tc[kTiny];
+  uint64_t tv[kTiny];
+  int tn = 0;
   if (cnt > 0) {
     if (cnt <= kTiny) {
-      for (int64_t x = 0; x < cnt; ++x) f(bcol[st + x], BV ? a * bval[st + x] : a);
+      tn = (int)cnt;
+#pragma unroll
+      for (int x = 0; x < kTiny; ++x)
+        if (x < tn) {
+          tc[x] = bcol[st + x];
+          tv[x] = BV ? bval[st + x] : 1ull;
+        }
     } else if (cnt <= kQuarter) {
       packed = 1;
     } else {
@@ -286,6 +298,9 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
   }
   int64_t total;
   const int64_t ex = block_exscan_i64(packed, L.red, &total);
+#pragma unroll
+  for (int x = 0; x < kTiny; ++x)
+    if (x < tn) f(tc[x], BV ? a * tv[x] : a);
   const int32_t nquarter = (int32_t)(total & 0xffff), nlong = (int32_t)((total >> 16) & 0xffff);
   const int32_t npieces = (int32_t)(total >> 32);
   if (packed & 0xffff) {
@@ -302,26 +317,29 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
   }
   if (threadIdx.x == 0) L.Q[nlong] = npieces;
   __syncthreads();
-  // quarter segments, four per wave step (lane / 16 selects the segment)
+  // quarter segments, four per wave step (lane / 16 selects the segment), QH steps of loads in
+  // flight per lane
   const int q = lane >> 4, x = lane & 15;
-  for (int32_t g = wid; 4 * g < nquarter; g += 2 * nwaves) {
-    int32_t c[2];
-    uint64_t v[2], av2[2];
-    bool ok[2];
+  for (int32_t g = wid; 4 * g < nquarter; g += QH * nwaves) {
+    int32_t c[QH];
+    uint64_t v[QH];
+    bool ok[QH];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < QH; ++h) {
       const int32_t sidx = 4 * (g + h * nwaves) + q;
       ok[h] = sidx < nquarter && x < L.len[sidx];
       if (ok[h]) {
         const int64_t b = L.start[sidx] + x;
         c[h] = bcol[b];
         v[h] = BV ? bval[b] : 1ull;
-        av2[h] = L.av[sidx];
       }
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-      if (ok[h]) f(c[h], BV ? av2[h] * v[h] : av2[h]);
+    for (int h = 0; h < QH; ++h)
+      if (ok[h]) {
+        const uint64_t av2 = L.av[4 * (g + h * nwaves) + q];
+        f(c[h], BV ? av2 * v[h] : av2);
+      }
   }
   // pieces of the long segments (E elements per lane), STEP pieces per wave step: E * STEP loads
   // in flight per lane
@@ -1105,6 +1123,10 @@ __global__ void __launch_bounds__(kThreads) window_prep_kernel(const int32_t* __
   }
 }
 
+// Tasks are software-pipelined: the task after next is claimed, and the next task's metadata
+// loaded, while the current task runs; the next task's first-round A entries are loaded while the
+// current task writes its output.  Only the B-row bounds and the gathers are left on a task's
+// critical path.
 template <bool AV, bool BV>
 __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
                                     const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
@@ -1119,40 +1141,69 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
   uint32_t* pres = (uint32_t*)(smem + (size_t)W * 8);
   uint64_t* pres64 = (uint64_t*)pres;
   __shared__ TravLds L;
-  __shared__ int32_t claim;
-  __shared__ int64_t meta[4];  // A extent, output offset and column count of the current task
+  __shared__ int32_t cur_t, next_t;       // current task, next task (claimed ahead)
+  __shared__ int64_t cur_m[4], next_m[4]; // A extent, output offset, column count
   __shared__ int32_t wcw[1024 / kWave];
   const int nthreads = blockDim.x, nwaves = nthreads / kWave;
   const int lane = lane_id(), wid = threadIdx.x / kWave;
   const int chunk = min(kChunk, nthreads);
   const uint64_t below = (1ull << lane) - 1ull;
   const int64_t ntasks = (int64_t)nrows * nwin;
+  // metadata of task t (thread 0): row extent, output offset of the window, its column count
+  auto task_meta = [&](int32_t t, int64_t* m) {
+    if (t >= ntasks) {
+      m[0] = m[1] = m[2] = m[3] = 0;
+      return;
+    }
+    const int32_t w = t / nrows, ri = t - w * nrows;
+    const WinRow r = info[ri];
+    const int32_t* p = pre + (size_t)ri * (nwin + 1) + w;
+    const int32_t p0 = p[0], p1 = p[1];
+    m[0] = r.rs;
+    m[1] = r.re;
+    m[2] = r.base + p0;
+    m[3] = p1 - p0;
+  };
   for (int i = threadIdx.x; i < W; i += nthreads) vals[i] = 0ull;
   for (int i = threadIdx.x; i < W / 32; i += nthreads) pres[i] = 0u;
-  if (threadIdx.x == 0) claim = atomicAdd(queue, 1);
+  if (threadIdx.x == 0) {
+    const int32_t t0 = atomicAdd(queue, 1);
+    int64_t m[4];
+    task_meta(t0, m);
+    cur_t = t0;
+    for (int q = 0; q < 4; ++q) cur_m[q] = m[q];
+    next_t = atomicAdd(queue, 1);
+  }
   __syncthreads();
+  bool have_pref = false;  // the first round's entries of the current task are in registers
+  int32_t pk = 0;          // prefetched entry: B row, A value
+  uint64_t pa = 1ull;
+  bool pvalid = false;
   while (true) {
-    const int32_t t = uniform_i32(claim);
+    const int32_t t = uniform_i32(cur_t);
     if (t >= ntasks) break;
+    const int32_t tn = uniform_i32(next_t);
+    const int64_t rs = uniform_i64(cur_m[0]), re = uniform_i64(cur_m[1]), rowbase = uniform_i64(cur_m[2]);
+    const int32_t target = (int32_t)uniform_i64(cur_m[3]);
     const int32_t w = t / nrows;
-    __syncthreads();  // every thread has read `claim`
+    __syncthreads();  // every thread has read the current task
+    // thread 0: claim the task after next and load the next task's metadata (registers)
+    int32_t tnn = 0;
+    int64_t nm[4] = {0, 0, 0, 0};
     if (threadIdx.x == 0) {
-      // the next task is claimed now, so that its round trip overlaps this task's metadata loads
-      const int32_t nx = atomicAdd(queue, 1);
-      const int32_t ri = t - w * nrows;
-      const WinRow r = info[ri];
-      const int32_t* p = pre + (size_t)ri * (nwin + 1) + w;
-      const int32_t p0 = p[0], p1 = p[1];
-      meta[0] = r.rs;
-      meta[1] = r.re;
-      meta[2] = r.base + p0;
-      meta[3] = p1 - p0;
-      claim = nx;
+      tnn = atomicAdd(queue, 1);
+      task_meta(tn, nm);
     }
-    __syncthreads();
-    const int64_t rs = uniform_i64(meta[0]), re = uniform_i64(meta[1]), rowbase = uniform_i64(meta[2]);
-    const int32_t target = (int32_t)uniform_i64(meta[3]);
-    if (target == 0) continue;
+    if (target == 0) {
+      if (threadIdx.x == 0) {
+        cur_t = tn;
+        for (int q = 0; q < 4; ++q) cur_m[q] = nm[q];
+        next_t = tnn;
+      }
+      __syncthreads();
+      have_pref = false;
+      continue;
+    }
     const int64_t w0 = (int64_t)w << log2w;
     const int32_t* b_lo = bnd + (size_t)w * nbrows;
     const int32_t* b_hi = b_lo + nbrows;
@@ -1166,13 +1217,24 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
       for (int64_t base = rs; base < re; base += chunk) {
         int64_t st = 0, cnt = 0;
         uint64_t a = 1ull;
-        const int64_t e = base + threadIdx.x;
-        if ((int)threadIdx.x < chunk && e < re) {
-          const int32_t k = acol[e];
+        bool valid;
+        int32_t k = 0;
+        if (pass == 0 && base == rs && have_pref) {
+          valid = pvalid;
+          k = pk;
+          a = pa;
+        } else {
+          const int64_t e = base + threadIdx.x;
+          valid = (int)threadIdx.x < chunk && e < re;
+          if (valid) {
+            k = acol[e];
+            a = AV ? aval[e] : 1ull;
+          }
+        }
+        if (valid) {
           const int32_t s0 = b_lo[k];
           cnt = b_hi[k] - s0;
           st = bptr[k] + s0;
-          a = AV ? aval[e] : 1ull;
         }
         trav_round<BV>(st, cnt, a, bcol, bval, L, acc);
       }
@@ -1187,6 +1249,8 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
         mine += __popcll(m);
       }
       if (lane == 0) wcw[wid] = mine;
+      if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) next_m[q] = nm[q];
       __syncthreads();
       int32_t woff = 0, wtot = 0;
       for (int v2 = 0; v2 < nwaves; ++v2) {
@@ -1195,6 +1259,18 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
         wtot += cw;
       }
       const bool write = with_pres || wtot == target;
+      if (write) {
+        // the next task's first-round entries: their loads overlap this task's output writes
+        const int64_t nrs = uniform_i64(next_m[0]), nre = uniform_i64(next_m[1]);
+        const int64_t e = nrs + threadIdx.x;
+        pvalid = (int)threadIdx.x < chunk && e < nre;
+        pk = 0;
+        pa = 1ull;
+        if (pvalid) {
+          pk = acol[e];
+          pa = AV ? aval[e] : 1ull;
+        }
+      }
       int64_t pos0 = woff;
       for (int32_t j = ja; j < jb; ++j) {
         const int32_t xi = j * 64 + lane;
@@ -1211,11 +1287,111 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
         if (with_pres && lane == 0) pres64[j] = 0ull;
         pos0 += __popcll(m);
       }
+      if (write && threadIdx.x == 0) {
+        // advance: the next task becomes current (every thread read the current one at the top)
+        cur_t = tn;
+        for (int q = 0; q < 4; ++q) cur_m[q] = nm[q];
+        next_t = tnn;
+      }
       // the next task's first accumulations (inline tiny segments) precede its first barrier:
       // every wave must be done with this window's slots before any wave moves on
       __syncthreads();
       if (write) break;
     }
+    have_pref = true;
+  }
+}
+
+// Window-major symbolic pass for the heavy rows: task t = w * nrows + ri counts the distinct
+// columns of row rows[ri] in window w with an LDS bitmap of the window, writes the count to
+// wcnt[u * nwin + w] (the window tier's output offsets) and adds it to the row's width.  Tasks
+// run window by window for the same reason as in the numeric pass: the window's slice of B stays
+// in L2.
+struct SymRow {
+  int64_t rs, re;
+  int32_t u, pad;
+};
+
+__global__ void __launch_bounds__(kThreads) sym_prep_kernel(const int32_t* __restrict__ rows, int32_t nrows,
+                                                            const int64_t* __restrict__ aptr,
+                                                            SymRow* __restrict__ info) {
+  for (int64_t ri = (int64_t)blockIdx.x * kThreads + threadIdx.x; ri < nrows; ri += (int64_t)gridDim.x * kThreads) {
+    const int32_t u = rows[ri];
+    info[ri] = SymRow{aptr[u], aptr[u + 1], u, 0};
+  }
+}
+
+// A symbolic window spans 2^sub numeric windows (2^(log2w + sub) columns): fewer, larger tasks;
+// the count of every numeric window inside it is taken from its part of the bitmap.
+__global__ void window_symbolic_kernel(const int32_t* __restrict__ acol, const int64_t* __restrict__ bptr,
+                                       const int32_t* __restrict__ bcol, const SymRow* __restrict__ info,
+                                       int32_t nrows, int log2w, int sub, int32_t nwin,
+                                       const int32_t* __restrict__ bnd, int32_t nbrows, int32_t* __restrict__ queue,
+                                       int32_t* __restrict__ wcnt, int64_t* __restrict__ row_nnz) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* bm = (uint32_t*)smem;
+  const int32_t R = 1 << sub;                   // numeric windows per symbolic window
+  const int32_t nw32 = (1 << (log2w + sub)) / 32;
+  const int32_t sub32 = (1 << log2w) / 32;      // bitmap words per numeric window
+  const int32_t nwin_s = (nwin + R - 1) / R;
+  __shared__ TravLds L;
+  __shared__ int32_t claim;
+  __shared__ int64_t meta[3];
+  const int nthreads = blockDim.x, nwaves = nthreads / kWave;
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int chunk = min(kChunk, nthreads);
+  const int64_t ntasks = (int64_t)nrows * nwin_s;
+  for (int i = threadIdx.x; i < nw32; i += nthreads) bm[i] = 0u;
+  if (threadIdx.x == 0) claim = atomicAdd(queue, 1);
+  __syncthreads();
+  while (true) {
+    const int32_t t = uniform_i32(claim);
+    if (t >= ntasks) break;
+    const int32_t ws = t / nrows;
+    __syncthreads();  // every thread has read `claim`
+    if (threadIdx.x == 0) {
+      const int32_t nx = atomicAdd(queue, 1);
+      const SymRow r = info[t - ws * nrows];
+      meta[0] = r.rs;
+      meta[1] = r.re;
+      meta[2] = r.u;
+      claim = nx;
+    }
+    __syncthreads();
+    const int64_t rs = uniform_i64(meta[0]), re = uniform_i64(meta[1]);
+    const int32_t u = (int32_t)uniform_i64(meta[2]);
+    const int32_t wa = ws * R, wb = min(wa + R, nwin);  // numeric windows [wa, wb)
+    const int64_t w0 = (int64_t)wa << log2w;
+    const int32_t* b_lo = bnd + (size_t)wa * nbrows;
+    const int32_t* b_hi = bnd + (size_t)wb * nbrows;
+    for (int64_t base = rs; base < re; base += chunk) {
+      int64_t st = 0, cnt = 0;
+      const int64_t e = base + threadIdx.x;
+      if ((int)threadIdx.x < chunk && e < re) {
+        const int32_t k = acol[e];
+        const int32_t s0 = b_lo[k];
+        cnt = b_hi[k] - s0;
+        st = bptr[k] + s0;
+      }
+      trav_round<false, 4, 2>(st, cnt, 1ull, bcol, nullptr, L, [&](int32_t c, uint64_t) {
+        const int32_t x = (int32_t)(c - w0);
+        atomicOr(&bm[x >> 5], 1u << (x & 31));
+      });
+    }
+    // count per numeric window (a wave each) and clear; trav_round ended with a barrier
+    int64_t total = 0;
+    for (int32_t x = wid; x < wb - wa; x += nwaves) {
+      int32_t c = 0;
+      for (int32_t i = lane; i < sub32; i += kWave) {
+        c += __popc(bm[x * sub32 + i]);
+        bm[x * sub32 + i] = 0u;
+      }
+      c = (int32_t)wave_sum_i64(c);
+      if (lane == 0 && c > 0) wcnt[(size_t)u * nwin + wa + x] = c;
+      total += c;
+    }
+    if (lane == 0 && total > 0) atomicAdd((unsigned long long*)&row_nnz[u], (unsigned long long)total);
+    __syncthreads();  // the bitmap is clear before the next task's first accumulations
   }
 }
 
@@ -1336,18 +1512,45 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     SMCM_LAUNCH_CHECK();
     T.stop(tok, st, tier_bytes(NUMERIC, tc, kSortTier, AV, BV));
   }
-  if (nB + nH > 0 && !NUMERIC) {
+  // heavy rows: window-major symbolic pass (needs the per-(row, window) count table)
+  const bool sym_window = !NUMERIC && split->wcnt && nH > 0 && o.window_major;
+  if (sym_window) {
+    if (!split->bnd) split->build_bounds(B, st);
+    const int32_t nwin = (int32_t)split->wl.nwin;
+    DevBuf<SymRow> info(*split->al, nH);
+    sym_prep_kernel<<<(unsigned)std::min<int64_t>(ceil_div(nH, kThreads), 65536), kThreads, 0, st>>>(
+        bins + (size_t)kHeavyTier * m, nH, A.ptr, info.p);
+    SMCM_LAUNCH_CHECK();
+    fill_i32(queue, 1, 0, st);
+    int sub = 0;
+    while (((int64_t)split->wl.w << (sub + 1)) <= o.sym_window_cols && (1 << (sub + 1)) <= nwin) ++sub;
+    const size_t lds = ((size_t)split->wl.w << sub) / 8;
+    int nb = 0;
+    SMCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, window_symbolic_kernel, o.d_threads, lds));
+    const unsigned g = (unsigned)std::max<int64_t>(1, (int64_t)o.cus * std::max(nb, 1));
+    TierCounts heavy;
+    heavy.rows[0] = nH;
+    heavy.a_nnz[0] = tc.a_nnz[kHeavyTier];
+    heavy.products[0] = tc.products[kHeavyTier];
+    const int tok = T.start("spgemm.symbolic.window", st);
+    window_symbolic_kernel<<<g, o.d_threads, lds, st>>>(A.col, B.ptr, B.col, info.p, nH, split->wl.log2w, sub,
+                                                        nwin, split->bnd, B.nrows, queue, split->wcnt, row_nnz);
+    SMCM_LAUNCH_CHECK();
+    T.stop(tok, st, tier_bytes(false, heavy, 0, AV, BV));
+  }
+  const int32_t nHb = sym_window ? 0 : nH;  // heavy rows left to the bitmap kernel
+  if (nB + nHb > 0 && !NUMERIC) {
     fill_i32(queue, 1, 0, st);
     const size_t lds = lay.bytes_sym();
     const int per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / o.d_threads, o.lds_bytes / (int64_t)(lds + 12 * 1024)));
-    const unsigned g = (unsigned)std::min<int64_t>(nB + nH, (int64_t)o.cus * per_cu);
+    const unsigned g = (unsigned)std::min<int64_t>(nB + nHb, (int64_t)o.cus * per_cu);
     TierCounts both;
-    both.rows[0] = nB + nH;
-    both.a_nnz[0] = tc.a_nnz[kBitmapTier] + tc.a_nnz[kHeavyTier];
-    both.products[0] = tc.products[kBitmapTier] + tc.products[kHeavyTier];
+    both.rows[0] = nB + nHb;
+    both.a_nnz[0] = tc.a_nnz[kBitmapTier] + (nHb ? tc.a_nnz[kHeavyTier] : 0);
+    both.products[0] = tc.products[kBitmapTier] + (nHb ? tc.products[kHeavyTier] : 0);
     const int tok = T.start("spgemm.symbolic.bitmap", st);
     bitmap_tier_kernel<NUMERIC, AV, BV><<<g, o.d_threads, lds, st>>>(
-        A.ptr, A.col, A.val, B.ptr, B.col, B.val, bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m,
+        A.ptr, A.col, A.val, B.ptr, B.col, B.val, bins + (size_t)kHeavyTier * m, nHb, bins + (size_t)kBitmapTier * m,
         nB, B.ncols, lay.nw, lay.vcap, queue, row_nnz, cptr, ccol, cval, split->wl.log2w, (int32_t)split->wl.nwin,
         split->wcnt);
     SMCM_LAUNCH_CHECK();
