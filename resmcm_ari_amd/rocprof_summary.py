@@ -32,6 +32,7 @@ TIMED = {
     "spgemm.numeric.bitmap": r"bitmap_tier_kernel<true",
     "spgemm.numeric.bitmap.hbm": None,  # the same kernel as spgemm.numeric.bitmap (values in HBM)
     "spgemm.symbolic.bitmap": r"bitmap_tier_kernel<false",
+    "spgemm.symbolic.window": r"window_symbolic_kernel",
     "spgemm.numeric.wavebitmap": r"wave_bitmap_kernel<true",
     "spgemm.symbolic.wavebitmap": r"wave_bitmap_kernel<false",
     "batch.executor": r"batch_kernel<",
