@@ -146,7 +146,7 @@ def test_spgemm_wraparound(ctx):
 @pytest.mark.parametrize("mode", list(WIDE_MODES))
 @pytest.mark.parametrize("window", [64, 256, 0])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("vals", ["none", "AB"])
+@pytest.mark.parametrize("vals", ["none", "AB", "AB32"])
 def test_spgemm_window_tier(ctx, case, window, vals, mode):
     """Every non-empty row through the bitmap tier with no LDS value slots: the numeric pass
     runs in the window kernel (dense LDS windows) in window-major task order (per-(row, window)
@@ -154,8 +154,11 @@ def test_spgemm_window_tier(ctx, case, window, vals, mode):
     accumulated in HBM."""
     name, m, k, n, la, lb = case
     rng = np.random.default_rng(zlib.crc32(f"win/{name}/{window}/{vals}".encode()))
-    A = random_csr(rng, m, k, row_len=la, with_vals=vals == "AB")
-    B = random_csr(rng, k, n, row_len=lb, with_vals=vals == "AB")
+    A = random_csr(rng, m, k, row_len=la, with_vals=vals != "none")
+    B = random_csr(rng, k, n, row_len=lb, with_vals=vals != "none")
+    if vals == "AB32":
+        # every value of B below 2^32: the window kernel reads B as 8-byte (column, value) pairs
+        B.val = (B.val >> np.uint64(33)).astype(np.uint64)
     ref = O.spgemm(A, B)
     with options(ctx, force_tier=2, bitmap_vcap=0, window_cols=window, **WIDE_MODES[mode]):
         ptr, col, val = _run(ctx, A, B)
