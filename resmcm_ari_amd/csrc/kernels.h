@@ -137,6 +137,8 @@ struct SpgemmOpts {
                             // rows wider than this use the window kernel
   int64_t lds_bytes = 65536;   // LDS available per workgroup
   int cus = 256;           // compute units (persistent grid size)
+  int device_cus = 0;      // compute units of the device; cus below it reserves the rest for other
+                           // streams, so the persistent window kernels then place one workgroup per CU
   int force_tier = -1;     // testing: 0 = wave, 1 = LDS sort, 2 = bitmap for every non-empty row
   int64_t wave_bitmap_cols = 32768;    // rows of products with at most this many columns and
   int64_t wave_bitmap_max_ub = 2048;   // 64 < ub <= wave_bitmap_max_ub: one wave per row over an LDS

@@ -512,6 +512,9 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
         // the persistent (queue-driven) kernels of large queries are sized for this share of
         // the CUs, so their workgroups leave room for the kernels of short queries
         w->opts.cus = std::max<int32_t>(1, ctx->opts.cus * o.large_cu_percent / 100);
+        // one window-tier workgroup per CU of the share: the largest window and workgroup
+        w->opts.w_threads = 1024;
+        w->opts.window_cols = 16384;
       }
     }
     // lanes only when both kinds of workers exist

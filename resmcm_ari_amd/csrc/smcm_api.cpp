@@ -72,6 +72,7 @@ int smcm_ctx_create(int device, uint64_t hbm_budget, smcm_ctx** out) {
     SMCM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     ctx->opts.lds_bytes = lds;
     ctx->opts.cus = cus;
+    ctx->opts.device_cus = cus;
     size_t freeb = 0, totalb = 0;
     SMCM_HIP(hipMemGetInfo(&freeb, &totalb));
     ctx->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)freeb);

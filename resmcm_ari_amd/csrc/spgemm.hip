@@ -27,6 +27,10 @@
 //            into the row's (pre-zeroed) output slots.
 // Products of an A entry are visited by the thread owning the entry when its B row is short,
 // or by a whole wave (coalesced B loads) when it is long.
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "kernels.h"
 
 #include <algorithm>
@@ -257,6 +261,30 @@ constexpr int kQuarter = 16;
 
 constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
 constexpr int64_t kWindowCountBytes = (int64_t)1 << 30;  // cap of the per-(row, window) count table
+
+// Resident workgroups per CU of a kernel at a block size and dynamic LDS size (cached: the
+// runtime query is made once per combination).
+int occupancy(const void* kernel, int threads, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, int, size_t>, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(kernel, threads, lds);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  SMCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, lds));
+  nb = std::max(nb, 1);
+  cache.emplace(key, nb);
+  return nb;
+}
+
+// persistent grid of the window kernels: every CU filled, or, when the options reserve part of
+// the device for other streams, one workgroup per CU of the share (the dispatcher spreads
+// workgroups over all CUs, so several per CU would reach into the reserved part)
+unsigned persistent_grid(const SpgemmOpts& o, int per_cu) {
+  const bool shared = o.device_cus > 0 && o.cus < o.device_cus;
+  return (unsigned)std::max<int64_t>(1, (int64_t)o.cus * (shared ? 1 : per_cu));
+}
 
 
 struct TravLds {
@@ -1525,9 +1553,7 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     int sub = 0;
     while (((int64_t)split->wl.w << (sub + 1)) <= o.sym_window_cols && (1 << (sub + 1)) <= nwin) ++sub;
     const size_t lds = ((size_t)split->wl.w << sub) / 8;
-    int nb = 0;
-    SMCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, window_symbolic_kernel, o.d_threads, lds));
-    const unsigned g = (unsigned)std::max<int64_t>(1, (int64_t)o.cus * std::max(nb, 1));
+    const unsigned g = persistent_grid(o, occupancy((const void*)window_symbolic_kernel, o.d_threads, lds));
     TierCounts heavy;
     heavy.rows[0] = nH;
     heavy.a_nnz[0] = tc.a_nnz[kHeavyTier];
@@ -1627,9 +1653,7 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
                              st>>>(split->wide, cnt[1], A.ptr, cptr, split->wcnt, nwin, info.p, pre.p);
         SMCM_LAUNCH_CHECK();
         auto kern = window_major_kernel<AV, BV>;
-        int nb = 0;
-        SMCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, o.w_threads, lds));
-        const unsigned gm = (unsigned)std::max<int64_t>(1, (int64_t)o.cus * std::max(nb, 1));
+        const unsigned gm = persistent_grid(o, occupancy((const void*)kern, o.w_threads, lds));
         kern<<<gm, o.w_threads, lds, st>>>(A.col, A.val, B.ptr, B.col, B.val, info.p, pre.p, cnt[1], B.ncols,
                                            split->wl.log2w, nwin, split->bnd, B.nrows, queue, ccol, cval);
       } else {
