@@ -115,3 +115,25 @@ def test_plan_cost_runs_provider_plans(ctx, graphs, prov):
 def test_plan_cost_rejects_unknown(ctx):
     with pytest.raises(smcm.SmcmError):
         ctx.set_option("plan_cost", 6)
+
+
+@pytest.mark.parametrize("prov", list(RTOL))
+def test_provider_edge_cases(ctx, graphs, prov):
+    """One-matrix chains, a relation with no entries (edge graph: V-P-T has no P-T entries for
+    some papers; P-A on the isolated paper), and the estimate-only variants agree with the oracle."""
+    name = "edge"
+    hg = graph_of(name)
+    G = graphs(name)
+    ctx.set_option("lgraph_seed", SEED)
+    for path in ("0-0-1", "2-4-0-2-3", "0-2-3-5-0-0-1", "1-3-0-0-1-3-0-0-1"):
+        mp = MetaPath(path)
+        for accuracy in (False, True):
+            est, c, m, s = G.cost_estimate(mp, prov, accuracy=accuracy)
+            ref = oracle_run(prov, hg, mp, accuracy)
+            close(est, ref["estimate"], RTOL[prov], f"{path} {prov} accuracy={accuracy}")
+            L = mp.pathLen
+            if accuracy and prov == "lgraph":
+                continue
+            for i in range(L):
+                for j in range(i + 1, L):
+                    assert int(s[i, j]) == int(ref["s"][i, j]), f"{path} {prov}: s[{i},{j}]"
