@@ -317,7 +317,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--block-gb", type=float, default=96.0)
-    ap.add_argument("--cpu-sample", type=float, default=0.02, help="fraction of source rows for the CPU baseline")
+    ap.add_argument("--cpu-sample", type=float, default=0.06,
+                    help="fraction of source rows for the CPU baseline (about 10 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains", default=",".join(CHAINS))
     ap.add_argument("--no-side", action="store_true", help="skip the config-1, batched and served measurements")

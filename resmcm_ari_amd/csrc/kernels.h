@@ -145,6 +145,9 @@ struct SpgemmOpts {
                                        // column bitmap (cols 0 = never)
   int64_t wave_bitmap_cap = 2048;      // LDS value slots per wave; wider rows accumulate in HBM
   int window_major = 1;     // window tier in window-major task order (0 = one row at a time)
+  int window_presence = 2;  // window-major tasks track occupied columns with presence bits from the
+                            // start (1), take the nonzero values as the columns (0), or choose by
+                            // the expected window fill (2: presence below 1/8)
   int64_t sym_window_cols = 262144;  // heavy rows' symbolic pass: columns per window-major task
   int64_t window_min_products = 128;   // rows wider than the bitmap tier's LDS value slots use the
                                        // window kernel when they average at least this many

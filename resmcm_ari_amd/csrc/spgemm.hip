@@ -1155,7 +1155,7 @@ __global__ void __launch_bounds__(kThreads) window_prep_kernel(const int32_t* __
 // loaded, while the current task runs; the next task's first-round A entries are loaded while the
 // current task writes its output.  Only the B-row bounds and the gathers are left on a task's
 // critical path.
-template <bool AV, bool BV>
+template <bool AV, bool BV, bool PRES>
 __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
                                     const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
                                     const uint64_t* __restrict__ bval, const WinRow* __restrict__ info,
@@ -1235,7 +1235,8 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
     const int64_t w0 = (int64_t)w << log2w;
     const int32_t* b_lo = bnd + (size_t)w * nbrows;
     const int32_t* b_hi = b_lo + nbrows;
-    for (int pass = 0; pass < 2; ++pass) {
+    // PRES: presence bits from the start (one pass; the compaction touches only occupied slots)
+    for (int pass = PRES ? 1 : 0; pass < 2; ++pass) {
       const bool with_pres = pass == 1;
       auto acc = [&](int32_t c, uint64_t v) {
         const int32_t x = (int32_t)(c - w0);
@@ -1247,7 +1248,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
         uint64_t a = 1ull;
         bool valid;
         int32_t k = 0;
-        if (pass == 0 && base == rs && have_pref) {
+        if (pass == (PRES ? 1 : 0) && base == rs && have_pref) {
           valid = pvalid;
           k = pk;
           a = pa;
@@ -1302,18 +1303,32 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
       int64_t pos0 = woff;
       for (int32_t j = ja; j < jb; ++j) {
         const int32_t xi = j * 64 + lane;
-        const uint64_t v = vals[xi];
-        const uint64_t m = with_pres ? pres64[j] : __ballot(v != 0ull);
-        if (write && ((m >> lane) & 1ull)) {
-          const int64_t slot = pos0 + __popcll(m & below);
-          if (slot < target) {
-            ccol[rowbase + slot] = (int32_t)(w0 + xi);
-            cval[rowbase + slot] = v;
+        if (with_pres) {
+          const uint64_t m = pres64[j];
+          if ((m >> lane) & 1ull) {
+            const uint64_t v = vals[xi];
+            const int64_t slot = pos0 + __popcll(m & below);
+            if (write && slot < target) {
+              ccol[rowbase + slot] = (int32_t)(w0 + xi);
+              cval[rowbase + slot] = v;
+            }
+            vals[xi] = 0ull;
           }
+          if (lane == 0) pres64[j] = 0ull;
+          pos0 += __popcll(m);
+        } else {
+          const uint64_t v = vals[xi];
+          const uint64_t m = __ballot(v != 0ull);
+          if (write && ((m >> lane) & 1ull)) {
+            const int64_t slot = pos0 + __popcll(m & below);
+            if (slot < target) {
+              ccol[rowbase + slot] = (int32_t)(w0 + xi);
+              cval[rowbase + slot] = v;
+            }
+          }
+          if (m) vals[xi] = 0ull;
+          pos0 += __popcll(m);
         }
-        if (m) vals[xi] = 0ull;
-        if (with_pres && lane == 0) pres64[j] = 0ull;
-        pos0 += __popcll(m);
       }
       if (write && threadIdx.x == 0) {
         // advance: the next task becomes current (every thread read the current one at the top)
@@ -1652,7 +1667,11 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
         window_prep_kernel<<<(unsigned)std::min<int64_t>(ceil_div(cnt[1], kThreads / kWave), 65536), kThreads, 0,
                              st>>>(split->wide, cnt[1], A.ptr, cptr, split->wcnt, nwin, info.p, pre.p);
         SMCM_LAUNCH_CHECK();
-        auto kern = window_major_kernel<AV, BV>;
+        // presence bits pay off when windows are sparsely filled (the compaction then touches few
+        // slots); dense windows are cheaper with values alone (no per-product bit update)
+        const double fill = (double)cls.c_nnz[1] / ((double)cnt[1] * nwin * (double)split->wl.w);
+        const bool pres = o.window_presence == 1 || (o.window_presence == 2 && fill < 0.125);
+        auto kern = pres ? window_major_kernel<AV, BV, true> : window_major_kernel<AV, BV, false>;
         const unsigned gm = persistent_grid(o, occupancy((const void*)kern, o.w_threads, lds));
         kern<<<gm, o.w_threads, lds, st>>>(A.col, A.val, B.ptr, B.col, B.val, info.p, pre.p, cnt[1], B.ncols,
                                            split->wl.log2w, nwin, split->bnd, B.nrows, queue, ccol, cval);
@@ -1690,8 +1709,10 @@ void set_lds_attr(size_t bytes) {
   if constexpr (NUMERIC) {
     (void)hipFuncSetAttribute((const void*)window_tier_kernel<AV, BV>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)bytes);
-    (void)hipFuncSetAttribute((const void*)window_major_kernel<AV, BV>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
+    (void)hipFuncSetAttribute((const void*)window_major_kernel<AV, BV, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute((const void*)window_major_kernel<AV, BV, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   }
 }
 

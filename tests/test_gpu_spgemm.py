@@ -13,13 +13,16 @@ pytestmark = pytest.mark.gpu
 
 # window-tier routes for rows wider than the bitmap tier's LDS value slots
 WIDE_MODES = {
-    "window_major": {"window_major": 1, "window_min_products": 0},
+    "window_major": {"window_major": 1, "window_min_products": 0, "window_presence": 0},
     "row_major": {"window_major": 0, "window_min_products": 0},
     "bitmap_hbm": {"window_min_products": 1 << 62},
     # every bitmap-tier row heavy: the symbolic pass runs window-major too
     "window_major_heavy": {"window_major": 1, "window_min_products": 0, "heavy_ub": 0},
     # ... with one numeric window per symbolic task
     "window_major_heavy_w1": {"window_major": 1, "window_min_products": 0, "heavy_ub": 0, "sym_window_cols": 64},
+    # presence bits from the start (one pass per window task)
+    "window_major_presence": {"window_major": 1, "window_min_products": 0, "window_presence": 1},
+    "window_major_auto": {"window_major": 1, "window_min_products": 0, "window_presence": 2},
 }
 
 
