@@ -1169,8 +1169,11 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
   uint32_t* pres = (uint32_t*)(smem + (size_t)W * 8);
   uint64_t* pres64 = (uint64_t*)pres;
   __shared__ TravLds L;
-  __shared__ int32_t cur_t, next_t;       // current task, next task (claimed ahead)
-  __shared__ int64_t cur_m[4], next_m[4]; // A extent, output offset, column count
+  // task records by parity: slot p holds the current task, slot p ^ 1 receives the next one
+  // (id, A extent, output offset, column count, and the id of the task after it) before the
+  // compaction barrier, so no barrier is needed between tasks
+  __shared__ int32_t slot_t[2], slot_nt[2];
+  __shared__ int64_t slot_m[2][4];
   __shared__ int32_t wcw[1024 / kWave];
   const int nthreads = blockDim.x, nwaves = nthreads / kWave;
   const int lane = lane_id(), wid = threadIdx.x / kWave;
@@ -1198,23 +1201,26 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
     const int32_t t0 = atomicAdd(queue, 1);
     int64_t m[4];
     task_meta(t0, m);
-    cur_t = t0;
-    for (int q = 0; q < 4; ++q) cur_m[q] = m[q];
-    next_t = atomicAdd(queue, 1);
+    slot_t[0] = t0;
+    for (int q = 0; q < 4; ++q) slot_m[0][q] = m[q];
+    slot_nt[0] = atomicAdd(queue, 1);
   }
   __syncthreads();
+  int par = 0;
   bool have_pref = false;  // the first round's entries of the current task are in registers
   int32_t pk = 0;          // prefetched entry: B row, A value
   uint64_t pa = 1ull;
   bool pvalid = false;
   while (true) {
-    const int32_t t = uniform_i32(cur_t);
+    // (slot par was published before the previous task's compaction barrier or a skip barrier;
+    // it is rewritten two tasks later, after this task's traversal barriers)
+    const int32_t t = uniform_i32(slot_t[par]);
     if (t >= ntasks) break;
-    const int32_t tn = uniform_i32(next_t);
-    const int64_t rs = uniform_i64(cur_m[0]), re = uniform_i64(cur_m[1]), rowbase = uniform_i64(cur_m[2]);
-    const int32_t target = (int32_t)uniform_i64(cur_m[3]);
+    const int32_t tn = uniform_i32(slot_nt[par]);
+    const int64_t rs = uniform_i64(slot_m[par][0]), re = uniform_i64(slot_m[par][1]);
+    const int64_t rowbase = uniform_i64(slot_m[par][2]);
+    const int32_t target = (int32_t)uniform_i64(slot_m[par][3]);
     const int32_t w = t / nrows;
-    __syncthreads();  // every thread has read the current task
     // thread 0: claim the task after next and load the next task's metadata (registers)
     int32_t tnn = 0;
     int64_t nm[4] = {0, 0, 0, 0};
@@ -1224,11 +1230,12 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
     }
     if (target == 0) {
       if (threadIdx.x == 0) {
-        cur_t = tn;
-        for (int q = 0; q < 4; ++q) cur_m[q] = nm[q];
-        next_t = tnn;
+        slot_t[par ^ 1] = tn;
+        for (int q = 0; q < 4; ++q) slot_m[par ^ 1][q] = nm[q];
+        slot_nt[par ^ 1] = tnn;
       }
       __syncthreads();
+      par ^= 1;
       have_pref = false;
       continue;
     }
@@ -1278,8 +1285,11 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
         mine += __popcll(m);
       }
       if (lane == 0) wcw[wid] = mine;
-      if (threadIdx.x == 0)
-        for (int q = 0; q < 4; ++q) next_m[q] = nm[q];
+      if (threadIdx.x == 0) {
+        slot_t[par ^ 1] = tn;
+        for (int q = 0; q < 4; ++q) slot_m[par ^ 1][q] = nm[q];
+        slot_nt[par ^ 1] = tnn;
+      }
       __syncthreads();
       int32_t woff = 0, wtot = 0;
       for (int v2 = 0; v2 < nwaves; ++v2) {
@@ -1290,7 +1300,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
       const bool write = with_pres || wtot == target;
       if (write) {
         // the next task's first-round entries: their loads overlap this task's output writes
-        const int64_t nrs = uniform_i64(next_m[0]), nre = uniform_i64(next_m[1]);
+        const int64_t nrs = uniform_i64(slot_m[par ^ 1][0]), nre = uniform_i64(slot_m[par ^ 1][1]);
         const int64_t e = nrs + threadIdx.x;
         pvalid = (int)threadIdx.x < chunk && e < nre;
         pk = 0;
@@ -1330,17 +1340,13 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
           pos0 += __popcll(m);
         }
       }
-      if (write && threadIdx.x == 0) {
-        // advance: the next task becomes current (every thread read the current one at the top)
-        cur_t = tn;
-        for (int q = 0; q < 4; ++q) cur_m[q] = nm[q];
-        next_t = tnn;
-      }
-      // the next task's first accumulations (inline tiny segments) precede its first barrier:
-      // every wave must be done with this window's slots before any wave moves on
-      __syncthreads();
+      // no barrier after a written window: the next task's first LDS writes (the traversal
+      // scan, then accumulations) come after that scan's barrier, which every wave reaches only
+      // when done with this window.  A redo (pass 1) reuses the slots at once, so it waits.
       if (write) break;
+      __syncthreads();
     }
+    par ^= 1;
     have_pref = true;
   }
 }
