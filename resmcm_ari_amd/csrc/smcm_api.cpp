@@ -131,6 +131,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "window_presence") {
       if (v < 0 || v > 2) fail(SMCM_EINVAL, "window_presence must be 0, 1 or 2");
       o.window_presence = (int)v;
+    } else if (k == "sym_merge") {
+      if (v < 0 || v > 2) fail(SMCM_EINVAL, "sym_merge must be 0, 1 or 2");
+      o.sym_merge = (int)v;
     } else if (k == "sym_window_cols") {
       if (v < 64) fail(SMCM_EINVAL, "sym_window_cols must be >= 64");
       o.sym_window_cols = v;
@@ -208,6 +211,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "window_major") *v = o.window_major;
     else if (k == "window_min_products") *v = o.window_min_products;
     else if (k == "sym_window_cols") *v = o.sym_window_cols;
+    else if (k == "sym_merge") *v = o.sym_merge;
     else if (k == "window_presence") *v = o.window_presence;
     else if (k == "force_tier") *v = o.force_tier;
     else if (k == "exact_cost_sum") *v = ctx->exact_cost_sum;

@@ -297,9 +297,16 @@ struct TravLds {
 
 // One traversal round: every participating thread brings at most one segment (st, cnt) of B
 // with the A value a of its entry (cnt = 0: none).  Ends with a barrier.
-template <bool BV, int E = 1, int STEP = 8, int QH = 2, class F>
+// COOP: f(valid, column, value) is called by every lane of the wave together (valid = the lane
+// holds a product), so that it can combine lanes (the symbolic pass merges bits of one word);
+// otherwise f(column, value) is called by the lanes that hold a product.
+template <bool BV, int E = 1, int STEP = 8, int QH = 2, bool COOP = false, class F>
 __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, const int32_t* __restrict__ bcol,
                                            const uint64_t* __restrict__ bval, TravLds& L, F&& f) {
+  auto call = [&](bool valid, int32_t c, uint64_t v) {
+    if constexpr (COOP) f(valid, c, v);
+    else if (valid) f(c, v);
+  };
   constexpr int kPieceLen = E * kWave;
   const int nwaves = blockDim.x / kWave;
   const int wid = threadIdx.x / kWave, lane = lane_id();
@@ -327,8 +334,7 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
   int64_t total;
   const int64_t ex = block_exscan_i64(packed, L.red, &total);
 #pragma unroll
-  for (int x = 0; x < kTiny; ++x)
-    if (x < tn) f(tc[x], BV ? a * tv[x] : a);
+  for (int x = 0; x < kTiny; ++x) call(x < tn, tc[x], BV ? a * tv[x] : a);
   const int32_t nquarter = (int32_t)(total & 0xffff), nlong = (int32_t)((total >> 16) & 0xffff);
   const int32_t npieces = (int32_t)(total >> 32);
   if (packed & 0xffff) {
@@ -363,11 +369,11 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
       }
     }
 #pragma unroll
-    for (int h = 0; h < QH; ++h)
-      if (ok[h]) {
-        const uint64_t av2 = L.av[4 * (g + h * nwaves) + q];
-        f(c[h], BV ? av2 * v[h] : av2);
-      }
+    for (int h = 0; h < QH; ++h) {
+      uint64_t av2 = 0;
+      if (ok[h]) av2 = L.av[4 * (g + h * nwaves) + q];
+      call(ok[h], c[h], BV ? av2 * v[h] : av2);
+    }
   }
   // pieces of the long segments (E elements per lane), STEP pieces per wave step: E * STEP loads
   // in flight per lane
@@ -405,8 +411,7 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
 #pragma unroll
     for (int h = 0; h < STEP; ++h) {
 #pragma unroll
-      for (int u = 0; u < E; ++u)
-        if (lane + u * kWave < n[h]) f(c[h][u], BV ? av1[h] * v[h][u] : av1[h]);
+      for (int u = 0; u < E; ++u) call(lane + u * kWave < n[h], c[h][u], BV ? av1[h] * v[h][u] : av1[h]);
     }
   }
   __syncthreads();
@@ -1372,6 +1377,7 @@ __global__ void __launch_bounds__(kThreads) sym_prep_kernel(const int32_t* __res
 
 // A symbolic window spans 2^sub numeric windows (2^(log2w + sub) columns): fewer, larger tasks;
 // the count of every numeric window inside it is taken from its part of the bitmap.
+template <bool MERGE>
 __global__ void window_symbolic_kernel(const int32_t* __restrict__ acol, const int64_t* __restrict__ bptr,
                                        const int32_t* __restrict__ bcol, const SymRow* __restrict__ info,
                                        int32_t nrows, int log2w, int sub, int32_t nwin,
@@ -1422,10 +1428,28 @@ __global__ void window_symbolic_kernel(const int32_t* __restrict__ acol, const i
         cnt = b_hi[k] - s0;
         st = bptr[k] + s0;
       }
-      trav_round<false, 4, 2>(st, cnt, 1ull, bcol, nullptr, L, [&](int32_t c, uint64_t) {
-        const int32_t x = (int32_t)(c - w0);
-        atomicOr(&bm[x >> 5], 1u << (x & 31));
-      });
+      if constexpr (MERGE) {
+        // very dense rows: lanes whose columns share a 32-bit word are merged (segmented OR over
+        // the wave) and one lane per word sets the bits
+        trav_round<false, 4, 2, 2, true>(st, cnt, 1ull, bcol, nullptr, L, [&](bool valid, int32_t c, uint64_t) {
+          const int32_t x = (int32_t)(c - w0);
+          const int32_t word = valid ? (x >> 5) : -1;
+          uint32_t bit = valid ? (1u << (x & 31)) : 0u;
+#pragma unroll
+          for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t ob = __shfl_up(bit, o);
+            const int32_t ow = __shfl_up(word, o);
+            if (lane >= o && ow == word) bit |= ob;
+          }
+          const int32_t nw = __shfl_down(word, 1);
+          if (valid && (lane == kWave - 1 || nw != word)) atomicOr(&bm[word], bit);
+        });
+      } else {
+        trav_round<false, 4, 2>(st, cnt, 1ull, bcol, nullptr, L, [&](int32_t c, uint64_t) {
+          const int32_t x = (int32_t)(c - w0);
+          atomicOr(&bm[x >> 5], 1u << (x & 31));
+        });
+      }
     }
     // count per numeric window (a wave each) and clear; trav_round ended with a barrier
     int64_t total = 0;
@@ -1574,14 +1598,18 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     int sub = 0;
     while (((int64_t)split->wl.w << (sub + 1)) <= o.sym_window_cols && (1 << (sub + 1)) <= nwin) ++sub;
     const size_t lds = ((size_t)split->wl.w << sub) / 8;
-    const unsigned g = persistent_grid(o, occupancy((const void*)window_symbolic_kernel, o.d_threads, lds));
+    // rows whose products hit every column many times (dense chains) merge lanes per word
+    const double per_col = (double)tc.products[kHeavyTier] / std::max(nH, 1) / std::max(B.ncols, 1);
+    const bool merge = o.sym_merge == 1 || (o.sym_merge == 2 && per_col >= 8.0);
+    auto skern = merge ? window_symbolic_kernel<true> : window_symbolic_kernel<false>;
+    const unsigned g = persistent_grid(o, occupancy((const void*)skern, o.d_threads, lds));
     TierCounts heavy;
     heavy.rows[0] = nH;
     heavy.a_nnz[0] = tc.a_nnz[kHeavyTier];
     heavy.products[0] = tc.products[kHeavyTier];
     const int tok = T.start("spgemm.symbolic.window", st);
-    window_symbolic_kernel<<<g, o.d_threads, lds, st>>>(A.col, B.ptr, B.col, info.p, nH, split->wl.log2w, sub,
-                                                        nwin, split->bnd, B.nrows, queue, split->wcnt, row_nnz);
+    skern<<<g, o.d_threads, lds, st>>>(A.col, B.ptr, B.col, info.p, nH, split->wl.log2w, sub, nwin, split->bnd,
+                                       B.nrows, queue, split->wcnt, row_nnz);
     SMCM_LAUNCH_CHECK();
     T.stop(tok, st, tier_bytes(false, heavy, 0, AV, BV));
   }

@@ -20,6 +20,8 @@ WIDE_MODES = {
     "window_major_heavy": {"window_major": 1, "window_min_products": 0, "heavy_ub": 0},
     # ... with one numeric window per symbolic task
     "window_major_heavy_w1": {"window_major": 1, "window_min_products": 0, "heavy_ub": 0, "sym_window_cols": 64},
+    # ... merging lanes that share a bitmap word
+    "window_major_heavy_merge": {"window_major": 1, "window_min_products": 0, "heavy_ub": 0, "sym_merge": 1},
     # presence bits from the start (one pass per window task)
     "window_major_presence": {"window_major": 1, "window_min_products": 0, "window_presence": 1},
     "window_major_auto": {"window_major": 1, "window_min_products": 0, "window_presence": 2},
