@@ -8,8 +8,10 @@
 // Every r_ij[u] is computed with the reference's operation order in IEEE binary64 (the file is
 // built with -ffp-contract=off), so per-row densities are bit-identical to a one-thread run of
 // the reference.  The product for one row is a dependent chain and cannot be reassociated; rows
-// of degree <= kLightDeg use one lane each, longer rows one wave each (the wave gathers 64
-// factors in parallel into LDS and lane j multiplies chain j through them in order).
+// of degree <= kLightDeg use one lane each.  Longer rows take two steps: every factor (1 - r) of
+// their entries is gathered in parallel into a chain-major buffer (items of kItem entries), then
+// one wave per row streams its factors through LDS, several chunks in flight, and lane j multiplies
+// chain j through them in row order.
 // Once all running products are exactly 0 they stay 0, so a row stops early - this is exact.
 //
 // c_ij is reduced in a fixed tree order (deterministic run to run); exact_sum instead sums in
@@ -22,6 +24,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLightDeg = 32;
+constexpr int kItem = 2048;  // entries per factor item of a heavy row
 constexpr int kMaxW = 15;
 
 template <int W>
@@ -29,7 +32,8 @@ __global__ void __launch_bounds__(kThreads) est_light_kernel(const int64_t* __re
                                                              const int32_t* __restrict__ col, int32_t nrows,
                                                              double leaf_dim, const double* __restrict__ Rn,
                                                              double* __restrict__ Ri, int32_t* __restrict__ heavy,
-                                                             int32_t* __restrict__ nheavy) {
+                                                             int32_t* __restrict__ nheavy, int2* __restrict__ items,
+                                                             int32_t* __restrict__ nitems) {
   const int32_t u = blockIdx.x * kThreads + threadIdx.x;
   if (u >= nrows) return;
   const int64_t p0 = ptr[u], p1 = ptr[u + 1];
@@ -39,8 +43,10 @@ __global__ void __launch_bounds__(kThreads) est_light_kernel(const int64_t* __re
   out[0] = s > 1 ? 1 : s;
   if constexpr (W > 0) {
     if (deg > kLightDeg) {
-      int32_t slot = atomicAdd(nheavy, 1);
-      heavy[slot] = u;
+      heavy[atomicAdd(nheavy, 1)] = u;
+      const int32_t nit = (int32_t)((deg + kItem - 1) / kItem);
+      const int32_t base = atomicAdd(nitems, nit);
+      for (int32_t k = 0; k < nit; ++k) items[base + k] = make_int2(u, k);
       return;
     }
     double acc[W];
@@ -57,59 +63,105 @@ __global__ void __launch_bounds__(kThreads) est_light_kernel(const int64_t* __re
   }
 }
 
-// one wave per heavy row: lanes compute the factors (1 - r) of 64 entries at a time and stage
-// them in LDS; lane j then multiplies chain j through them in row order (the reference's order),
-// reading its factors from LDS (loads issued ahead of the dependent multiplies)
+// factors of heavy rows: Fh[j * nnz + e] = 1 - R_{i+1}[col[e]][j], one block per item (kItem
+// consecutive entries of one row), every gather independent
 template <int W>
-__global__ void __launch_bounds__(kThreads) est_heavy_kernel(const int64_t* __restrict__ ptr,
-                                                             const int32_t* __restrict__ col,
-                                                             const double* __restrict__ Rn, double* __restrict__ Ri,
+__global__ void __launch_bounds__(kThreads) est_factor_kernel(const int64_t* __restrict__ ptr,
+                                                              const int32_t* __restrict__ col, int64_t nnz,
+                                                              const double* __restrict__ Rn,
+                                                              const int2* __restrict__ items,
+                                                              const int32_t* __restrict__ nitems,
+                                                              double* __restrict__ Fh) {
+  constexpr int kPer = kItem / kThreads;
+  const int32_t nit = *nitems;
+  for (int32_t it = blockIdx.x; it < nit; it += gridDim.x) {
+    const int2 item = items[it];
+    const int64_t p1 = ptr[item.x + 1];
+    const int64_t b = ptr[item.x] + (int64_t)item.y * kItem + threadIdx.x;
+    int32_t c[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int64_t x = b + q * kThreads;
+      c[q] = x < p1 ? col[x] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      if (c[q] < 0) continue;
+      const int64_t x = b + q * kThreads;
+      const double* r = Rn + (size_t)c[q] * W;
+#pragma unroll
+      for (int j = 0; j < W; ++j) Fh[(size_t)j * nnz + x] = 1 - r[j];
+    }
+  }
+}
+
+// one wave per heavy row: the wave loads chunks of 64 entries' factors (chain-major, coalesced,
+// kDepth chunks in flight in registers) and stages one chunk at a time in LDS; lane j < W
+// multiplies chain j through the chunk in row order (the reference's order).  Entries past the
+// row's end load as 1.0, which leaves a product unchanged.
+template <int W>
+__global__ void __launch_bounds__(kThreads) est_chain_kernel(const int64_t* __restrict__ ptr, int64_t nnz,
+                                                             const double* __restrict__ Fh,
+                                                             double* __restrict__ Ri,
                                                              const int32_t* __restrict__ heavy,
                                                              const int32_t* __restrict__ nheavy) {
-  __shared__ double F[kThreads / kWave][kWave * (W > 0 ? W : 1)];
+  constexpr int kDepth = 24 / W < 2 ? 2 : (24 / W > 8 ? 8 : 24 / W);
+  __shared__ __align__(16) double F[kThreads / kWave][W * kWave];
   const int lane = lane_id();
   const int32_t nh = *nheavy;
   const int32_t wpb = kThreads / kWave;
   double* fw = F[threadIdx.x / kWave];
   for (int32_t h = blockIdx.x * wpb + threadIdx.x / kWave; h < nh; h += gridDim.x * wpb) {
-    const int32_t u = heavy[h];
-    const int64_t p0 = ptr[u], p1 = ptr[u + 1];
-    double acc = 1.0;  // lane j < W: the running product of chain j
-    double f[W];
-    {
-      const int64_t x = p0 + lane;
-      const double* r = Rn + (size_t)(x < p1 ? col[x] : 0) * W;
+    const int32_t u = uniform_i32(heavy[h]);
+    const int64_t p0 = uniform_i64(ptr[u]), p1 = uniform_i64(ptr[u + 1]);
+    const int64_t nch = (p1 - p0 + kWave - 1) / kWave;
+    double r[kDepth][W];
 #pragma unroll
-      for (int j = 0; j < W; ++j) f[j] = x < p1 ? (1 - r[j]) : 1.0;
+    for (int d = 0; d < kDepth; ++d) {
+      const int64_t x = p0 + (int64_t)d * kWave + lane;
+#pragma unroll
+      for (int j = 0; j < W; ++j) r[d][j] = x < p1 ? Fh[(size_t)j * nnz + x] : 1.0;
     }
-    for (int64_t b = p0; b < p1; b += kWave) {
+    double acc = 1.0;  // lane j < W: the running product of chain j
+    bool done = false;
+    for (int64_t c0 = 0; c0 < nch && !done; c0 += kDepth) {
 #pragma unroll
-      for (int j = 0; j < W; ++j) fw[lane * W + j] = f[j];
-      // prefetch the next chunk's factors before the dependent multiply chain
-      {
-        const int64_t x = b + kWave + lane;
-        const double* r = Rn + (size_t)(x < p1 ? col[x] : 0) * W;
+      for (int d = 0; d < kDepth; ++d) {
+        if (c0 + d >= nch) {
+          done = true;
+          break;
+        }
 #pragma unroll
-        for (int j = 0; j < W; ++j) f[j] = x < p1 ? (1 - r[j]) : 1.0;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int cnt = (int)((p1 - b) < kWave ? (p1 - b) : kWave);
-      if (lane < W) {
-        if (cnt == kWave) {
-#pragma unroll 16
-          for (int k = 0; k < kWave; ++k) acc *= fw[k * W + lane];
-        } else {
-          for (int k = 0; k < cnt; ++k) acc *= fw[k * W + lane];
+        for (int j = 0; j < W; ++j) fw[j * kWave + lane] = r[d][j];
+        {
+          const int64_t x = p0 + (c0 + d + kDepth) * kWave + lane;
+#pragma unroll
+          for (int j = 0; j < W; ++j) r[d][j] = x < p1 ? Fh[(size_t)j * nnz + x] : 1.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < W) {
+          const double2* q = (const double2*)(fw + lane * kWave);
+          double2 v[kWave / 2];
+#pragma unroll
+          for (int k = 0; k < kWave / 2; ++k) v[k] = q[k];
+#pragma unroll
+          for (int k = 0; k < kWave / 2; ++k) {
+            acc *= v[k].x;
+            acc *= v[k].y;
+          }
+        }
+        // every chain exactly 0: it stays 0 (exact early exit)
+        const bool live = lane < W && acc != 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (__ballot(live) == 0) {
+          done = true;
+          break;
         }
       }
-      // every chain exactly 0: it stays 0 (exact early exit)
-      const bool live = lane < W && acc != 0.0;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (__ballot(live) == 0) break;
     }
     if (lane < W) Ri[(size_t)u * (W + 1) + 1 + lane] = 1 - acc;
   }
@@ -258,20 +310,31 @@ void run_pass(const EstimatorPass& p, DevAlloc& al) {
   const Csr& M = *p.M;
   const int32_t n = M.nrows;
   hipStream_t st = al.stream;
-  DevBuf<int32_t> heavy(al, n > 0 ? n : 1), nheavy(al, 1);
-  fill_i32(nheavy.p, 1, 0, st);
+  // heavy rows have more than kLightDeg entries: fewer than nnz / (kLightDeg + 1) rows, and
+  // fewer than nnz / kItem + (heavy rows) items
+  const int64_t cap_items = M.nnz / kItem + M.nnz / (kLightDeg + 1) + 1;
+  DevBuf<int32_t> heavy(al, n > 0 ? n : 1), counts(al, 2);
+  DevBuf<int2> items(al, W > 0 ? cap_items : 1);
+  int32_t* nheavy = counts.p;
+  int32_t* nitems = counts.p + 1;
+  fill_i32(counts.p, 2, 0, st);
   const unsigned blocks = (unsigned)ceil_div(n > 0 ? n : 1, kThreads);
   // algorithmic bytes of the pass: row pointers, column ids, W gathered densities per entry,
   // W+1 densities written per row
   const double bytes = 16.0 * (n + 1) + 4.0 * (double)M.nnz + 8.0 * W * (double)M.nnz + 8.0 * (W + 1) * n;
   const int tok = p.tm ? p.tm->start("estimator.pass", st) : -1;
   if (n > 0) {
-    est_light_kernel<W><<<blocks, kThreads, 0, st>>>(M.ptr, M.col, n, p.leaf_dim, p.Rn, p.Ri, heavy.p, nheavy.p);
+    est_light_kernel<W><<<blocks, kThreads, 0, st>>>(M.ptr, M.col, n, p.leaf_dim, p.Rn, p.Ri, heavy.p, nheavy,
+                                                     items.p, nitems);
     SMCM_LAUNCH_CHECK();
   }
   if constexpr (W > 0) {
-    if (n > 0) {
-      est_heavy_kernel<W><<<2048, kThreads, 0, st>>>(M.ptr, M.col, p.Rn, p.Ri, heavy.p, nheavy.p);
+    if (n > 0 && M.nnz > kLightDeg) {
+      DevBuf<double> Fh(al, (int64_t)W * M.nnz);
+      const unsigned fblocks = (unsigned)(cap_items < 4096 ? cap_items : 4096);
+      est_factor_kernel<W><<<fblocks, kThreads, 0, st>>>(M.ptr, M.col, M.nnz, p.Rn, items.p, nitems, Fh.p);
+      SMCM_LAUNCH_CHECK();
+      est_chain_kernel<W><<<2048, kThreads, 0, st>>>(M.ptr, M.nnz, Fh.p, p.Ri, heavy.p, nheavy);
       SMCM_LAUNCH_CHECK();
     }
     if (p.tm) p.tm->stop(tok, st, bytes);
