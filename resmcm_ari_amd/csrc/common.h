@@ -39,6 +39,18 @@ struct HipError : std::runtime_error {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Pinned host buffer of the calling thread for small device-to-host readbacks (counters, sizes).
+// A copy into pageable memory blocks the host until the stream has drained; a copy into pinned
+// memory is queued like a kernel, so several readbacks share one stream synchronisation.  Each
+// thread runs one library call at a time and reads its values right after that call's
+// synchronisation.  Allocated on first use and kept for the life of the thread.
+constexpr size_t kHostStageBytes = 1024;
+inline unsigned char* host_stage() {
+  thread_local unsigned char* p = nullptr;
+  if (!p) SMCM_HIP(hipHostMalloc((void**)&p, kHostStageBytes, hipHostMallocDefault));
+  return p;
+}
+
 #ifdef __HIP_DEVICE_COMPILE__
 #define SMCM_DEV __device__ __forceinline__
 #else

@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <vector>
 
@@ -45,6 +46,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kShortRow = 16;  // B rows up to this length are walked by one thread
+// byte offsets of the readbacks in the thread's pinned host stage
+constexpr size_t kStageBin = 0, kStageTierNnz = 192, kStageNnz = 256, kStageSplit = 320;
 
 // ------------------------------------------------------------------------------------------
 // analysis
@@ -1641,9 +1644,14 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     SMCM_LAUNCH_CHECK();
     int32_t cnt[3];
     unsigned long long sums[6];
-    SMCM_HIP(hipMemcpyAsync(cnt, split->counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
-    SMCM_HIP(hipMemcpyAsync(sums, split->sums, sizeof(sums), hipMemcpyDeviceToHost, st));
-    SMCM_HIP(hipStreamSynchronize(st));
+    {
+      unsigned char* hs = host_stage() + kStageSplit;
+      SMCM_HIP(hipMemcpyAsync(hs, split->counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
+      SMCM_HIP(hipMemcpyAsync(hs + 64, split->sums, sizeof(sums), hipMemcpyDeviceToHost, st));
+      SMCM_HIP(hipStreamSynchronize(st));
+      memcpy(cnt, hs, sizeof(cnt));
+      memcpy(sums, hs + 64, sizeof(sums));
+    }
     TierCounts cls;
     for (int c = 0; c < 2; ++c) {
       cls.rows[c] = cnt[c];
@@ -1861,7 +1869,6 @@ struct SpgemmCore {
       wcnt = DevBuf<int32_t>(al, cells);
       split.wcnt = wcnt.p;
     }
-    SMCM_HIP(hipMemcpyAsync(&products, S.p + A.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   }
 
   // bin rows [r0, r1) (bins hold absolute row ids); results in tc (synchronises)
@@ -1875,18 +1882,28 @@ struct SpgemmCore {
           o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_max_ub : 0, bins.p, counts.p, sums.p);
       SMCM_LAUNCH_CHECK();
     }
-    unsigned long long sh[2 * kNumBins];
-    SMCM_HIP(hipMemcpyAsync(tc.rows, counts.p, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, st));
-    SMCM_HIP(hipMemcpyAsync(sh, sums.p, sizeof(sh), hipMemcpyDeviceToHost, st));
+    unsigned char* hs = host_stage() + kStageBin;
+    SMCM_HIP(hipMemcpyAsync(hs, counts.p, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipMemcpyAsync(hs + 64, sums.p, sizeof(unsigned long long) * 2 * kNumBins, hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipStreamSynchronize(st));
+    unsigned long long sh[2 * kNumBins];
+    memcpy(tc.rows, hs, sizeof(int32_t) * kNumBins);
+    memcpy(sh, hs + 64, sizeof(sh));
     for (int t = 0; t < kNumBins; ++t) {
       tc.a_nnz[t] = (int64_t)sh[2 * t];
       tc.products[t] = (int64_t)sh[2 * t + 1];
       tc.c_nnz[t] = 0;
     }
+    // every row of A binned: the products of all tiers are the product's total (S[nnz(A)])
+    if (r0 == 0 && r1 == A.nrows) {
+      products = 0;
+      for (int t = 0; t < kNumBins; ++t) products += tc.products[t];
+    }
   }
 
-  // widths of the binned rows into row_nnz (other entries untouched); fills tc.c_nnz
+  // widths of the binned rows into row_nnz (other entries untouched).  The per-tier widths are
+  // copied to the host stage without a synchronisation: read_symbolic() fills tc.c_nnz after
+  // the caller's next stream synchronisation.
   void symbolic(int64_t* row_nnz) {
     hipStream_t st = al.stream;
     if (split.wcnt) fill_i32(split.wcnt, (int64_t)A.nrows * split.wl.nwin, 0, st);
@@ -1895,9 +1912,12 @@ struct SpgemmCore {
     fill_i64((int64_t*)tn, kNumBins, 0, st);
     tier_nnz_kernel<<<dim3(64, kNumBins - 1 - kEmpty), kThreads, 0, st>>>(bins.p, A.nrows, counts.p, row_nnz, tn);
     SMCM_LAUNCH_CHECK();
+    SMCM_HIP(hipMemcpyAsync(host_stage() + kStageTierNnz, tn, sizeof(unsigned long long) * kNumBins,
+                            hipMemcpyDeviceToHost, st));
+  }
+  void read_symbolic() {
     unsigned long long h[kNumBins];
-    SMCM_HIP(hipMemcpyAsync(h, tn, sizeof(h), hipMemcpyDeviceToHost, st));
-    SMCM_HIP(hipStreamSynchronize(st));
+    memcpy(h, host_stage() + kStageTierNnz, sizeof(h));
     for (int t = 0; t < kNumBins; ++t) tc.c_nnz[t] = (int64_t)h[t];
   }
 
@@ -1957,8 +1977,10 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
     scan_exclusive_i64(row_nnz.p, C.ptr, m, al);
   }
   int64_t nnz = 0;
-  SMCM_HIP(hipMemcpyAsync(&nnz, C.ptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SMCM_HIP(hipMemcpyAsync(host_stage() + kStageNnz, C.ptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SMCM_HIP(hipStreamSynchronize(st));
+  memcpy(&nnz, host_stage() + kStageNnz, sizeof(nnz));
+  core.read_symbolic();
   if (stats) stats->ms_symbolic = ms_since(t1);
   auto t2 = std::chrono::steady_clock::now();
   C.nnz = nnz;
@@ -2004,6 +2026,7 @@ void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_str
   std::vector<int64_t> hptr((size_t)m + 1);
   SMCM_HIP(hipMemcpyAsync(hptr.data(), ptr.p, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost, st));
   SMCM_HIP(hipStreamSynchronize(st));
+  core.read_symbolic();
   if (tm) tm->flush();
   const int64_t max_entries = std::max<int64_t>(1, block_bytes / 12);
   // cut rows into blocks: widest [b, e) whose output fits; a single row always forms a block
