@@ -105,6 +105,7 @@ struct smcm_graph {
   int32_t* adj_et = nullptr;
   int32_t* local = nullptr;
   std::vector<int32_t> h_vtype, h_local, dims;
+  std::vector<std::vector<int32_t>> h_bins;  // host: ascending global ids of every type
   std::vector<int32_t*> bins;
   std::map<std::tuple<int32_t, int32_t, int32_t>, Csr> cache;
   std::recursive_mutex cache_mu;  // typed-matrix cache, shared by the threads of a server
@@ -268,9 +269,7 @@ struct Chain {
     if (subset) {
       // rows of M_0 are the sources: their global ids
       std::vector<int32_t> h(src_local.size());
-      std::vector<int32_t> ids;
-      for (int32_t v = 0; v < g->n; ++v)
-        if (g->h_vtype[v] == vts[0]) ids.push_back(v);
+      const std::vector<int32_t>& ids = g->h_bins[vts[0]];
       for (size_t r = 0; r < h.size(); ++r) h[r] = ids[src_local[r]];
       src_g = DevBuf<int32_t>(al, (int64_t)h.size());
       if (!h.empty())
@@ -289,12 +288,7 @@ struct Chain {
       pc.bhi.push_back(hi[t] < 0 ? -1 : hi[t] / 256);
     }
     if (provider == smcm::kProviderLGraph) {
-      for (int32_t i = 0; i < L; ++i) {
-        std::vector<int32_t> ids;
-        for (int32_t v = 0; v < g->n; ++v)
-          if (g->h_vtype[v] == vts[i + 1]) ids.push_back(v);
-        pc.col_type_ids.push_back(std::move(ids));
-      }
+      for (int32_t i = 0; i < L; ++i) pc.col_type_ids.push_back(g->h_bins[vts[i + 1]]);
     }
     smcm::run_provider(provider, accuracy, pc, out, al);
   }
@@ -512,7 +506,7 @@ struct Chain {
       if (root && consumer)
         smcm::spgemm_blocked(a.m, b.m, est, L - i, block_bytes, *consumer, ctx->opts, &ss, al, tm);
       else
-        smcm::spgemm(a.m, b.m, est, L - i, C, ctx->opts, &ss, al, true, tm);
+        smcm::spgemm(a.m, b.m, est, L - i, C, ctx->opts, &ss, al, false, tm);
       if (on_product && !(root && consumer)) on_product(i, j, C);
       if (stats) {
         stats->products += ss.products;
@@ -549,6 +543,7 @@ struct Chain {
         if (o.owned) free_csr(al, o.m);
       throw;
     }
+    SMCM_HIP(hipStreamSynchronize(al.stream));
     spgemm_time = now_s() - t0;
     return st.at(0).m;
   }
@@ -583,17 +578,12 @@ inline smcm_result* make_result(Chain& ch, Csr C) {
     SMCM_HIP(hipMemcpyAsync(r->col_map, ch.g->bins[tcol], sizeof(int32_t) * dcol, hipMemcpyDeviceToDevice,
                             ctx->al.stream));
   const int32_t t0 = ch.vts[0];
+  const std::vector<int32_t>& b = ch.g->h_bins[t0];  // global id of local row x of type t0
   if (ch.subset) {
     r->row_ids.resize(ch.src_local.size());
-    std::vector<int32_t> b(ch.g->dims[t0]);
-    // global id of local row x of type t0
-    for (int32_t v = 0; v < ch.g->n; ++v)
-      if (ch.g->h_vtype[v] == t0) b[ch.g->h_local[v]] = v;
     for (size_t q = 0; q < ch.src_local.size(); ++q) r->row_ids[q] = b[ch.src_local[q]];
   } else {
-    r->row_ids.reserve(ch.g->dims[t0]);
-    for (int32_t v = 0; v < ch.g->n; ++v)
-      if (ch.g->h_vtype[v] == t0) r->row_ids.push_back(v);
+    r->row_ids = b;
   }
   SMCM_HIP(hipStreamSynchronize(ctx->al.stream));
   return r;
@@ -617,12 +607,13 @@ inline void run_core(Chain& ch, const int32_t* forced_pairs, Csr* C, const smcm:
       if (consumer)
         smcm::spgemm_blocked(ch.mats[0], I, nullptr, 0, block_bytes, *consumer, ctx->opts, &ss, ctx->al, tm);
       else
-        smcm::spgemm(ch.mats[0], I, nullptr, 0, out, ctx->opts, &ss, ctx->al, true, tm);
+        smcm::spgemm(ch.mats[0], I, nullptr, 0, out, ctx->opts, &ss, ctx->al, false, tm);
     } catch (...) {
       free_csr(ctx->al, I);
       throw;
     }
     free_csr(ctx->al, I);
+    SMCM_HIP(hipStreamSynchronize(ctx->al.stream));
     ch.spgemm_time = now_s() - t0;
     st.products = ss.products;
     st.nnz = ss.nnz;

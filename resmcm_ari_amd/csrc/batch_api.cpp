@@ -75,16 +75,9 @@ T* upload(DevAlloc& al, const std::vector<T>& h) {
 
 // global ids of the rows of a result (as make_result)
 std::vector<int32_t> result_rows(smcm_graph* g, int32_t t0, const std::vector<int32_t>& src_local, bool whole) {
+  const std::vector<int32_t>& b = g->h_bins[t0];
+  if (whole) return b;
   std::vector<int32_t> rows;
-  if (whole) {
-    rows.reserve(g->dims[t0]);
-    for (int32_t v = 0; v < g->n; ++v)
-      if (g->h_vtype[v] == t0) rows.push_back(v);
-    return rows;
-  }
-  std::vector<int32_t> b(g->dims[t0]);
-  for (int32_t v = 0; v < g->n; ++v)
-    if (g->h_vtype[v] == t0) b[g->h_local[v]] = v;
   rows.resize(src_local.size());
   for (size_t q = 0; q < src_local.size(); ++q) rows[q] = b[src_local[q]];
   return rows;

@@ -162,11 +162,14 @@ struct SpgemmStats {
   int64_t nnz = 0;
   int64_t rows[4] = {0, 0, 0, 0};  // empty / wave / sort / bitmap
   int64_t windows = 1;
+  // host milliseconds per phase; ms_numeric covers the numeric kernels only when spgemm()
+  // synchronised (timing, a kernel timer or SMCM_TRACE), otherwise just their launches
   double ms_analysis = 0, ms_symbolic = 0, ms_numeric = 0;
 };
 
 // C = A * B (uint64 counts, columns sorted).  est_r (nullable, device) gives the estimator's
 // row densities r[u * est_stride]; the predicted width is r * ncols(B).  B rows must be sorted.
+// C is complete in stream order on al.stream; the host waits for it only when timing is set.
 void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr& C,
             const SpgemmOpts& o, SpgemmStats* st, DevAlloc& al, bool timing = false,
             KernelTimer* tm = nullptr);

@@ -354,9 +354,7 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
     // other (row u of the result depends only on source row u), each planned on its own.
     const int64_t K = std::min<int64_t>(nrows, (int64_t)std::ceil(pred / (double)quarter));
     std::vector<int32_t> rows = q.src;
-    if (q.whole)
-      for (int32_t v = 0; v < g->n; ++v)
-        if (g->h_vtype[v] == q.vts[0]) rows.push_back(v);
+    if (q.whole) rows.insert(rows.end(), g->h_bins[q.vts[0]].begin(), g->h_bins[q.vts[0]].end());
     adm.admit(quarter, q);
     int64_t total = 0;
     for (int64_t b = 0; b < K; ++b) {

@@ -268,6 +268,9 @@ int smcm_graph_upload(smcm_ctx* ctx, int32_t n, const int32_t* vtype, const int6
     g->dims.assign(g->ntypes, 0);
     g->h_local.resize(n);
     for (int32_t v = 0; v < n; ++v) g->h_local[v] = g->dims[vtype[v]]++;
+    g->h_bins.resize(g->ntypes);
+    for (int32_t t = 0; t < g->ntypes; ++t) g->h_bins[t].reserve(g->dims[t]);
+    for (int32_t v = 0; v < n; ++v) g->h_bins[vtype[v]].push_back(v);
     DevAlloc& al = ctx->al;
     g->vtype = al.alloc_n<int32_t>(n);
     g->adj_ptr = al.alloc_n<int64_t>((int64_t)n + 1);

@@ -48,6 +48,8 @@ constexpr int kThreads = 256;
 constexpr int kShortRow = 16;  // B rows up to this length are walked by one thread
 // byte offsets of the readbacks in the thread's pinned host stage
 constexpr size_t kStageBin = 0, kStageTierNnz = 192, kStageNnz = 256, kStageSplit = 320;
+// byte offsets in a product's device counter block
+constexpr size_t kMetaCounts = 0, kMetaQueue = 32, kMetaSums = 64, kMetaSplit = 192, kMetaBytes = 320;
 
 // ------------------------------------------------------------------------------------------
 // analysis
@@ -1645,9 +1647,9 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     int32_t cnt[3];
     unsigned long long sums[6];
     {
+      // split->sums sits 64 bytes after split->counts (SpgemmCore's counter block): one copy
       unsigned char* hs = host_stage() + kStageSplit;
-      SMCM_HIP(hipMemcpyAsync(hs, split->counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
-      SMCM_HIP(hipMemcpyAsync(hs + 64, split->sums, sizeof(sums), hipMemcpyDeviceToHost, st));
+      SMCM_HIP(hipMemcpyAsync(hs, split->counts, 64 + sizeof(sums), hipMemcpyDeviceToHost, st));
       SMCM_HIP(hipStreamSynchronize(st));
       memcpy(cnt, hs, sizeof(cnt));
       memcpy(sums, hs + 64, sizeof(sums));
@@ -1823,10 +1825,14 @@ struct SpgemmCore {
   DevAlloc& al;
   KernelTimer* tm;
   DevBuf<int64_t> S;
-  DevBuf<int32_t> bins, counts, queue;
-  DevBuf<unsigned long long> sums;
-  DevBuf<int32_t> split_rows, split_counts, bnd, wcnt;
-  DevBuf<unsigned long long> split_sums;
+  DevBuf<int32_t> bins;
+  // counters of the pass in one block (laid out like the host stage's kStageBin / kStageSplit
+  // sections, so each group is read back with one copy): tier row counts, queue, tier sums,
+  // split counts, split sums
+  DevBuf<unsigned char> meta;
+  int32_t *counts = nullptr, *queue = nullptr;
+  unsigned long long* sums = nullptr;
+  DevBuf<int32_t> split_rows, bnd, wcnt;
   SplitScratch split;
   TierCounts tc;
   BitmapLayout lay;
@@ -1847,19 +1853,19 @@ struct SpgemmCore {
     }
     scan_exclusive_i64(len.p, S.p, A.nnz, al);
     bins = DevBuf<int32_t>(al, (int64_t)kNumBins * (A.nrows > 0 ? A.nrows : 1));
-    counts = DevBuf<int32_t>(al, kNumBins + 1);
-    queue = DevBuf<int32_t>(al, 1);
-    sums = DevBuf<unsigned long long>(al, 3 * kNumBins);
+    meta = DevBuf<unsigned char>(al, kMetaBytes);
+    counts = (int32_t*)(meta.p + kMetaCounts);
+    queue = (int32_t*)(meta.p + kMetaQueue);
+    sums = (unsigned long long*)(meta.p + kMetaSums);
     lay = choose_layout(B.ncols, o);
     split.wl = choose_window(B.ncols, o);
     set_all_lds_attrs(std::max({lay.bytes_num(), lay.bytes_sym(), split.wl.bytes()}));
     split_rows = DevBuf<int32_t>(al, 2 * (int64_t)(A.nrows > 0 ? A.nrows : 1));
-    split_counts = DevBuf<int32_t>(al, 3);
-    split_sums = DevBuf<unsigned long long>(al, 6);
+
     split.narrow = split_rows.p;
     split.wide = split_rows.p + (A.nrows > 0 ? A.nrows : 1);
-    split.counts = split_counts.p;
-    split.sums = split_sums.p;
+    split.counts = (int32_t*)(meta.p + kMetaSplit);
+    split.sums = (unsigned long long*)(meta.p + kMetaSplit + 64);
     split.owner = this;
     split.al = &al;
     // per-(row, window) column counts for the window-major window tier; the task index
@@ -1874,17 +1880,18 @@ struct SpgemmCore {
   // bin rows [r0, r1) (bins hold absolute row ids); results in tc (synchronises)
   void bin(int32_t r0, int32_t r1) {
     hipStream_t st = al.stream;
-    fill_i32(counts.p, kNumBins + 1, 0, st);
-    fill_i64((int64_t*)sums.p, 3 * kNumBins, 0, st);
+    fill_i32(counts, kNumBins + 1, 0, st);
+    fill_i64((int64_t*)sums, 3 * kNumBins, 0, st);
     if (r1 > r0) {
       bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, kThreads), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
-          o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_max_ub : 0, bins.p, counts.p, sums.p);
+          o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_max_ub : 0, bins.p, counts, sums);
       SMCM_LAUNCH_CHECK();
     }
     unsigned char* hs = host_stage() + kStageBin;
-    SMCM_HIP(hipMemcpyAsync(hs, counts.p, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, st));
-    SMCM_HIP(hipMemcpyAsync(hs + 64, sums.p, sizeof(unsigned long long) * 2 * kNumBins, hipMemcpyDeviceToHost, st));
+    // tier row counts at +0, tier sums at +64 in both blocks: one copy
+    SMCM_HIP(hipMemcpyAsync(hs, meta.p + kMetaCounts, kMetaSums + sizeof(unsigned long long) * 2 * kNumBins,
+                            hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipStreamSynchronize(st));
     unsigned long long sh[2 * kNumBins];
     memcpy(tc.rows, hs, sizeof(int32_t) * kNumBins);
@@ -1907,10 +1914,10 @@ struct SpgemmCore {
   void symbolic(int64_t* row_nnz) {
     hipStream_t st = al.stream;
     if (split.wcnt) fill_i32(split.wcnt, (int64_t)A.nrows * split.wl.nwin, 0, st);
-    dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, &split, queue.p, st, tm);
-    unsigned long long* tn = sums.p + 2 * kNumBins;
+    dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, &split, queue, st, tm);
+    unsigned long long* tn = sums + 2 * kNumBins;
     fill_i64((int64_t*)tn, kNumBins, 0, st);
-    tier_nnz_kernel<<<dim3(64, kNumBins - 1 - kEmpty), kThreads, 0, st>>>(bins.p, A.nrows, counts.p, row_nnz, tn);
+    tier_nnz_kernel<<<dim3(64, kNumBins - 1 - kEmpty), kThreads, 0, st>>>(bins.p, A.nrows, counts, row_nnz, tn);
     SMCM_LAUNCH_CHECK();
     SMCM_HIP(hipMemcpyAsync(host_stage() + kStageTierNnz, tn, sizeof(unsigned long long) * kNumBins,
                             hipMemcpyDeviceToHost, st));
@@ -1924,7 +1931,7 @@ struct SpgemmCore {
   // numeric pass of the binned rows; cptr holds absolute offsets, ccol/cval are shifted so
   // that ccol[cptr[u]] is the first output of row u
   void numeric(const int64_t* cptr, int32_t* ccol, uint64_t* cval) {
-    dispatch<true>(A, B, S.p, bins.p, tc, nullptr, cptr, ccol, cval, o, lay, &split, queue.p, al.stream, tm);
+    dispatch<true>(A, B, S.p, bins.p, tc, nullptr, cptr, ccol, cval, o, lay, &split, queue, al.stream, tm);
   }
 
   void add_stats(SpgemmStats* st) const {
@@ -1987,7 +1994,8 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
   C.col = al.alloc_n<int32_t>(nnz);
   C.val = al.alloc_n<uint64_t>(nnz);
   core.numeric(C.ptr, C.col, C.val);
-  if (timing || stats || tm) SMCM_HIP(hipStreamSynchronize(st));
+  // callers order later work on the same stream; a synchronisation here only serves timing
+  if (timing || tm || trace) SMCM_HIP(hipStreamSynchronize(st));
   if (tm) tm->flush();
   if (stats) {
     stats->ms_numeric = ms_since(t2);
