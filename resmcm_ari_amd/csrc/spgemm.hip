@@ -1538,6 +1538,7 @@ struct SplitScratch {
   int32_t* wcnt = nullptr;            // columns per (row, window) from the symbolic pass, or null
                                       // (then the window tier runs one row at a time)
   WindowLayout wl;
+  bool staged = false;  // split counts already copied to the host stage and synchronised
   struct SpgemmCore* owner = nullptr;
   DevAlloc* al = nullptr;             // the owner's allocator (per-launch task metadata)
   void build_bounds(const Csr& B, hipStream_t st);
@@ -1551,6 +1552,9 @@ double tier_bytes(bool numeric, const TierCounts& tc, int t, bool av, bool bv) {
   if (!numeric) return 16.0 * (r + 1) + 20.0 * d + 4.0 * p;
   return 16.0 * (r + 1) + (20.0 + (av ? 8 : 0)) * d + (4.0 + (bv ? 8 : 0)) * p + 12.0 * w;
 }
+
+void launch_split(const int32_t* bins, int32_t m, int32_t nB, int32_t nH, const Csr& A, const int64_t* S,
+                  const int64_t* cptr, const BitmapLayout& lay, SplitScratch* split, hipStream_t st);
 
 template <bool NUMERIC, bool AV, bool BV>
 void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const TierCounts& tc,
@@ -1638,19 +1642,16 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
   }
   if (nB + nH > 0 && NUMERIC) {
     // split by exact width: narrow rows keep the LDS bitmap + rank kernel, wide rows use windows
-    fill_i32(split->counts, 3, 0, st);
-    fill_i64((int64_t*)split->sums, 6, 0, st);
-    split_wide_kernel<<<(unsigned)ceil_div(nB + nH, kThreads), kThreads, 0, st>>>(
-        bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m, nB, A.ptr, S, cptr, lay.vcap,
-        split->narrow, split->wide, split->counts, split->sums);
-    SMCM_LAUNCH_CHECK();
+    // (launched and read back with the output size when the whole product is one block)
+    if (!split->staged) {
+      launch_split(bins, m, nB, nH, A, S, cptr, lay, split, st);
+      SMCM_HIP(hipStreamSynchronize(st));
+    }
+    split->staged = false;
     int32_t cnt[3];
     unsigned long long sums[6];
     {
-      // split->sums sits 64 bytes after split->counts (SpgemmCore's counter block): one copy
-      unsigned char* hs = host_stage() + kStageSplit;
-      SMCM_HIP(hipMemcpyAsync(hs, split->counts, 64 + sizeof(sums), hipMemcpyDeviceToHost, st));
-      SMCM_HIP(hipStreamSynchronize(st));
+      const unsigned char* hs = host_stage() + kStageSplit;
       memcpy(cnt, hs, sizeof(cnt));
       memcpy(sums, hs + 64, sizeof(sums));
     }
@@ -1728,6 +1729,21 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
       T.stop(tok, st, tier_bytes(true, cls, 1, AV, BV));
     }
   }
+}
+
+// classify the bitmap-tier rows by exact width and copy the split counts to the host stage
+// (no synchronisation)
+void launch_split(const int32_t* bins, int32_t m, int32_t nB, int32_t nH, const Csr& A, const int64_t* S,
+                  const int64_t* cptr, const BitmapLayout& lay, SplitScratch* split, hipStream_t st) {
+  fill_i32(split->counts, 3, 0, st);
+  fill_i64((int64_t*)split->sums, 6, 0, st);
+  split_wide_kernel<<<(unsigned)ceil_div(nB + nH, kThreads), kThreads, 0, st>>>(
+      bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m, nB, A.ptr, S, cptr, lay.vcap, split->narrow,
+      split->wide, split->counts, split->sums);
+  SMCM_LAUNCH_CHECK();
+  // split->sums sits 64 bytes after split->counts (SpgemmCore's counter block): one copy
+  SMCM_HIP(hipMemcpyAsync(host_stage() + kStageSplit, split->counts, 64 + 6 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
 }
 
 template <bool NUMERIC>
@@ -1922,6 +1938,14 @@ struct SpgemmCore {
     SMCM_HIP(hipMemcpyAsync(host_stage() + kStageTierNnz, tn, sizeof(unsigned long long) * kNumBins,
                             hipMemcpyDeviceToHost, st));
   }
+  // whole-product numeric pass ahead: classify its bitmap-tier rows now (cptr = scanned widths),
+  // so the split counts come back with the caller's next synchronisation
+  void stage_split(const int64_t* cptr) {
+    const int32_t nB = tc.rows[kBitmapTier], nH = tc.rows[kHeavyTier];
+    if (nB + nH == 0) return;
+    launch_split(bins.p, A.nrows, nB, nH, A, S.p, cptr, lay, &split, al.stream);
+    split.staged = true;
+  }
   void read_symbolic() {
     unsigned long long h[kNumBins];
     memcpy(h, host_stage() + kStageTierNnz, sizeof(h));
@@ -1983,6 +2007,7 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
     core.symbolic(row_nnz.p);
     scan_exclusive_i64(row_nnz.p, C.ptr, m, al);
   }
+  core.stage_split(C.ptr);
   int64_t nnz = 0;
   SMCM_HIP(hipMemcpyAsync(host_stage() + kStageNnz, C.ptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SMCM_HIP(hipStreamSynchronize(st));
