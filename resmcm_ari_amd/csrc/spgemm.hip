@@ -1777,7 +1777,9 @@ void set_lds_attr(size_t bytes) {
 }
 
 void set_all_lds_attrs(size_t bytes) {
+  static std::mutex mu;  // server threads run products concurrently
   static size_t done = 0;
+  std::lock_guard<std::mutex> lk(mu);
   if (bytes <= done) return;
   set_lds_attr<false, false, false>(bytes);
   set_lds_attr<false, true, false>(bytes);
