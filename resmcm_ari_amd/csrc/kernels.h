@@ -155,6 +155,8 @@ struct SpgemmOpts {
                                        // window kernel when they average at least this many
                                        // products per (row, window); otherwise the bitmap kernel
                                        // accumulates their values in HBM (device atomics)
+  int64_t window_all_nwin = 4;  // products spanning 2..this many windows send every bitmap-tier row
+                                // to the window kernel (LDS value slots are bypassed)
 };
 
 struct SpgemmStats {

@@ -1554,7 +1554,15 @@ double tier_bytes(bool numeric, const TierCounts& tc, int t, bool av, bool bv) {
 }
 
 void launch_split(const int32_t* bins, int32_t m, int32_t nB, int32_t nH, const Csr& A, const int64_t* S,
-                  const int64_t* cptr, const BitmapLayout& lay, SplitScratch* split, hipStream_t st);
+                  const int64_t* cptr, const BitmapLayout& lay, const SpgemmOpts& o, SplitScratch* split,
+                  hipStream_t st);
+
+// widest row that keeps the bitmap kernel's LDS value slots: none when the product spans only
+// a few windows (the window kernel's dense accumulator then beats rank lookups per product)
+inline int64_t split_vcap(const BitmapLayout& lay, const SplitScratch* split, const SpgemmOpts& o) {
+  const int64_t nwin = split->wl.nwin;
+  return (nwin >= 2 && nwin <= o.window_all_nwin) ? 0 : lay.vcap;
+}
 
 template <bool NUMERIC, bool AV, bool BV>
 void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins, const TierCounts& tc,
@@ -1644,7 +1652,7 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     // split by exact width: narrow rows keep the LDS bitmap + rank kernel, wide rows use windows
     // (launched and read back with the output size when the whole product is one block)
     if (!split->staged) {
-      launch_split(bins, m, nB, nH, A, S, cptr, lay, split, st);
+      launch_split(bins, m, nB, nH, A, S, cptr, lay, o, split, st);
       SMCM_HIP(hipStreamSynchronize(st));
     }
     split->staged = false;
@@ -1734,12 +1742,13 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
 // classify the bitmap-tier rows by exact width and copy the split counts to the host stage
 // (no synchronisation)
 void launch_split(const int32_t* bins, int32_t m, int32_t nB, int32_t nH, const Csr& A, const int64_t* S,
-                  const int64_t* cptr, const BitmapLayout& lay, SplitScratch* split, hipStream_t st) {
+                  const int64_t* cptr, const BitmapLayout& lay, const SpgemmOpts& o, SplitScratch* split,
+                  hipStream_t st) {
   fill_i32(split->counts, 3, 0, st);
   fill_i64((int64_t*)split->sums, 6, 0, st);
   split_wide_kernel<<<(unsigned)ceil_div(nB + nH, kThreads), kThreads, 0, st>>>(
-      bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m, nB, A.ptr, S, cptr, lay.vcap, split->narrow,
-      split->wide, split->counts, split->sums);
+      bins + (size_t)kHeavyTier * m, nH, bins + (size_t)kBitmapTier * m, nB, A.ptr, S, cptr,
+      split_vcap(lay, split, o), split->narrow, split->wide, split->counts, split->sums);
   SMCM_LAUNCH_CHECK();
   // split->sums sits 64 bytes after split->counts (SpgemmCore's counter block): one copy
   SMCM_HIP(hipMemcpyAsync(host_stage() + kStageSplit, split->counts, 64 + 6 * sizeof(unsigned long long),
@@ -1945,7 +1954,7 @@ struct SpgemmCore {
   void stage_split(const int64_t* cptr) {
     const int32_t nB = tc.rows[kBitmapTier], nH = tc.rows[kHeavyTier];
     if (nB + nH == 0) return;
-    launch_split(bins.p, A.nrows, nB, nH, A, S.p, cptr, lay, &split, al.stream);
+    launch_split(bins.p, A.nrows, nB, nH, A, S.p, cptr, lay, o, &split, al.stream);
     split.staged = true;
   }
   void read_symbolic() {

@@ -170,6 +170,20 @@ def test_spgemm_window_tier(ctx, case, window, vals, mode):
     assert_same_csr(ptr, col, val, ref, what=f"{name} window={window} vals={vals} mode={mode}")
 
 
+@pytest.mark.parametrize("nwin_all", [0, 4])
+@pytest.mark.parametrize("window", [0, 4096])
+def test_window_all_nwin_routing(ctx, nwin_all, window):
+    """Products spanning a few windows: the bitmap-tier rows take the bitmap kernel's LDS value
+    slots (window_all_nwin = 0) or all go to the window kernel (the default); same result."""
+    rng = np.random.default_rng(1234 + window)
+    A = random_csr(rng, 300, 200, row_len=lambda r: r.integers(1, 6), with_vals=True)
+    B = random_csr(rng, 200, 20000, row_len=lambda r: int(r.zipf(1.2)) % 6000, with_vals=True)
+    ref = O.spgemm(A, B)
+    with options(ctx, force_tier=2, window_all_nwin=nwin_all, window_cols=window):
+        ptr, col, val = _run(ctx, A, B)
+    assert_same_csr(ptr, col, val, ref, what=f"window_all_nwin={nwin_all} window={window}")
+
+
 def test_spgemm_window_many_entries(ctx):
     """Rows with more A entries than one traversal round (several rounds per window)."""
     rng = np.random.default_rng(77)
