@@ -802,16 +802,21 @@ __global__ void __launch_bounds__(kThreads) sort_tier_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
-// bitmap tier.  Dynamic LDS: bm[NW/32] u32 | rank[NW/64] u32 | val[VCAP] u64 (numeric)
+// bitmap tier.  Dynamic LDS: bm[NW/32] u32 | sum[ceil(NW/2048)] u32 (8-byte padded) |
+// rank[NW/64] u32 | val[VCAP] u64 (numeric).  Bit i of the summary marks a nonzero 64-bit word i
+// of the bitmap, so scans, ranks, column writes and clears visit occupied words only.
+__host__ __device__ inline int64_t bm_sum_words(int64_t nw) { return (nw / 64 + 31) / 32; }
+__host__ __device__ inline int64_t bm_sum_bytes(int64_t nw) { return (bm_sum_words(nw) * 4 + 7) / 8 * 8; }
 struct BitmapLayout {
   int64_t nw = 0;    // window width in columns (multiple of 128)
   int64_t vcap = 0;  // LDS value slots
-  size_t bytes_sym() const { return (size_t)nw / 8; }
-  size_t bytes_num() const { return (size_t)nw / 8 + (size_t)nw / 16 + (size_t)vcap * 8; }
+  size_t bytes_sym() const { return (size_t)(nw / 8 + bm_sum_bytes(nw)); }
+  size_t bytes_num() const { return bytes_sym() + (size_t)nw / 16 + (size_t)vcap * 8; }
 };
 // offsets inside the dynamic LDS block (nw multiple of 128 keeps every section 8-byte aligned)
-__host__ __device__ inline int64_t bm_rank_off(int64_t nw) { return nw / 8; }
-__host__ __device__ inline int64_t bm_val_off(int64_t nw) { return nw / 8 + nw / 16; }
+__host__ __device__ inline int64_t bm_sum_off(int64_t nw) { return nw / 8; }
+__host__ __device__ inline int64_t bm_rank_off(int64_t nw) { return nw / 8 + bm_sum_bytes(nw); }
+__host__ __device__ inline int64_t bm_val_off(int64_t nw) { return bm_rank_off(nw) + nw / 16; }
 
 template <bool NUMERIC, bool AV, bool BV>
 __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol,
@@ -824,7 +829,8 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
                                    int32_t nwin = 0, int32_t* __restrict__ wcnt = nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* bm = (uint32_t*)smem;
-  const uint64_t* bm64 = (const uint64_t*)smem;
+  uint64_t* bm64 = (uint64_t*)smem;
+  uint32_t* sm = (uint32_t*)(smem + bm_sum_off(nw));
   uint32_t* rank = (uint32_t*)(smem + bm_rank_off(nw));
   uint64_t* sval = (uint64_t*)(smem + bm_val_off(nw));
   __shared__ TravLds L;
@@ -839,6 +845,13 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
     cnt = len;
   };
   const int32_t total_rows = n0 + n1;
+  if constexpr (NUMERIC) {
+    // bitmap and summary start all zero; each window of the numeric pass clears only the words
+    // it set
+    for (int64_t i = threadIdx.x; i < nw / 32; i += nthreads) bm[i] = 0u;
+    for (int64_t i = threadIdx.x; i < bm_sum_words(nw); i += nthreads) sm[i] = 0u;
+    __syncthreads();
+  }
   while (true) {
     if (threadIdx.x == 0) cur = atomicAdd(queue, 1);
     __syncthreads();
@@ -861,12 +874,24 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
       const int64_t wlen = min((int64_t)ncols - w0, nw);
       const int64_t nwords32 = (wlen + 31) / 32;
       const int64_t nwords64 = (wlen + 63) / 64;
-      for (int64_t i = threadIdx.x; i < nwords64 * 2; i += nthreads) bm[i] = 0u;
-      __syncthreads();
+      const int64_t nsum = (nwords64 + 31) / 32;
+      if constexpr (!NUMERIC) {
+        // the symbolic pass scans every word anyway: plain clear, no summary
+        for (int64_t i = threadIdx.x; i < nwords64 * 2; i += nthreads) bm[i] = 0u;
+        __syncthreads();
+      }
       traverse<false, false, 4, 2>(rs, re, acol, nullptr, bptr, bcol, nullptr, L, whole_row,
                              [&](int32_t c, uint64_t) {
                                const int64_t x = (int64_t)c - w0;
-                               if (x >= 0 && x < wlen) atomicOr(&bm[x >> 5], 1u << (x & 31));
+                               if (x >= 0 && x < wlen) {
+                                 if constexpr (NUMERIC) {
+                                   // the first bit of a 32-bit word marks its 64-bit word
+                                   if (atomicOr(&bm[x >> 5], 1u << (x & 31)) == 0u)
+                                     atomicOr(&sm[x >> 11], 1u << ((x >> 6) & 31));
+                                 } else {
+                                   atomicOr(&bm[x >> 5], 1u << (x & 31));
+                                 }
+                               }
                              });
       if constexpr (!NUMERIC) {
         int64_t local = 0;
@@ -899,14 +924,17 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
           off += __popcll(bm64[i]);
         }
         __syncthreads();
-        // columns (and zeroed values when accumulating globally): wave per word, lane = bit
-        for (int64_t i = wid; i < nwords64; i += nwaves) {
-          const uint64_t word = bm64[i];
-          if (!word) continue;
-          if ((word >> lane) & 1ull) {
-            const int64_t pos = rowbase + done + rank[i] + __popcll(word & ((1ull << lane) - 1ull));
-            ccol[pos] = (int32_t)(w0 + i * 64 + lane);
-            if (!lds_vals) cval[pos] = 0ull;
+        // columns (and zeroed values when accumulating globally): a wave per summary word, its
+        // occupied 64-bit words one at a time, lane = bit
+        for (int64_t q = wid; q < nsum; q += nwaves) {
+          for (uint32_t bits = sm[q]; bits; bits &= bits - 1) {
+            const int64_t i = q * 32 + __ffs(bits) - 1;
+            const uint64_t word = bm64[i];
+            if ((word >> lane) & 1ull) {
+              const int64_t pos = rowbase + done + rank[i] + __popcll(word & ((1ull << lane) - 1ull));
+              ccol[pos] = (int32_t)(w0 + i * 64 + lane);
+              if (!lds_vals) cval[pos] = 0ull;
+            }
           }
         }
         if (lds_vals) {
@@ -932,6 +960,14 @@ __global__ void bitmap_tier_kernel(const int64_t* __restrict__ aptr, const int32
           for (int64_t i = threadIdx.x; i < wtot; i += nthreads) cval[rowbase + done + i] = sval[i];
         }
         done += wtot;
+        __syncthreads();
+      }
+      if constexpr (NUMERIC) {
+        // clear the words this window set, then their summary
+        for (int64_t q = threadIdx.x; q < nsum; q += nthreads) {
+          for (uint32_t bits = sm[q]; bits; bits &= bits - 1) bm64[q * 32 + __ffs(bits) - 1] = 0ull;
+          sm[q] = 0u;
+        }
         __syncthreads();
       }
     }
@@ -1807,10 +1843,13 @@ BitmapLayout choose_layout(int64_t ncols, const SpgemmOpts& o) {
   const int64_t budget = o.lds_bytes - kStaticLds;
   BitmapLayout l;
   const int64_t want = std::max<int64_t>(128, (ncols + 127) / 128 * 128);
-  // numeric: nw/8 + nw/16 + 8*vcap <= budget with vcap >= 2048
+  // numeric: nw/8 + summary + nw/16 + 8*vcap <= budget with vcap >= 2048
   int64_t max_nw = (budget - 8 * 2048) * 16 / 3;
   max_nw = std::max<int64_t>(128, max_nw / 128 * 128);
   l.nw = std::min(want, max_nw);
+  l.vcap = 2048;
+  while (l.nw > 128 && (int64_t)l.bytes_num() > budget) l.nw -= 128;
+  l.vcap = 0;
   l.vcap = (budget - (int64_t)l.bytes_num()) / 8;
   if (l.vcap > (1 << 16)) l.vcap = 1 << 16;
   if (l.vcap < 0) l.vcap = 0;
