@@ -336,7 +336,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard by source rows, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by source rows: each GPU runs the chain on a slice of start vertices)", file=sys.stderr)
         return 2
     if args.only:
         hg1 = configs.CFG1.build() if args.only != "config3" else None
