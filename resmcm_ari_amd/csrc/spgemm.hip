@@ -428,22 +428,57 @@ __device__ __forceinline__ void traverse(const int64_t rs, const int64_t re, con
                                          const int32_t* __restrict__ bcol, const uint64_t* __restrict__ bval,
                                          TravLds& L, SegF&& seg, F&& f) {
   const int chunk = min(kChunk, (int)blockDim.x);  // entries handled per round (one per thread)
+  const bool mine = (int)threadIdx.x < chunk;
+  // Two-stage prefetch of the entries: while round r expands, the B row extent of round r+1 (its
+  // column arrived during round r-1) and the column of round r+2 are loaded, so a round with
+  // short segments does not wait for acol, then bptr, then bcol.
+  int32_t k0 = -1, k1 = -1;  // columns of rounds r and r+1 (-1: no entry)
+  int64_t lo0 = 0, hi0 = 0;  // B row extent of round r's entry
+  uint64_t a0 = 1ull, a1 = 1ull;
+  {
+    const int64_t e = rs + threadIdx.x;
+    if (mine && e < re) {
+      k0 = acol[e];
+      a0 = AV ? aval[e] : 1ull;
+    }
+    if (k0 >= 0) {
+      lo0 = bptr[k0];
+      hi0 = bptr[k0 + 1];
+    }
+    const int64_t e1 = e + chunk;
+    if (mine && e1 < re) {
+      k1 = acol[e1];
+      a1 = AV ? aval[e1] : 1ull;
+    }
+  }
   for (int64_t base = rs; base < re; base += chunk) {
+    int64_t lo1 = 0, hi1 = 0;
+    if (k1 >= 0) {
+      lo1 = bptr[k1];
+      hi1 = bptr[k1 + 1];
+    }
+    int32_t k2 = -1;
+    uint64_t a2 = 1ull;
+    const int64_t e2 = base + 2 * (int64_t)chunk + threadIdx.x;
+    if (mine && e2 < re) {
+      k2 = acol[e2];
+      a2 = AV ? aval[e2] : 1ull;
+    }
     int64_t st = 0, cnt = 0;
     uint64_t a = 1ull;
-    if ((int)threadIdx.x < chunk) {
-      const int64_t e = base + threadIdx.x;
-      if (e < re) {
-        const int32_t k = acol[e];
-        const int64_t b0 = bptr[k];
-        const int64_t len = bptr[k + 1] - b0;
-        int64_t off = 0;
-        seg(e, k, b0, len, off, cnt);
-        a = AV ? aval[e] : 1ull;
-        st = b0 + off;
-      }
+    if (k0 >= 0) {
+      int64_t off = 0;
+      seg(base + threadIdx.x, k0, lo0, hi0 - lo0, off, cnt);
+      a = a0;
+      st = lo0 + off;
     }
     trav_round<BV, E, STEP>(st, cnt, a, bcol, bval, L, f);
+    k0 = k1;
+    lo0 = lo1;
+    hi0 = hi1;
+    a0 = a1;
+    k1 = k2;
+    a1 = a2;
   }
 }
 
