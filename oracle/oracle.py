@@ -235,21 +235,32 @@ def products_estimate(mats: list[CSR], R: list[np.ndarray], coldims) -> np.ndarr
     return P
 
 
-def dp_products(c: np.ndarray, P: np.ndarray):
+def dp_products(c: np.ndarray, P: np.ndarray, cap: float = np.inf):
     """cost(i,j) = min_k cost(i,k) + cost(k+1,j) + P[i,k,j] (+ c[i,j] below the root), lowest k on
-    ties; returns (cost table, split table, plan pairs)."""
+    ties; intervals below the root with c[i,j] > cap (estimated entries that would not fit in
+    device memory) are excluded unless every plan needs one, then the DP runs without the bound.
+    Returns (cost table, split table, plan pairs)."""
     L = c.shape[0]
     m = np.zeros((L, L))
     s = np.zeros((L, L), np.int32)
-    for l in range(1, L):
-        for i in range(L - l):
-            j = i + l
-            best = np.inf
-            for k in range(i, j):
-                v = m[i, k] + m[k + 1, j] + P[i, k, j]
-                if v < best:
-                    best, s[i, j] = v, k
-            m[i, j] = best + (c[i, j] if l < L - 1 else 0.0)
+
+    def run(bound):
+        for l in range(1, L):
+            for i in range(L - l):
+                j = i + l
+                best = np.inf
+                for k in range(i, j):
+                    v = m[i, k] + m[k + 1, j] + P[i, k, j]
+                    if v < best:
+                        best, s[i, j] = v, k
+                if l == L - 1:
+                    m[i, j] = best
+                else:
+                    m[i, j] = np.inf if c[i, j] > bound else best + c[i, j]
+        return L < 2 or m[0, L - 1] < np.inf
+
+    if not run(cap):
+        run(np.inf)
     pairs = []
 
     def rec(i, j):

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -28,6 +29,11 @@ using smcm::HipError;
 namespace smcm_host {
 
 inline thread_local std::string g_err;
+
+// Device bytes per estimated entry of an intermediate in the product-aware DP's memory bound:
+// 12 for its column and count, 16 for the SpGEMM's per-entry scratch when it is the left
+// operand (row lengths and their prefix), doubled for the operand beside it and the output.
+constexpr double kPlanBytesPerEntry = 56.0;
 
 inline int set_err(int code, const std::string& msg) {
   g_err = msg;
@@ -384,21 +390,9 @@ struct Chain {
       // cost(i,j) = min_k cost(i,k) + cost(k+1,j) + P(i,k,j), plus the estimated size of every
       // intermediate (not of the root); same evaluation order and lowest-k tie rule as the
       // reference DP.  Any plan gives the same product (exact integer arithmetic), so this only
-      // changes the amount of work.
-      for (int32_t l = 1; l < L; ++l) {
-        for (int32_t i = 0; i < L - l; ++i) {
-          const int32_t j = i + l;
-          double best = DBL_MAX;
-          for (int32_t k = i; k < j; ++k) {
-            const double cost = mtab[i * L + k] + mtab[(k + 1) * L + j] + ptab[((size_t)i * L + k) * L + j];
-            if (cost < best) {
-              best = cost;
-              stab[i * L + j] = k;
-            }
-          }
-          mtab[i * L + j] = best + (l < L - 1 ? c[i * L + j] : 0.0);
-        }
-      }
+      // changes the amount of work.  An intermediate estimated above the context's budget (at
+      // kPlanBytesPerEntry bytes per entry) is excluded, unless every plan needs one.
+      if (!product_dp((double)ctx->budget / kPlanBytesPerEntry)) product_dp(HUGE_VAL);
       pairs.clear();
       if (L > 1) plan_rec(0, L - 1);
       dp_time = now_s() - t0;
@@ -421,6 +415,27 @@ struct Chain {
     pairs.clear();
     if (L > 1) plan_rec(0, L - 1);
     dp_time = now_s() - t0;
+  }
+
+  // the product DP with non-root intervals above cap estimated entries excluded; false when
+  // every plan is excluded
+  bool product_dp(double cap) {
+    for (int32_t l = 1; l < L; ++l) {
+      for (int32_t i = 0; i < L - l; ++i) {
+        const int32_t j = i + l;
+        double best = HUGE_VAL;
+        for (int32_t k = i; k < j; ++k) {
+          const double cost = mtab[i * L + k] + mtab[(k + 1) * L + j] + ptab[((size_t)i * L + k) * L + j];
+          if (cost < best) {
+            best = cost;
+            stab[i * L + j] = k;
+          }
+        }
+        if (l == L - 1) mtab[i * L + j] = best;
+        else mtab[i * L + j] = c[i * L + j] > cap ? HUGE_VAL : best + c[i * L + j];
+      }
+    }
+    return mtab[L - 1] < HUGE_VAL;
   }
 
   int32_t plan_rec(int32_t i, int32_t j) {

@@ -186,6 +186,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
       ctx->batch.arena_entries = v * ((int64_t)1 << 20) / 12;
     } else if (k == "kernel_timing") {
       ctx->timer.enabled = v != 0;
+    } else if (k == "budget") {
+      if (v <= 0) fail(SMCM_EINVAL, "budget must be > 0 bytes");
+      ctx->budget = (uint64_t)v;
     } else if (k == "lds_bytes") {
       if (v < 16384) fail(SMCM_EINVAL, "lds_bytes too small");
       int dev_max = 65536;

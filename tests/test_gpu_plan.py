@@ -71,3 +71,49 @@ def test_product_plan_same_result(ctx, graphs, product_plans, schema, p, sub):
     ctx.set_option("plan_cost", 1)
     for a, b in zip(got, ref):
         np.testing.assert_array_equal(a, b)
+
+
+def plan_intervals(pairs, L):
+    """(i, j) of every product of a plan below the root."""
+    st, out = [], []
+    for k, l in np.asarray(pairs).reshape(-1, 2).tolist():
+        a = (k, k) if k >= 0 else st.pop(-2 if l < 0 else -1)
+        b = (l, l) if l >= 0 else st.pop()
+        st.append((a[0], b[1]))
+        if not (a[0] == 0 and b[1] == L - 1):
+            out.append((a[0], b[1]))
+    return out
+
+
+@pytest.mark.parametrize("schema,p", [("bib", p) for p in BIB if p.count("-") >= 3] + [("loc", p) for p in LOC])
+def test_product_plan_memory_bound(ctx, graphs, product_plans, schema, p):
+    """A budget just below the largest intermediate of the unbounded product plan: the plan is
+    the oracle's bounded product DP on the GPU's tables (bound = budget / 56 entries), and the
+    result is unchanged."""
+    hg, G = graphs[schema]
+    mp = configs.metapath_from_letters(p, schema)
+    L = mp.pathLen
+    c, _, _, pairs = G.plan(mp)
+    P = G.products_estimate(mp)
+    big = [c[i, j] for i, j in plan_intervals(pairs, L)]
+    if not big:
+        pytest.skip("the unbounded plan has no intermediate")
+    budget = max(1, int((max(big) - 0.5) * 56))
+    old = ctx.get_option("budget")
+    ctx.set_option("budget", budget)
+    try:
+        _, _, _, got = G.plan(mp)
+        r, _ = G.run(mp)
+        res = r.to_host()
+        r.free()
+    finally:
+        ctx.set_option("budget", old)
+    _, _, want = O.dp_products(c, P, budget / 56.0)
+    np.testing.assert_array_equal(got, want)
+    ctx.set_option("plan_cost", 0)
+    r0, _ = G.run(mp)
+    ref = r0.to_host()
+    r0.free()
+    ctx.set_option("plan_cost", 1)
+    for a, b in zip(res, ref):
+        np.testing.assert_array_equal(a, b)

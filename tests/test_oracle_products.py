@@ -79,3 +79,45 @@ def test_product_dp_vs_bruteforce(seed):
     best = min(tree_cost(t, c, P) for t in trees(0, L - 1))
     assert m[0, L - 1] == best
     assert plan_cost(pairs.tolist(), c, P, L) == best
+
+
+def tree_intervals(t, root=True):
+    """(i, j) of every product below the root of tree t."""
+    if isinstance(t, int):
+        return []
+    a, b, i, k, j = t
+    return tree_intervals(a, False) + tree_intervals(b, False) + ([] if root else [(i, j)])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_product_dp_memory_bound_vs_bruteforce(seed):
+    """Intermediates above the bound are never materialised when some plan avoids them all; when
+    none does, the plan is the unbounded optimum."""
+    rng = np.random.default_rng(100 + seed)
+    L = int(rng.integers(3, 7))
+    c = rng.integers(0, 50, size=(L, L)).astype(np.float64)
+    P = rng.integers(0, 20, size=(L, L, L)).astype(np.float64)
+    all_trees = list(trees(0, L - 1))
+    # the bound just below the largest intermediate of an unbounded optimum (that plan is then
+    # excluded), or at a quantile of the table
+    opt = min(all_trees, key=lambda t: tree_cost(t, c, P))
+    big = [c[i, j] for i, j in tree_intervals(opt)]
+    cap = max(big) - 0.5 if big and seed % 2 == 0 else float(np.quantile(c, [0.3, 0.9][seed % 4 // 2]))
+    ok = [t for t in all_trees if all(c[i, j] <= cap for i, j in tree_intervals(t))]
+    pool = ok if ok else all_trees
+    best = min(tree_cost(t, c, P) for t in pool)
+    m, s, pairs = O.dp_products(c, P, cap)
+    assert m[0, L - 1] == best
+    assert plan_cost(pairs.tolist(), c, P, L) == best
+    if ok:
+        # the chosen plan respects the bound
+        st = []
+        for k, l in pairs.tolist():
+            a = (k, k) if k >= 0 else st.pop(-2 if l < 0 else -1)
+            b = (l, l) if l >= 0 else st.pop()
+            st.append((a[0], b[1]))
+            if not (a[0] == 0 and b[1] == L - 1):
+                assert c[a[0], b[1]] <= cap
+    # no bound: the plain product DP
+    m0, _, pairs0 = O.dp_products(c, P)
+    assert m0[0, L - 1] == min(tree_cost(t, c, P) for t in all_trees)
