@@ -74,8 +74,8 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
  * products, 2-5 the reference's other providers MetaAC / MNC / DMap / LGraph, see
  * smcm_cost_estimate), "lgraph_seed", "window_major", "window_min_products", "window_cols",
  * "w_threads", "window_all_nwin", "window_presence", "sym_window_cols", "sym_merge", "budget" (device
- * bytes the product-aware plan may assume: intermediates estimated above budget / 56 bytes per
- * entry are not materialised when another plan avoids them). */
+ * bytes the product-aware plan may assume: products whose estimated operands and output exceed it
+ * are not planned when another plan avoids them). */
 int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);
 int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* value);
 const char* smcm_last_error(void);

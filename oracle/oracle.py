@@ -235,11 +235,20 @@ def products_estimate(mats: list[CSR], R: list[np.ndarray], coldims) -> np.ndarr
     return P
 
 
-def dp_products(c: np.ndarray, P: np.ndarray, cap: float = np.inf):
+def split_bytes(c: np.ndarray, i: int, k: int, j: int, root: bool) -> float:
+    """Estimated device bytes of the product X_ik * X_k+1,j: 28 per entry of an intermediate left
+    operand (column, count and SpGEMM scratch), 12 per entry of an intermediate right operand and
+    of the output below the root; base relations count 0."""
+    left = c[i, k] if k > i else 0.0
+    right = c[k + 1, j] if j > k + 1 else 0.0
+    out = 0.0 if root else c[i, j]
+    return 28.0 * left + 12.0 * right + 12.0 * out
+
+
+def dp_products(c: np.ndarray, P: np.ndarray, budget: float = np.inf):
     """cost(i,j) = min_k cost(i,k) + cost(k+1,j) + P[i,k,j] (+ c[i,j] below the root), lowest k on
-    ties; intervals below the root with c[i,j] > cap (estimated entries that would not fit in
-    device memory) are excluded unless every plan needs one, then the DP runs without the bound.
-    Returns (cost table, split table, plan pairs)."""
+    ties; splits whose split_bytes exceed budget are excluded unless every plan needs one, then
+    the DP runs without the bound.  Returns (cost table, split table, plan pairs)."""
     L = c.shape[0]
     m = np.zeros((L, L))
     s = np.zeros((L, L), np.int32)
@@ -250,16 +259,15 @@ def dp_products(c: np.ndarray, P: np.ndarray, cap: float = np.inf):
                 j = i + l
                 best = np.inf
                 for k in range(i, j):
+                    if split_bytes(c, i, k, j, l == L - 1) > bound:
+                        continue
                     v = m[i, k] + m[k + 1, j] + P[i, k, j]
                     if v < best:
                         best, s[i, j] = v, k
-                if l == L - 1:
-                    m[i, j] = best
-                else:
-                    m[i, j] = np.inf if c[i, j] > bound else best + c[i, j]
+                m[i, j] = best + (c[i, j] if l < L - 1 else 0.0)
         return L < 2 or m[0, L - 1] < np.inf
 
-    if not run(cap):
+    if not run(budget):
         run(np.inf)
     pairs = []
 

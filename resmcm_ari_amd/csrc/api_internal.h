@@ -30,10 +30,10 @@ namespace smcm_host {
 
 inline thread_local std::string g_err;
 
-// Device bytes per estimated entry of an intermediate in the product-aware DP's memory bound:
-// 12 for its column and count, 16 for the SpGEMM's per-entry scratch when it is the left
-// operand (row lengths and their prefix), doubled for the operand beside it and the output.
-constexpr double kPlanBytesPerEntry = 56.0;
+// Device bytes per estimated entry in the product-aware DP's memory bound: an entry of an
+// operand or output takes 12 (column and count); a left operand's entry also needs 16 of SpGEMM
+// scratch (its B row length and their prefix).
+constexpr double kPlanBytesLeft = 28.0, kPlanBytesEntry = 12.0;
 
 inline int set_err(int code, const std::string& msg) {
   g_err = msg;
@@ -390,9 +390,10 @@ struct Chain {
       // cost(i,j) = min_k cost(i,k) + cost(k+1,j) + P(i,k,j), plus the estimated size of every
       // intermediate (not of the root); same evaluation order and lowest-k tie rule as the
       // reference DP.  Any plan gives the same product (exact integer arithmetic), so this only
-      // changes the amount of work.  An intermediate estimated above the context's budget (at
-      // kPlanBytesPerEntry bytes per entry) is excluded, unless every plan needs one.
-      if (!product_dp((double)ctx->budget / kPlanBytesPerEntry)) product_dp(HUGE_VAL);
+      // changes the amount of work.  A product whose estimated operands and output (the root's
+      // output aside: it may be streamed) exceed the context's budget is excluded, unless every
+      // plan needs one.
+      if (!product_dp((double)ctx->budget)) product_dp(HUGE_VAL);
       pairs.clear();
       if (L > 1) plan_rec(0, L - 1);
       dp_time = now_s() - t0;
@@ -417,22 +418,25 @@ struct Chain {
     dp_time = now_s() - t0;
   }
 
-  // the product DP with non-root intervals above cap estimated entries excluded; false when
-  // every plan is excluded
-  bool product_dp(double cap) {
+  // the product DP without the splits whose estimated device bytes exceed budget (left operand
+  // kPlanBytesLeft per entry, right operand and a non-root output kPlanBytesEntry; base
+  // relations are resident and count 0); false when every plan is excluded
+  bool product_dp(double budget) {
     for (int32_t l = 1; l < L; ++l) {
       for (int32_t i = 0; i < L - l; ++i) {
         const int32_t j = i + l;
         double best = HUGE_VAL;
         for (int32_t k = i; k < j; ++k) {
+          const double left = k > i ? c[i * L + k] : 0.0, right = j > k + 1 ? c[(k + 1) * L + j] : 0.0;
+          const double out = l < L - 1 ? c[i * L + j] : 0.0;
+          if (kPlanBytesLeft * left + kPlanBytesEntry * right + kPlanBytesEntry * out > budget) continue;
           const double cost = mtab[i * L + k] + mtab[(k + 1) * L + j] + ptab[((size_t)i * L + k) * L + j];
           if (cost < best) {
             best = cost;
             stab[i * L + j] = k;
           }
         }
-        if (l == L - 1) mtab[i * L + j] = best;
-        else mtab[i * L + j] = c[i * L + j] > cap ? HUGE_VAL : best + c[i * L + j];
+        mtab[i * L + j] = best + (l < L - 1 ? c[i * L + j] : 0.0);
       }
     }
     return mtab[L - 1] < HUGE_VAL;

@@ -73,32 +73,30 @@ def test_product_plan_same_result(ctx, graphs, product_plans, schema, p, sub):
         np.testing.assert_array_equal(a, b)
 
 
-def plan_intervals(pairs, L):
-    """(i, j) of every product of a plan below the root."""
+def plan_splits(pairs, L):
+    """(i, k, j, is_root) of every product of a plan."""
     st, out = [], []
     for k, l in np.asarray(pairs).reshape(-1, 2).tolist():
         a = (k, k) if k >= 0 else st.pop(-2 if l < 0 else -1)
         b = (l, l) if l >= 0 else st.pop()
         st.append((a[0], b[1]))
-        if not (a[0] == 0 and b[1] == L - 1):
-            out.append((a[0], b[1]))
+        out.append((a[0], a[1], b[1], a[0] == 0 and b[1] == L - 1))
     return out
 
 
 @pytest.mark.parametrize("schema,p", [("bib", p) for p in BIB if p.count("-") >= 3] + [("loc", p) for p in LOC])
 def test_product_plan_memory_bound(ctx, graphs, product_plans, schema, p):
-    """A budget just below the largest intermediate of the unbounded product plan: the plan is
-    the oracle's bounded product DP on the GPU's tables (bound = budget / 56 entries), and the
-    result is unchanged."""
+    """A budget just below the largest product of the unbounded product plan: the plan is the
+    oracle's bounded product DP on the GPU's tables, and the result is unchanged."""
     hg, G = graphs[schema]
     mp = configs.metapath_from_letters(p, schema)
     L = mp.pathLen
     c, _, _, pairs = G.plan(mp)
     P = G.products_estimate(mp)
-    big = [c[i, j] for i, j in plan_intervals(pairs, L)]
-    if not big:
-        pytest.skip("the unbounded plan has no intermediate")
-    budget = max(1, int((max(big) - 0.5) * 56))
+    need = max(O.split_bytes(c, i, k, j, r) for i, k, j, r in plan_splits(pairs, L))
+    if need < 2:
+        pytest.skip("the unbounded plan needs no intermediate")
+    budget = int(need) - 1
     old = ctx.get_option("budget")
     ctx.set_option("budget", budget)
     try:
@@ -108,7 +106,7 @@ def test_product_plan_memory_bound(ctx, graphs, product_plans, schema, p):
         r.free()
     finally:
         ctx.set_option("budget", old)
-    _, _, want = O.dp_products(c, P, budget / 56.0)
+    _, _, want = O.dp_products(c, P, float(budget))
     np.testing.assert_array_equal(got, want)
     ctx.set_option("plan_cost", 0)
     r0, _ = G.run(mp)

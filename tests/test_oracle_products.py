@@ -81,43 +81,44 @@ def test_product_dp_vs_bruteforce(seed):
     assert plan_cost(pairs.tolist(), c, P, L) == best
 
 
-def tree_intervals(t, root=True):
-    """(i, j) of every product below the root of tree t."""
+def tree_splits(t, root=True):
+    """(i, k, j, is_root) of every product of tree t."""
     if isinstance(t, int):
         return []
     a, b, i, k, j = t
-    return tree_intervals(a, False) + tree_intervals(b, False) + ([] if root else [(i, j)])
+    return tree_splits(a, False) + tree_splits(b, False) + [(i, k, j, root)]
 
 
-@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("seed", range(40))
 def test_product_dp_memory_bound_vs_bruteforce(seed):
-    """Intermediates above the bound are never materialised when some plan avoids them all; when
-    none does, the plan is the unbounded optimum."""
+    """Products whose estimated bytes exceed the budget are never planned when some plan avoids
+    them all; when none does, the plan is the unbounded optimum."""
     rng = np.random.default_rng(100 + seed)
     L = int(rng.integers(3, 7))
     c = rng.integers(0, 50, size=(L, L)).astype(np.float64)
     P = rng.integers(0, 20, size=(L, L, L)).astype(np.float64)
     all_trees = list(trees(0, L - 1))
-    # the bound just below the largest intermediate of an unbounded optimum (that plan is then
-    # excluded), or at a quantile of the table
+
+    def need(t):
+        return max(O.split_bytes(c, i, k, j, r) for i, k, j, r in tree_splits(t))
+
+    # the budget just below the largest product of an unbounded optimum (that plan is then
+    # excluded), or at a quantile of every tree's largest product
     opt = min(all_trees, key=lambda t: tree_cost(t, c, P))
-    big = [c[i, j] for i, j in tree_intervals(opt)]
-    cap = max(big) - 0.5 if big and seed % 2 == 0 else float(np.quantile(c, [0.3, 0.9][seed % 4 // 2]))
-    ok = [t for t in all_trees if all(c[i, j] <= cap for i, j in tree_intervals(t))]
+    budget = need(opt) - 0.5 if seed % 2 == 0 else float(np.quantile([need(t) for t in all_trees], 0.4))
+    ok = [t for t in all_trees if need(t) <= budget]
     pool = ok if ok else all_trees
     best = min(tree_cost(t, c, P) for t in pool)
-    m, s, pairs = O.dp_products(c, P, cap)
+    m, s, pairs = O.dp_products(c, P, budget)
     assert m[0, L - 1] == best
     assert plan_cost(pairs.tolist(), c, P, L) == best
     if ok:
-        # the chosen plan respects the bound
         st = []
         for k, l in pairs.tolist():
             a = (k, k) if k >= 0 else st.pop(-2 if l < 0 else -1)
             b = (l, l) if l >= 0 else st.pop()
             st.append((a[0], b[1]))
-            if not (a[0] == 0 and b[1] == L - 1):
-                assert c[a[0], b[1]] <= cap
+            assert O.split_bytes(c, a[0], a[1], b[1], a[0] == 0 and b[1] == L - 1) <= budget
     # no bound: the plain product DP
-    m0, _, pairs0 = O.dp_products(c, P)
+    m0, _, _ = O.dp_products(c, P)
     assert m0[0, L - 1] == min(tree_cost(t, c, P) for t in all_trees)
