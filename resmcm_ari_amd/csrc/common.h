@@ -43,7 +43,8 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // A copy into pageable memory blocks the host until the stream has drained; a copy into pinned
 // memory is queued like a kernel, so several readbacks share one stream synchronisation.  Each
 // thread runs one library call at a time and reads its values right after that call's
-// synchronisation.  Allocated on first use and kept for the life of the thread.
+// synchronisation.  Allocated on first use and never freed (1 KB per host thread that runs a
+// product): freeing it from a thread-exit destructor could run after the HIP runtime has shut down.
 constexpr size_t kHostStageBytes = 1024;
 inline unsigned char* host_stage() {
   thread_local unsigned char* p = nullptr;
