@@ -273,22 +273,29 @@ __global__ void __launch_bounds__(kT) lg_ranks_kernel(const int64_t* __restrict_
   }
 }
 
+// row of every entry: a wave per row, lanes over its entries
+__global__ void __launch_bounds__(kT) lg_rowid_kernel(const int64_t* __restrict__ ptr, int32_t nrows,
+                                                      int32_t* __restrict__ rowid) {
+  const int lane = threadIdx.x % kWave;
+  for (int64_t r = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave; r < nrows; r += (int64_t)gridDim.x * (kT / kWave))
+    for (int64_t e = ptr[r] + lane; e < ptr[r + 1]; e += kWave) rowid[e] = (int32_t)r;
+}
+
 // out[v] = min over entries (u, v) of in[u], elementwise over the 32 ranks (non-negative
-// doubles order like their bit patterns)
-__global__ void __launch_bounds__(kT) lg_minprop_kernel(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                                                        int32_t nrows, const int32_t* __restrict__ rowg,
+// doubles order like their bit patterns).  A thread per (entry, rank), the ranks of an entry in
+// consecutive lanes, so a hub row is spread over many threads and each wave's atomics touch two
+// columns; the minimum does not depend on the order.
+__global__ void __launch_bounds__(kT) lg_minprop_kernel(const int32_t* __restrict__ rowid,
+                                                        const int32_t* __restrict__ col, int64_t nnz,
+                                                        const int32_t* __restrict__ rowg,
                                                         const int32_t* __restrict__ colg, const double* __restrict__ in,
                                                         double* __restrict__ out) {
-  const int64_t total = (int64_t)nrows * kLgR;
+  const int64_t total = nnz * kLgR;
   for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
-    const int64_t r = t / kLgR;
+    const int64_t e = t / kLgR;
     const int idx = (int)(t % kLgR);
-    const int64_t u = rowg[r];
-    const unsigned long long x = (unsigned long long)__double_as_longlong(in[u * kLgR + idx]);
-    for (int64_t e = ptr[r]; e < ptr[r + 1]; ++e) {
-      const int64_t v = colg[col[e]];
-      atomicMin((unsigned long long*)&out[v * kLgR + idx], x);
-    }
+    const double x = in[(int64_t)rowg[rowid[e]] * kLgR + idx];
+    atomicMin((unsigned long long*)&out[(int64_t)colg[col[e]] * kLgR + idx], (unsigned long long)__double_as_longlong(x));
   }
 }
 
@@ -708,8 +715,16 @@ void run_lgraph(const ProviderChain& ch, bool accuracy, ProviderOut& out, DevAll
   const int L = ch.L;
   const int64_t n = ch.n;
   const int64_t cells = n * kLgR;
-  std::vector<DevBuf<int32_t>> mult(L);
-  for (int i = 0; i < L; ++i) lg_column_list(ch, i, accuracy, mult[i], al);
+  std::vector<DevBuf<int32_t>> mult(L), rowid(L);
+  for (int i = 0; i < L; ++i) {
+    lg_column_list(ch, i, accuracy, mult[i], al);
+    const Csr& M = ch.mats[i];
+    rowid[i] = DevBuf<int32_t>(al, M.nnz);
+    if (M.nnz > 0) {
+      lg_rowid_kernel<<<grid_for((int64_t)M.nrows * kWave), kT, 0, al.stream>>>(M.ptr, M.nrows, rowid[i].p);
+      SMCM_LAUNCH_CHECK();
+    }
+  }
   std::vector<long double> c((size_t)L * L, 0.0L);
   DevBuf<double> R(al, cells), T(al, cells), U(al, cells);
   for (int i = 0; i < L; ++i) {
@@ -720,8 +735,9 @@ void run_lgraph(const ProviderChain& ch, bool accuracy, ProviderOut& out, DevAll
     if (M.nrows > 0) {
       lg_ranks_kernel<<<grid_for((int64_t)M.nrows * kLgR), kT, 0, al.stream>>>(M.ptr, M.nrows, ch.rowg[i],
                                                                                 accuracy ? 1 : 0, ch.seed, i, R.p);
-      lg_minprop_kernel<<<grid_for((int64_t)M.nrows * kLgR), kT, 0, al.stream>>>(M.ptr, M.col, M.nrows, ch.rowg[i],
-                                                                                  ch.colg[i], R.p, T.p);
+      if (M.nnz > 0)
+        lg_minprop_kernel<<<grid_for(M.nnz * kLgR), kT, 0, al.stream>>>(rowid[i].p, M.col, M.nnz, ch.rowg[i], ch.colg[i],
+                                                                 R.p, T.p);
     }
     lg_select_kernel<<<grid_for(cells), kT, 0, al.stream>>>(R.p, T.p, mult[i].p, n, DBL_MAX);
     SMCM_LAUNCH_CHECK();
@@ -733,9 +749,9 @@ void run_lgraph(const ProviderChain& ch, bool accuracy, ProviderOut& out, DevAll
       const Csr& Mj = ch.mats[j];
       double* dst = left == T.p ? U.p : T.p;
       lg_fill_kernel<<<grid_for(cells), kT, 0, al.stream>>>(dst, cells, DBL_MAX);
-      if (Mj.nrows > 0)
-        lg_minprop_kernel<<<grid_for((int64_t)Mj.nrows * kLgR), kT, 0, al.stream>>>(Mj.ptr, Mj.col, Mj.nrows,
-                                                                                     ch.rowg[j], ch.colg[j], left, dst);
+      if (Mj.nnz > 0)
+        lg_minprop_kernel<<<grid_for(Mj.nnz * kLgR), kT, 0, al.stream>>>(rowid[j].p, Mj.col, Mj.nnz, ch.rowg[j], ch.colg[j],
+                                                                  left, dst);
       if (!accuracy) lg_select_kernel<<<grid_for(cells), kT, 0, al.stream>>>(dst, nullptr, mult[j].p, n, DBL_MAX);
       SMCM_LAUNCH_CHECK();
       c[i * L + j] = lg_estimate(dst, mult[j].p, n, al);
