@@ -60,33 +60,38 @@ __global__ void __launch_bounds__(kT) metaac_terms_kernel(const int64_t* __restr
 }
 
 // ---- MNC --------------------------------------------------------------------------------
+// a wave per row (lanes over its entries), so hub rows are spread over 64 lanes
 __global__ void __launch_bounds__(kT) mnc_counts_kernel(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                         int32_t nrows, const int32_t* __restrict__ rowg,
                                                         const int32_t* __restrict__ colg, int32_t* __restrict__ h_r,
                                                         int32_t* __restrict__ h_c) {
-  for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kT) {
+  const int lane = lane_id();
+  for (int64_t r = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave; r < nrows; r += (int64_t)gridDim.x * (kT / kWave)) {
     const int64_t a = ptr[r], b = ptr[r + 1];
-    h_r[rowg[r]] = (int32_t)(b - a);
-    for (int64_t e = a; e < b; ++e) atomicAdd(&h_c[colg[col[e]]], 1);
+    if (lane == 0) h_r[rowg[r]] = (int32_t)(b - a);
+    for (int64_t e = a + lane; e < b; e += kWave) atomicAdd(&h_c[colg[col[e]]], 1);
   }
 }
 
 // extended sketches (mnc.cpp:640-656): h_er[u] counts entries whose column has count 1;
-// h_ec[v] counts entries whose ROW id has column count 1 (the reference tests colSketch at u)
+// h_ec[v] counts entries whose ROW id has column count 1 (the reference tests colSketch at u).
+// A wave per row, as above.
 __global__ void __launch_bounds__(kT) mnc_ext_kernel(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                      int32_t nrows, const int32_t* __restrict__ rowg,
                                                      const int32_t* __restrict__ colg, const int32_t* __restrict__ h_c,
                                                      int32_t* __restrict__ h_er, int32_t* __restrict__ h_ec) {
-  for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kT) {
+  const int lane = lane_id();
+  for (int64_t r = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave; r < nrows; r += (int64_t)gridDim.x * (kT / kWave)) {
     const int32_t u = rowg[r];
     const bool urow1 = h_c[u] == 1;
-    int32_t er = 0;
-    for (int64_t e = ptr[r]; e < ptr[r + 1]; ++e) {
+    int64_t er = 0;
+    for (int64_t e = ptr[r] + lane; e < ptr[r + 1]; e += kWave) {
       const int32_t v = colg[col[e]];
       er += h_c[v] == 1;
       if (urow1) atomicAdd(&h_ec[v], 1);
     }
-    if (er) atomicAdd(&h_er[u], er);
+    er = wave_sum_i64(er);
+    if (lane == 0 && er) atomicAdd(&h_er[u], (int32_t)er);
   }
 }
 
@@ -201,9 +206,13 @@ __global__ void __launch_bounds__(kT) dmap_count_kernel(const int64_t* __restric
                                                         int32_t nrows, const int32_t* __restrict__ rowg,
                                                         const int32_t* __restrict__ colg, int32_t r0, int32_t c0,
                                                         int32_t nc, double* __restrict__ map) {
-  for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kT) {
+  // a wave per row (lanes over its entries); counts are exact in binary64, so the order of the
+  // additions does not matter
+  const int lane = lane_id();
+  for (int64_t r = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave; r < nrows; r += (int64_t)gridDim.x * (kT / kWave)) {
     const int64_t bi = rowg[r] / kDmapB - r0;
-    for (int64_t e = ptr[r]; e < ptr[r + 1]; ++e) atomicAdd(&map[bi * nc + (colg[col[e]] / kDmapB - c0)], 1.0);
+    for (int64_t e = ptr[r] + lane; e < ptr[r + 1]; e += kWave)
+      atomicAdd(&map[bi * nc + (colg[col[e]] / kDmapB - c0)], 1.0);
   }
 }
 
@@ -440,9 +449,9 @@ void mnc_leaf(const ProviderChain& ch, int i, Sk& k, DevAlloc& al) {
   fill_i32(k.h_er.p, n, 0, al.stream);
   fill_i32(k.h_ec.p, n, 0, al.stream);
   if (M.nrows > 0) {
-    mnc_counts_kernel<<<grid_for(M.nrows), kT, 0, al.stream>>>(M.ptr, M.col, M.nrows, ch.rowg[i], ch.colg[i], k.h_r.p,
+    mnc_counts_kernel<<<grid_for((int64_t)M.nrows * kWave), kT, 0, al.stream>>>(M.ptr, M.col, M.nrows, ch.rowg[i], ch.colg[i], k.h_r.p,
                                                               k.h_c.p);
-    mnc_ext_kernel<<<grid_for(M.nrows), kT, 0, al.stream>>>(M.ptr, M.col, M.nrows, ch.rowg[i], ch.colg[i], k.h_c.p,
+    mnc_ext_kernel<<<grid_for((int64_t)M.nrows * kWave), kT, 0, al.stream>>>(M.ptr, M.col, M.nrows, ch.rowg[i], ch.colg[i], k.h_c.p,
                                                            k.h_er.p, k.h_ec.p);
     SMCM_LAUNCH_CHECK();
   }
@@ -587,7 +596,7 @@ void dmap_leaf(const ProviderChain& ch, int i, BMap& b, DevAlloc& al) {
   b.d = DevBuf<double>(al, cells);
   fill_i64((int64_t*)b.d.p, cells, 0, al.stream);
   if (M.nrows > 0) {
-    dmap_count_kernel<<<grid_for(M.nrows), kT, 0, al.stream>>>(M.ptr, M.col, M.nrows, ch.rowg[i], ch.colg[i], b.r0,
+    dmap_count_kernel<<<grid_for((int64_t)M.nrows * kWave), kT, 0, al.stream>>>(M.ptr, M.col, M.nrows, ch.rowg[i], ch.colg[i], b.r0,
                                                               b.c0, b.nc, b.d.p);
     SMCM_LAUNCH_CHECK();
   }
