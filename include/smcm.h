@@ -64,7 +64,12 @@ typedef struct smcm_stats {
 } smcm_stats;
 
 /* ---- context ---------------------------------------------------------------------------- */
-/* hbm_budget: bytes the library may hold for results and intermediates (0 = 90% of free HBM). */
+/* The context (device, stream, allocator), options, error text, statistics and timing replace
+ * the reference's process-global state: the timers es_time / dp_time / spgemm_time / smcm_time
+ * (efficient/resmcm.cpp:18-21), the OpenMP thread count of main (:795-796) and its thrown
+ * messages (:257, :295).  smcm_stats, smcm_ctx_sync, smcm_kernel_stats and smcm_copy_bandwidth
+ * have no counterpart there.
+ * hbm_budget: bytes the library may hold for results and intermediates (0 = 90% of free HBM). */
 int smcm_ctx_create(int device, uint64_t hbm_budget, smcm_ctx** out);
 int smcm_ctx_destroy(smcm_ctx* ctx);
 /* Tuning / mode keys: "s_max", "m_max", "dense_div", "heavy_ub", "d_threads", "force_tier",
