@@ -49,6 +49,9 @@ def lib() -> C.CDLL:
         L.orc_execute_plan.argtypes = [C.c_int32, C.POINTER(PC), C.c_void_p, PC]
         L.orc_row_digest.argtypes = [PC, C.c_void_p]
         L.orc_row_digest.restype = None
+        L.orc_plan_digest.argtypes = [C.c_int32, C.POINTER(PC), C.c_void_p, C.c_int32, C.c_void_p,
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.orc_plan_digest.restype = C.c_int
         L.orc_products.argtypes = [PC, PC]
         L.orc_products.restype = C.c_int64
         L.orc_num_threads.restype = C.c_int
@@ -168,6 +171,39 @@ def execute_plan(mats: list[CSR], pairs) -> CSR:
     if lib().orc_execute_plan(L, arr, pp.ctypes.data if pp.size else None, C.byref(out)) != 0:
         raise ValueError("plan execution failed")
     return CSR._from_struct(out)
+
+
+def plan_digest(mats: list[CSR], pairs, block_rows: int = 1 << 14):
+    """Per-row digests of the plan's result (L >= 2) without materialising it, row block by row
+    block of M_0; returns (digests, nnz, intermediate products of the plan over all rows)."""
+    L = len(mats)
+    arr, keep = _struct_array(mats)
+    pp = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).ravel())
+    dig = np.zeros(mats[0].nrows, dtype=np.uint64)
+    nnz, prods = C.c_int64(), C.c_int64()
+    if lib().orc_plan_digest(L, arr, pp.ctypes.data, int(block_rows), dig.ctypes.data, C.byref(nnz),
+                             C.byref(prods)) != 0:
+        raise ValueError("plan digest failed")
+    return dig, int(nnz.value), int(prods.value)
+
+
+def plan_products(mats: list[CSR], pairs) -> int:
+    """Intermediate products of a plan (sum over its SpGEMMs of sum_{(u,k) in A} |row_k(B)|),
+    from the operands' structure: the product sizes come from the oracle's SpGEMM."""
+    stack, total = [], 0
+    steps = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+    for q, (k, l) in enumerate(steps):
+        A = mats[k] if k >= 0 else stack.pop(-2 if l < 0 else -1)
+        B = mats[l] if l >= 0 else stack.pop()
+        total += products(A, B)
+        if q < len(steps) - 1:  # the root's own result is not needed
+            stack.append(spgemm(A, B))
+    return total
+
+
+def left_to_right(L: int) -> np.ndarray:
+    """The plan (0,1), (-1,2), ..., (-1,L-1)."""
+    return np.array([[0, 1]] + [[-1, q] for q in range(2, L)], dtype=np.int32)[: max(L - 1, 0)]
 
 
 def products(A: CSR, B: CSR) -> int:

@@ -341,6 +341,147 @@ void orc_row_digest(const orc_csr* C, uint64_t* out) {
   }
 }
 
+/* Rows of A (a row range of a CSR, ptr may start above 0) times B: per-row digests of the
+ * product as orc_row_digest computes them, without materialising it.  The digest is a sum over
+ * the row's entries, so the columns need no sorting.  Returns the number of entries; products
+ * receives sum over A's entries of |row_k(B)|. */
+static int64_t spgemm_digest_rows(const orc_csr* A, const orc_csr* B, uint64_t* out, int64_t* products) {
+  const int32_t m = A->nrows, n = B->ncols;
+  int nt = omp_get_max_threads();
+  unsigned char* marks = (unsigned char*)calloc((size_t)nt * (size_t)(n > 0 ? n : 1), 1);
+  int32_t** lists = (int32_t**)calloc((size_t)nt, sizeof(int32_t*));
+  uint64_t** accs = (uint64_t**)calloc((size_t)nt, sizeof(uint64_t*));
+  for (int t = 0; t < nt; ++t) {
+    lists[t] = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    accs[t] = (uint64_t*)calloc((size_t)(n > 0 ? n : 1), sizeof(uint64_t));
+  }
+  int64_t nnz = 0, prods = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : nnz, prods)
+  for (int32_t u = 0; u < m; ++u) {
+    int tid = omp_get_thread_num();
+    unsigned char* mk = marks + (size_t)tid * (size_t)(n > 0 ? n : 1);
+    int32_t* ls = lists[tid];
+    uint64_t* acc = accs[tid];
+    int64_t w = 0;
+    for (int64_t x = A->ptr[u]; x < A->ptr[u + 1]; ++x) {
+      int32_t k = A->col[x];
+      uint64_t a = A->val ? A->val[x] : 1u;
+      prods += B->ptr[k + 1] - B->ptr[k];
+      for (int64_t y = B->ptr[k]; y < B->ptr[k + 1]; ++y) {
+        int32_t v = B->col[y];
+        acc[v] += a * (B->val ? B->val[y] : 1u);
+        if (!mk[v]) {
+          mk[v] = 1;
+          ls[w++] = v;
+        }
+      }
+    }
+    uint64_t h = 0;
+    for (int64_t q = 0; q < w; ++q) {
+      int32_t v = ls[q];
+      h += mix64(mix64((uint64_t)(uint32_t)v ^ 0x9E3779B97F4A7C15ull) ^ acc[v]);
+      acc[v] = 0;
+      mk[v] = 0;
+    }
+    out[u] = h + mix64((uint64_t)w ^ 0x243F6A8885A308D3ull);
+    nnz += w;
+  }
+  for (int t = 0; t < nt; ++t) {
+    free(lists[t]);
+    free(accs[t]);
+  }
+  free(lists);
+  free(accs);
+  free(marks);
+  *products += prods;
+  return nnz;
+}
+
+/* products of the rows of A (ptr may start above 0) against B */
+static int64_t products_rows(const orc_csr* A, const orc_csr* B) {
+  int64_t ip = 0;
+  for (int32_t u = 0; u < A->nrows; ++u)
+    for (int64_t x = A->ptr[u]; x < A->ptr[u + 1]; ++x) ip += B->ptr[A->col[x] + 1] - B->ptr[A->col[x]];
+  return ip;
+}
+
+int orc_plan_digest(int32_t L, const orc_csr* const* mats, const int32_t* pairs, int32_t block_rows,
+                    uint64_t* digest, int64_t* nnz_out, int64_t* products_out) {
+  if (L < 2 || block_rows < 1) return -1;
+  /* phase 1: the plan's products that do not involve position 0, once (the stack machine of
+   * resmcm.cpp:709-760 over intervals; a step's result covers [lo, hi]) */
+  orc_csr* pre = (orc_csr*)calloc((size_t)L, sizeof(orc_csr));
+  int32_t* lo = (int32_t*)malloc(sizeof(int32_t) * (size_t)L);
+  int32_t* src = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)L); /* operand: leaf x >= 0, step -(p+1) */
+  int32_t* st_step = (int32_t*)malloc(sizeof(int32_t) * (size_t)L);  /* stack of step ids */
+  int32_t top = 0;
+  int64_t products = 0;
+  int rc = 0;
+  for (int32_t p = 0; p < L - 1; ++p) {
+    int32_t k = pairs[2 * p], l = pairs[2 * p + 1];
+    int32_t a, b;
+    if (k >= 0 && l >= 0) {
+      a = k;
+      b = l;
+    } else if (k < 0 && l >= 0) {
+      a = -(st_step[--top] + 1);
+      b = l;
+    } else if (k >= 0 && l < 0) {
+      a = k;
+      b = -(st_step[--top] + 1);
+    } else {
+      b = -(st_step[--top] + 1);
+      a = -(st_step[--top] + 1);
+    }
+    src[2 * p] = a;
+    src[2 * p + 1] = b;
+    lo[p] = a >= 0 ? a : lo[-a - 1];
+    st_step[top++] = p;
+    if (lo[p] > 0) {
+      const orc_csr* A = a >= 0 ? mats[a] : &pre[-a - 1];
+      const orc_csr* B = b >= 0 ? mats[b] : &pre[-b - 1];
+      products += products_rows(A, B);
+      if ((rc = orc_spgemm(A, B, &pre[p])) != 0) break;
+    }
+  }
+  /* phase 2: row blocks of M_0 through the steps that involve position 0; the root (the last
+   * step) is reduced to row digests */
+  int64_t nnz = 0;
+  orc_csr* blk = (orc_csr*)calloc((size_t)L, sizeof(orc_csr));
+  const orc_csr* M0 = mats[0];
+  for (int32_t r0 = 0; rc == 0 && r0 < M0->nrows; r0 += block_rows) {
+    const int32_t r1 = M0->nrows - r0 < block_rows ? M0->nrows : r0 + block_rows;
+    orc_csr view = *M0;
+    view.nrows = r1 - r0;
+    view.ptr = M0->ptr + r0;
+    for (int32_t p = 0; p < L - 1 && rc == 0; ++p) {
+      if (lo[p] > 0) continue;
+      const int32_t a = src[2 * p], b = src[2 * p + 1];
+      const orc_csr* A = a == 0 ? &view : (a > 0 ? mats[a] : &blk[-a - 1]);
+      const orc_csr* B = b >= 0 ? mats[b] : (lo[-b - 1] > 0 ? &pre[-b - 1] : &blk[-b - 1]);
+      if (p == L - 2) {
+        nnz += spgemm_digest_rows(A, B, digest + r0, &products);
+      } else {
+        products += products_rows(A, B);
+        rc = orc_spgemm(A, B, &blk[p]);
+      }
+      if (a < 0) orc_csr_free(&blk[-a - 1]);
+    }
+  }
+  for (int32_t p = 0; p < L - 1; ++p) {
+    if (pre[p].ptr) orc_csr_free(&pre[p]);
+    if (blk[p].ptr) orc_csr_free(&blk[p]);
+  }
+  free(pre);
+  free(blk);
+  free(lo);
+  free(src);
+  free(st_step);
+  if (nnz_out) *nnz_out = nnz;
+  if (products_out) *products_out = products;
+  return rc;
+}
+
 int64_t orc_products(const orc_csr* A, const orc_csr* B) {
   int64_t ip = 0;
   for (int64_t x = 0; x < A->nnz; ++x) ip += B->ptr[A->col[x] + 1] - B->ptr[A->col[x]];

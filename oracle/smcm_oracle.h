@@ -68,7 +68,16 @@ int orc_spgemm(const orc_csr* A, const orc_csr* B, orc_csr* C);
  * base matrices are not freed; the result is written to out. */
 int orc_execute_plan(int32_t L, const orc_csr* const* mats, const int32_t* pairs, orc_csr* out);
 
-/* Per-row digest of (col, val) sequences (order-sensitive). */
+/* The result of a plan (L >= 2) reduced to per-row digests (as orc_row_digest) without
+ * materialising it: the plan's products that do not involve position 0 are computed once,
+ * those that do run on row blocks of M_0 of block_rows rows, and the root is digested row by
+ * row.  digest[mats[0]->nrows]; *nnz_out = entries of the result; *products_out = the plan's
+ * intermediate products over all rows.  Returns 0, or -1 for L < 2 or block_rows < 1. */
+int orc_plan_digest(int32_t L, const orc_csr* const* mats, const int32_t* pairs, int32_t block_rows,
+                    uint64_t* digest, int64_t* nnz_out, int64_t* products_out);
+
+/* Per-row digest: mix(width) + sum over the row's entries of mix(mix(col) ^ val), mod 2^64
+ * (mix = the splitmix64 finaliser; rows are sorted, so the order-free sum identifies a row). */
 void orc_row_digest(const orc_csr* C, uint64_t* out);
 
 /* Number of intermediate products of C = A*B: sum over A's entries of |row_k(B)|. */

@@ -5,6 +5,7 @@
   C++ chain executor.  Built with ``hipcc --offload-arch=gfx950``.
 * ``lib/smcm_resmcm``    - the drop-in command line program (same argv and output line as the
   reference's ``efficient/resmcm``).
+* ``lib/smcm_res``       - the drop-in estimate-only program (``accuracy/res``).
 """
 from __future__ import annotations
 
@@ -84,8 +85,11 @@ def build_smcm(jobs: int = 8) -> str:
 def build_cli() -> str:
     out = os.path.join(LIBDIR, "smcm_resmcm")
     srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES]
-    _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, *srcs, "-o", out,
+    _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, "-I" + CSRC, *srcs, "-o", out,
           "-L" + LIBDIR, "-lsmcm", "-lsmcm_hin", "-Wl,-rpath,$ORIGIN"])
+    # the estimate-only program (accuracy/res)
+    _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, "-I" + CSRC, os.path.join(CSRC, "cli_res.cpp"), "-o",
+          os.path.join(LIBDIR, "smcm_res"), "-L" + LIBDIR, "-lsmcm", "-lsmcm_hin", "-Wl,-rpath,$ORIGIN"])
     # meta-path enumeration (host only)
     _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(CSRC, "cli_metapaths.cpp"), "-o",
           os.path.join(LIBDIR, "smcm_metapaths"), "-L" + LIBDIR, "-lsmcm_hin", "-Wl,-rpath,$ORIGIN"])

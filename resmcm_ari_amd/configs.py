@@ -85,7 +85,8 @@ CFG3 = LocSpec(nU=100_000, nL=200_000, nC=500, nY=1_000, nS=168, seed=SEED_BASE 
 
 CFG1_PATHS = ["A-P-V-P-A"]
 CFG2_PATHS = ["A-P-V-P-A", "A-P-T-P-A"]
-CFG3_PATHS = ["U-L-C-L-Y-L-U-L-S-L-U", "U-L-Y-L-U-L-U", "U-L-S-L-C-L-U-L-Y",
+# long location meta-paths, L = 8-12 matrices
+CFG3_PATHS = ["U-L-C-L-Y-L-U-L-S-L-U", "U-L-Y-L-S-L-C-L-U", "U-L-S-L-C-L-U-L-Y",
               "U-L-C-L-U-L-Y-L-S-L-U-L-U"]
 
 

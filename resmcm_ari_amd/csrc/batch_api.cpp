@@ -51,21 +51,6 @@ struct Member {
   int64_t dig_off = 0;
 };
 
-__global__ void segment_sum_kernel(const int32_t* len, const int64_t* off, const int32_t* cnt, int64_t* out) {
-  __shared__ int64_t red[256];
-  const int32_t s = blockIdx.x;
-  const int64_t o = off[s];
-  int64_t x = 0;
-  for (int32_t i = threadIdx.x; i < cnt[s]; i += 256) x += len[o + i];
-  red[threadIdx.x] = x;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[s] = red[0];
-}
-
 template <class T>
 T* upload(DevAlloc& al, const std::vector<T>& h) {
   T* d = al.alloc_n<T>((int64_t)h.size());

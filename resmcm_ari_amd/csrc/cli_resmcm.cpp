@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "cli_common.h"
 #include "smcm.h"
 #include "smcm_hin.h"
 
@@ -27,15 +28,7 @@ int main(int argc, char** argv) {
   const int threads = std::atoi(argv[1]);
   const double iterations = std::atof(argv[2]);
   const int device = argc == 9 ? std::atoi(argv[8]) : 0;
-  std::cout << "******************** input parameters ***************************" << std::endl;
-  std::cout << "*\t# of thread:" << threads << std::endl;
-  std::cout << "*\t# of iteration:" << iterations << std::endl;
-  std::cout << "*\tthe input graph file:" << argv[3] << std::endl;
-  std::cout << "*\tthe input vertex file:" << argv[4] << std::endl;
-  std::cout << "*\tthe input edge file:" << argv[5] << std::endl;
-  std::cout << "*\tthe input meta-path:" << argv[6] << std::endl;
-  std::cout << "*\tthe output file:" << argv[7] << std::endl;
-  std::cout << "*****************************************************************" << std::endl;
+  smcm_cli::banner(threads, iterations, argv);
 
   hin_graph* hg = nullptr;
   if (hin_read_text(argv[3], argv[4], argv[5], &hg) != 0) {
@@ -64,30 +57,7 @@ int main(int argc, char** argv) {
 
   // read_metapathesN (:339-357): stop at the first malformed line
   std::vector<std::vector<int32_t>> vts, ets;
-  {
-    std::ifstream f(argv[6]);
-    std::string line;
-    while (std::getline(f, line)) {
-      std::string tok;
-      for (char ch : line) {
-        if (ch == ' ' || ch == '\t' || ch == '\r') {
-          if (!tok.empty()) break;
-          continue;
-        }
-        tok.push_back(ch);
-      }
-      std::vector<int32_t> v(4096), e(4096);
-      int32_t L = 0;
-      if (tok.empty() || hin_parse_metapath(tok.c_str(), v.data(), e.data(), 4096, &L) != 0) {
-        std::cerr << "the meta-path is incorrect" << std::endl;
-        break;
-      }
-      v.resize(L + 1);
-      e.resize(L);
-      vts.push_back(v);
-      ets.push_back(e);
-    }
-  }
+  smcm_cli::read_metapaths(argv[6], vts, ets);
   std::cout << "*\tsuccessfully get metapathesN." << std::endl;
 
   int rc = 0;

@@ -23,6 +23,7 @@
 #include <thread>
 
 #include "api_internal.h"
+#include "server_sched.h"
 #include "smcm.h"
 
 using namespace smcm_host;
@@ -33,16 +34,15 @@ double mono_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-struct Query {
-  int64_t ticket = 0;
+// A query: its meta-path, sources and graph, and what the executor reports (scheduling state,
+// status and times live in the Job base)
+struct Query : smcm_sched::Job {
   int32_t graph = 0;
   int32_t L = 0;
   std::vector<int32_t> vts, ets, src;  // src: global ids (empty + whole: all rows)
   bool whole = false;
   uint64_t* digest = nullptr;  // caller's buffer (rows of the result), may be null
-  smcm_query_out out{};
-  bool done = false;
-  bool large = false;  // planned as large: runs on a large-lane worker
+  smcm_query_out out{};        // nnz, predicted bytes, cache use, blocks
 };
 
 struct CacheEntry {
@@ -174,21 +174,8 @@ struct smcm_server {
   std::vector<smcm_graph*> graphs;
   smcm_server_opts opts{};
   std::vector<smcm_ctx*> wctx;
-  std::vector<std::thread> workers;
-  std::mutex mu;
-  std::condition_variable cv_work, cv_done, cv_budget;
-  std::deque<Query*> pending;      // submitted, not yet planned
-  std::deque<Query*> large_ready;  // planned as large, waiting for a large-lane worker
-  int small_lanes = 0;             // workers 0..small_lanes-1 hand large queries over
-  std::map<int64_t, std::unique_ptr<Query>> all;
-  int64_t next_ticket = 1;
-  uint64_t inflight = 0;  // predicted bytes of the admitted queries
-  int admitted = 0;       // queries between admission and release
-  int running = 0;
-  bool stop = false;
-  double t0 = 0;
+  std::unique_ptr<smcm_sched::Scheduler> sched;  // queues, lanes, admission, shutdown
   PrefixCache cache;
-  int64_t completed = 0;
 };
 
 namespace {
@@ -287,30 +274,17 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
   };
   // admission: held from admit() until the query's device work is done (RAII, also on errors)
   struct Admission {
-    smcm_server* s;
+    smcm_sched::Scheduler& sc;
     Query* q = nullptr;
-    uint64_t bytes = 0;
-    bool held = false;
     void admit(uint64_t b, Query& qq) {
       q = &qq;
-      std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_budget.wait(lk, [&] { return s->admitted == 0 || s->inflight + b <= s->opts.budget_bytes; });
-      s->inflight += b;
-      s->admitted++;
-      bytes = b;
-      held = true;
-      q->out.t_start = mono_s() - s->t0;
-      q->out.predicted_bytes = b;
+      sc.admit(qq, b);
+      qq.out.predicted_bytes = b;
     }
     ~Admission() {
-      if (!held) return;
-      std::lock_guard<std::mutex> lk(s->mu);
-      q->out.t_done = mono_s() - s->t0;  // the query's device work ends before its budget is freed
-      s->inflight -= bytes;
-      s->admitted--;
-      s->cv_budget.notify_all();
+      if (q) sc.release(*q);  // the query's device work ends before its budget is freed
     }
-  } adm{s};
+  } adm{*s->sched};
   if (hit && kk == q.L - 1) {  // the whole chain is cached
     adm.admit(0, q);
     finish_digest(hit->C);
@@ -342,10 +316,7 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
   double root = 0;
   const double pred = predicted_bytes(ch, s->opts.block_bytes, root);
   if (small_lane && pred > (double)s->opts.large_bytes) {
-    std::lock_guard<std::mutex> lk(s->mu);
-    q.large = true;
-    s->large_ready.push_back(&q);
-    s->cv_work.notify_all();
+    s->sched->hand_over(q);
     return false;
   }
   const uint64_t quarter = std::max<uint64_t>(s->opts.budget_bytes / 4, 1);
@@ -385,64 +356,52 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
   return true;
 }
 
-void worker_loop(smcm_server* s, int wi) {
+// the scheduler's executor: runs one query on worker wi's context and maps failures to a status
+bool run_job(smcm_server* s, smcm_sched::Job& job, int wi, bool small_lane) {
+  Query& q = static_cast<Query&>(job);
   smcm_ctx* w = s->wctx[wi];
-  (void)hipSetDevice(w->device);
-  const bool small_lane = wi < s->small_lanes;
-  for (;;) {
-    Query* q = nullptr;
-    {
-      std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] {
-        return s->stop || !s->pending.empty() || (!small_lane && !s->large_ready.empty());
-      });
-      if (!small_lane && !s->large_ready.empty()) {
-        q = s->large_ready.front();
-        s->large_ready.pop_front();
-      } else if (!s->pending.empty()) {
-        q = s->pending.front();
-        s->pending.pop_front();
-      } else {
-        return;  // stop, nothing left
-      }
-      s->running++;
+  int status = SMCM_OK;
+  std::string msg;
+  bool finished = true;
+  try {
+    SMCM_HIP(hipSetDevice(w->device));
+    finished = run_query(s, w, q, small_lane);
+  } catch (const ApiError& e) {
+    status = e.code;
+    msg = e.msg;
+  } catch (const HipError& e) {
+    status = e.code == (int)hipErrorOutOfMemory ? SMCM_ENOMEM : SMCM_EDEVICE;
+    msg = e.what();
+    (void)hipGetLastError();
+  } catch (const std::exception& e) {
+    status = SMCM_EDEVICE;
+    msg = e.what();
+  }
+  if (!finished) return false;
+  q.status = status;
+  q.error = msg;
+  if (status != SMCM_OK && std::getenv("SMCM_TRACE")) fprintf(stderr, "[smcm] query failed: %s\n", msg.c_str());
+  return true;
+}
+
+// checks of a query that smcm_server_submit and smcm_server_replay make before anything is queued
+void check_query(smcm_server* s, int32_t graph, const int32_t* vtypes, const int32_t* etypes, int32_t L,
+                 const int32_t* src_rows, int64_t n_src) {
+  if (!vtypes || !etypes) fail(SMCM_EINVAL, "null meta-path");
+  if (graph < 0 || graph >= (int32_t)s->graphs.size()) fail(SMCM_EINVAL, "graph index out of range");
+  if (L < 1 || L > 16) fail(SMCM_EINVAL, "the meta-path is incorrect: L must be in [1, 16]");
+  smcm_graph* g = s->graphs[graph];
+  for (int32_t q = 0; q <= L; ++q)
+    if (vtypes[q] < 0 || vtypes[q] >= g->ntypes) fail(SMCM_EINVAL, "the meta-path is incorrect: vertex type");
+  for (int32_t q = 0; q < L; ++q)
+    if (etypes[q] < 0) fail(SMCM_EINVAL, "the meta-path is incorrect: negative edge type");
+  if (src_rows) {
+    if (n_src < 0 || n_src > INT32_MAX) fail(SMCM_EINVAL, "bad source row count");
+    for (int64_t r = 0; r < n_src; ++r) {
+      const int32_t v = src_rows[r];
+      if (v < 0 || v >= g->n || g->h_vtype[v] != vtypes[0])
+        fail(SMCM_EINVAL, "source row " + std::to_string(v) + " is not a vertex of the start type");
     }
-    int status = SMCM_OK;
-    std::string msg;
-    bool finished = true;
-    try {
-      finished = run_query(s, w, *q, small_lane && !q->large);
-    } catch (const ApiError& e) {
-      status = e.code;
-      msg = e.msg;
-    } catch (const HipError& e) {
-      status = e.code == (int)hipErrorOutOfMemory ? SMCM_ENOMEM : SMCM_EDEVICE;
-      msg = e.what();
-      (void)hipGetLastError();
-    } catch (const std::exception& e) {
-      status = SMCM_EDEVICE;
-      msg = e.what();
-    }
-    if (!finished) {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->running--;
-      continue;
-    }
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      q->out.status = status;
-      if (status != SMCM_OK) {
-        std::strncpy(q->out.error, msg.c_str(), sizeof(q->out.error) - 1);
-        q->out.error[sizeof(q->out.error) - 1] = 0;
-      }
-      if (q->out.t_done == 0) q->out.t_done = mono_s() - s->t0;
-      if (q->out.t_start == 0) q->out.t_start = q->out.t_done;
-      q->done = true;
-      s->running--;
-      s->completed++;
-      if (status != SMCM_OK && std::getenv("SMCM_TRACE")) fprintf(stderr, "[smcm] query failed: %s\n", msg.c_str());
-    }
-    s->cv_done.notify_all();
   }
 }
 
@@ -479,6 +438,12 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
     if (o.plan_cost != 0 && o.plan_cost != 1) fail(SMCM_EINVAL, "plan_cost must be 0 or 1");
     if (o.block_bytes < (1ull << 20)) fail(SMCM_EINVAL, "block_bytes too small");
     s->opts = o;
+    // the workers allocate on their own streams: blocks the main context keeps cached for reuse
+    // (it stays idle while the server runs) go back to the device pool, or they would be out of
+    // the workers' reach
+    ctx->al.release_cache();
+    SMCM_HIP(hipStreamSynchronize(ctx->stream));
+    smcm::DevAlloc::trim_pool();
     s->cache.cap = o.cache_bytes;
     s->cache.entry_max = o.cache_entry_max;
     // every typed matrix of the graphs, built now so workers only read the cache
@@ -515,11 +480,10 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
         w->opts.window_cols = 16384;
       }
     }
-    // lanes only when both kinds of workers exist
-    s->small_lanes = (o.large_slots > 0 && o.large_slots < o.slots) ? o.slots - o.large_slots : 0;
-    s->t0 = mono_s();
+    // lanes only when both kinds of workers exist (the scheduler's small_lanes)
+    s->sched = std::make_unique<smcm_sched::Scheduler>(o.slots, o.large_slots, o.budget_bytes);
     smcm_server* raw = s.get();
-    for (int32_t i = 0; i < o.slots; ++i) raw->workers.emplace_back(worker_loop, raw, i);
+    s->sched->start([raw](smcm_sched::Job& j, int wi, bool small) { return run_job(raw, j, wi, small); });
     ctx->live++;
     *out = s.release();
   });
@@ -528,12 +492,7 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
 int smcm_server_destroy(smcm_server* s) {
   return guarded([&] {
     if (!s) return;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->stop = true;
-    }
-    s->cv_work.notify_all();
-    for (auto& t : s->workers) t.join();
+    s->sched->stop();  // every submitted query finishes first
     s->cache.clear();
     for (smcm_ctx* w : s->wctx) smcm_ctx_destroy(w);
     s->main->live--;
@@ -544,48 +503,34 @@ int smcm_server_destroy(smcm_server* s) {
 int smcm_server_submit(smcm_server* s, int32_t graph, const int32_t* vtypes, const int32_t* etypes, int32_t L,
                        const int32_t* src_rows, int64_t n_src, uint64_t* row_digest, int64_t* ticket) {
   return guarded([&] {
-    if (!s || !ticket || !vtypes || !etypes) fail(SMCM_EINVAL, "null argument");
-    if (graph < 0 || graph >= (int32_t)s->graphs.size()) fail(SMCM_EINVAL, "graph index out of range");
-    if (L < 1 || L > 16) fail(SMCM_EINVAL, "the meta-path is incorrect: L must be in [1, 16]");
-    smcm_graph* g = s->graphs[graph];
-    for (int32_t q = 0; q <= L; ++q)
-      if (vtypes[q] < 0 || vtypes[q] >= g->ntypes) fail(SMCM_EINVAL, "the meta-path is incorrect: vertex type");
+    if (!s || !ticket) fail(SMCM_EINVAL, "null argument");
+    check_query(s, graph, vtypes, etypes, L, src_rows, n_src);
     auto q = std::make_unique<Query>();
     q->graph = graph;
     q->L = L;
     q->vts.assign(vtypes, vtypes + L + 1);
     q->ets.assign(etypes, etypes + L);
     q->whole = src_rows == nullptr;
-    if (!q->whole) {
-      if (n_src < 0) fail(SMCM_EINVAL, "bad source row count");
-      q->src.assign(src_rows, src_rows + n_src);
-      for (int32_t v : q->src)
-        if (v < 0 || v >= g->n || g->h_vtype[v] != vtypes[0])
-          fail(SMCM_EINVAL, "source row " + std::to_string(v) + " is not a vertex of the start type");
-    }
+    if (!q->whole) q->src.assign(src_rows, src_rows + n_src);
     q->digest = row_digest;
-    std::lock_guard<std::mutex> lk(s->mu);
-    q->ticket = s->next_ticket++;
-    q->out.t_submit = mono_s() - s->t0;
-    *ticket = q->ticket;
-    s->pending.push_back(q.get());
-    s->all.emplace(q->ticket, std::move(q));
-    s->cv_work.notify_one();
+    *ticket = s->sched->submit(std::move(q));
   });
 }
 
 int smcm_server_poll(smcm_server* s, int64_t ticket, int32_t wait, smcm_query_out* out) {
   if (!s || !out) return set_err(SMCM_EINVAL, "null argument");
-  std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->all.find(ticket);
-  if (it == s->all.end()) return set_err(SMCM_EINVAL, "unknown ticket");
-  Query* q = it->second.get();
-  if (!q->done) {
-    if (!wait) return SMCM_EAGAIN;
-    s->cv_done.wait(lk, [&] { return q->done; });
-  }
-  *out = q->out;
-  s->all.erase(it);
+  std::unique_ptr<smcm_sched::Job> j;
+  const int rc = s->sched->poll(ticket, wait != 0, &j);
+  if (rc < 0) return set_err(SMCM_EINVAL, "unknown ticket");
+  if (rc > 0) return SMCM_EAGAIN;
+  const Query& q = static_cast<const Query&>(*j);
+  *out = q.out;
+  out->status = q.status;
+  out->t_submit = q.t_submit;
+  out->t_start = q.t_start;
+  out->t_done = q.t_done;
+  std::memset(out->error, 0, sizeof(out->error));
+  if (q.status != SMCM_OK) std::strncpy(out->error, q.error.c_str(), sizeof(out->error) - 1);
   return SMCM_OK;
 }
 
@@ -605,34 +550,66 @@ int smcm_server_replay(smcm_server* s, int32_t n, const int32_t* graph, const in
     if (!s || n < 0 || (n > 0 && (!graph || !path_len || !vtypes || !etypes || !arrival || !out)))
       fail(SMCM_EINVAL, "bad arguments");
     if (src_rows && !src_off) fail(SMCM_EINVAL, "src_rows without src_off");
-    std::vector<int64_t> tickets(n, 0);
+    // every query is checked before the first is submitted: a submitted query writes into
+    // row_digest until it is polled, so the call must not return while one still runs
+    {
+      int64_t vo = 0, eo = 0;
+      for (int32_t i = 0; i < n; ++i) {
+        const int32_t L = path_len[i];
+        if (L < 1 || L > 16) fail(SMCM_EINVAL, "query " + std::to_string(i) + ": the meta-path is incorrect");
+        if (src_rows && src_off[i + 1] < src_off[i]) fail(SMCM_EINVAL, "src_off must not decrease");
+        try {
+          check_query(s, graph[i], vtypes + vo, etypes + eo, L, src_rows ? src_rows + src_off[i] : nullptr,
+                      src_rows ? src_off[i + 1] - src_off[i] : 0);
+        } catch (const ApiError& e) {
+          fail(e.code, "query " + std::to_string(i) + ": " + e.msg);
+        }
+        vo += L + 1;
+        eo += L;
+      }
+    }
+    std::vector<int64_t> tickets;
+    tickets.reserve(n);
     int64_t vo = 0, eo = 0, dig = 0;
     const double base = mono_s();
-    for (int32_t i = 0; i < n; ++i) {
+    const double shift = base - s->sched->t0();  // scheduler times -> times relative to this call
+    int rc = SMCM_OK;
+    std::string err;
+    for (int32_t i = 0; i < n && rc == SMCM_OK; ++i) {
       const double due = base + arrival[i];
       for (double now = mono_s(); now < due; now = mono_s())
         std::this_thread::sleep_for(std::chrono::duration<double>(std::min(due - now, 0.002)));
       const int32_t L = path_len[i];
-      if (graph[i] < 0 || graph[i] >= (int32_t)s->graphs.size()) fail(SMCM_EINVAL, "graph index out of range");
       const int32_t t0 = vtypes[vo];
-      if (t0 < 0 || t0 >= s->graphs[graph[i]]->ntypes) fail(SMCM_EINVAL, "the meta-path is incorrect");
       const int32_t* src = src_rows ? src_rows + src_off[i] : nullptr;
       const int64_t ns = src_rows ? src_off[i + 1] - src_off[i] : 0;
       uint64_t* d = row_digest ? row_digest + dig : nullptr;
-      const int rc = smcm_server_submit(s, graph[i], vtypes + vo, etypes + eo, L, src, ns, d, &tickets[i]);
-      if (rc != SMCM_OK) fail(rc, smcm_last_error());
+      int64_t t = 0;
+      rc = smcm_server_submit(s, graph[i], vtypes + vo, etypes + eo, L, src, ns, d, &t);
+      if (rc != SMCM_OK) {
+        err = smcm_last_error();
+        break;
+      }
+      tickets.push_back(t);
       vo += L + 1;
       eo += L;
       dig += src_rows ? ns : s->graphs[graph[i]]->dims[t0];
     }
-    for (int32_t i = 0; i < n; ++i) {
-      const int rc = smcm_server_poll(s, tickets[i], 1, &out[i]);
-      if (rc != SMCM_OK) fail(rc, smcm_last_error());
-      // times relative to the replay's start
-      out[i].t_submit -= base - s->t0;
-      out[i].t_start -= base - s->t0;
-      out[i].t_done -= base - s->t0;
+    // wait for every submitted query, also after a failed submission (they write row_digest)
+    for (size_t i = 0; i < tickets.size(); ++i) {
+      const int prc = smcm_server_poll(s, tickets[i], 1, &out[i]);
+      if (prc != SMCM_OK) {
+        if (rc == SMCM_OK) {
+          rc = prc;
+          err = smcm_last_error();
+        }
+        continue;
+      }
+      out[i].t_submit -= shift;
+      out[i].t_start -= shift;
+      out[i].t_done -= shift;
     }
+    if (rc != SMCM_OK) fail(rc, err);
   });
 }
 

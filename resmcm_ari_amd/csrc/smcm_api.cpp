@@ -573,6 +573,29 @@ int smcm_result_free(smcm_result* r) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+// Host CSR operand of smcm_spgemm_csr: ptr[0] == 0, non-decreasing row pointers, a column array
+// whenever there are entries, columns in [0, ncols) and, for a right operand, ascending rows.
+void check_host_csr(const char* name, int32_t rows, int32_t ncols, const int64_t* ptr, const int32_t* col,
+                    bool sorted) {
+  using smcm_host::fail;
+  const std::string nm(name);
+  if (ptr[0] != 0) fail(SMCM_EINVAL, nm + ": row pointer 0 must be 0");
+  for (int32_t r = 0; r < rows; ++r)
+    if (ptr[r + 1] < ptr[r]) fail(SMCM_EINVAL, nm + ": row pointers decrease at row " + std::to_string(r));
+  if (ptr[rows] > 0 && !col) fail(SMCM_EINVAL, nm + ": null column array");
+  for (int32_t r = 0; r < rows; ++r)
+    for (int64_t x = ptr[r]; x < ptr[r + 1]; ++x) {
+      if (col[x] < 0 || col[x] >= ncols) fail(SMCM_EINVAL, nm + ": column out of range");
+      if (sorted && x > ptr[r] && col[x] < col[x - 1]) fail(SMCM_EINVAL, nm + ": rows must be sorted");
+    }
+}
+}  // namespace
+
+extern "C" {
+
 int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, const int32_t* a_col,
                     const uint64_t* a_val, const int64_t* b_ptr, const int32_t* b_col, const uint64_t* b_val,
                     smcm_result** out) {
@@ -581,14 +604,10 @@ int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_
     if (!out || !a_ptr || !b_ptr) fail(SMCM_EINVAL, "null argument");
     *out = nullptr;
     if (m < 0 || k < 0 || n < 0) fail(SMCM_EINVAL, "negative dimension");
+    // every check runs before the first upload, so a malformed operand launches no kernel
+    check_host_csr("A", m, k, a_ptr, a_col, false);
+    check_host_csr("B", k, n, b_ptr, b_col, true);
     const int64_t annz = a_ptr[m], bnnz = b_ptr[k];
-    for (int64_t x = 0; x < annz; ++x)
-      if (a_col[x] < 0 || a_col[x] >= k) fail(SMCM_EINVAL, "A column out of range");
-    for (int32_t r = 0; r < k; ++r)
-      for (int64_t x = b_ptr[r]; x < b_ptr[r + 1]; ++x) {
-        if (b_col[x] < 0 || b_col[x] >= n) fail(SMCM_EINVAL, "B column out of range");
-        if (x > b_ptr[r] && b_col[x] < b_col[x - 1]) fail(SMCM_EINVAL, "B rows must be sorted");
-      }
     DevAlloc& al = ctx->al;
     auto up = [&](const void* h, size_t bytes) {
       void* d = al.alloc(bytes);

@@ -1842,20 +1842,28 @@ void dispatch(const Csr& A, const Csr& B, const int64_t* S, const int32_t* bins,
                                         tm);
 }
 
-template <bool NUMERIC, bool AV, bool BV>
-void set_lds_attr(size_t bytes) {
-  (void)hipFuncSetAttribute((const void*)bitmap_tier_kernel<NUMERIC, AV, BV>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if constexpr (NUMERIC) {
-    (void)hipFuncSetAttribute((const void*)window_tier_kernel<AV, BV>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
-    (void)hipFuncSetAttribute((const void*)window_major_kernel<AV, BV, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipFuncSetAttribute((const void*)window_major_kernel<AV, BV, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+// raises the dynamic-LDS limit of one kernel; a failure names the kernel and the size
+void lds_attr(const void* kernel, const char* name, size_t bytes) {
+  const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw HipError((int)e, std::string("hipFuncSetAttribute(") + name + ", " + std::to_string(bytes) +
+                               " bytes of LDS): " + hipGetErrorString(e));
   }
 }
 
+template <bool NUMERIC, bool AV, bool BV>
+void set_lds_attr(size_t bytes) {
+  lds_attr((const void*)bitmap_tier_kernel<NUMERIC, AV, BV>, "bitmap_tier_kernel", bytes);
+  if constexpr (NUMERIC) {
+    lds_attr((const void*)window_tier_kernel<AV, BV>, "window_tier_kernel", bytes);
+    lds_attr((const void*)window_major_kernel<AV, BV, false>, "window_major_kernel", bytes);
+    lds_attr((const void*)window_major_kernel<AV, BV, true>, "window_major_kernel", bytes);
+  }
+}
+
+// every tiered kernel's limit raised to `bytes`; recorded as done only once all calls succeeded,
+// so a failed call is retried by the next product
 void set_all_lds_attrs(size_t bytes) {
   static std::mutex mu;  // server threads run products concurrently
   static size_t done = 0;
@@ -1869,7 +1877,6 @@ void set_all_lds_attrs(size_t bytes) {
   set_lds_attr<true, true, false>(bytes);
   set_lds_attr<true, false, true>(bytes);
   set_lds_attr<true, true, true>(bytes);
-  (void)hipGetLastError();
   done = bytes;
 }
 

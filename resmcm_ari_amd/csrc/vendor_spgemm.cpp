@@ -14,7 +14,8 @@
 
 extern "C" {
 
-// status 0 ok; negative: a HIP or rocSPARSE step failed (message in err, cap bytes)
+// status 0 ok; -22 invalid operand (checked before any device work); -5 a HIP or rocSPARSE step
+// failed (message in err, cap bytes)
 int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, const int32_t* a_col,
                        const double* a_val, const int64_t* b_ptr, const int32_t* b_col, const double* b_val,
                        int32_t reps, double* ms_per_product, int64_t* c_nnz, int64_t* c_ptr_out,
@@ -25,6 +26,7 @@ namespace {
 
 struct Fail {
   std::string msg;
+  int code = -5;  // -22: invalid operand (nothing reached the device)
 };
 
 #define V_HIP(x)                                                         \
@@ -37,6 +39,20 @@ struct Fail {
     rocsparse_status s_ = (x);                                           \
     if (s_ != rocsparse_status_success) throw Fail{std::string(#x) + ": status " + std::to_string((int)s_)}; \
   } while (0)
+
+// ptr[0] == 0, non-decreasing row pointers, columns present when there are entries and in
+// [0, ncols): checked on the host before anything reaches the device
+void check_operand(const char* name, int32_t rows, int32_t ncols, const int64_t* ptr, const int32_t* col,
+                   const double* val) {
+  const std::string nm(name);
+  if (!ptr) throw Fail{nm + ": null row pointers", -22};
+  if (ptr[0] != 0) throw Fail{nm + ": row pointer 0 must be 0", -22};
+  for (int32_t r = 0; r < rows; ++r)
+    if (ptr[r + 1] < ptr[r]) throw Fail{nm + ": row pointers decrease at row " + std::to_string(r), -22};
+  if (ptr[rows] > 0 && (!col || !val)) throw Fail{nm + ": null column or value array", -22};
+  for (int64_t x = 0; x < ptr[rows]; ++x)
+    if (col[x] < 0 || col[x] >= ncols) throw Fail{nm + ": column out of range", -22};
+}
 
 template <class T>
 T* dev_copy(const T* h, size_t n) {
@@ -59,7 +75,9 @@ int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, co
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
   try {
-    if (m < 0 || k < 0 || n < 0 || reps < 1) throw Fail{"bad arguments"};
+    if (m < 0 || k < 0 || n < 0 || reps < 1) throw Fail{"bad arguments", -22};
+    check_operand("A", m, k, a_ptr, a_col, a_val);
+    check_operand("B", k, n, b_ptr, b_col, b_val);
     const int64_t annz = a_ptr[m], bnnz = b_ptr[k];
     auto keep = [&](void* p) {
       dev.push_back(p);
@@ -134,7 +152,7 @@ int smcm_vendor_spgemm(int32_t m, int32_t k, int32_t n, const int64_t* a_ptr, co
       std::strncpy(err, f.msg.c_str(), (size_t)cap - 1);
       err[cap - 1] = 0;
     }
-    rc = -5;
+    rc = f.code;
   }
   if (A) rocsparse_destroy_spmat_descr(A);
   if (B) rocsparse_destroy_spmat_descr(B);

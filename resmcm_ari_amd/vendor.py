@@ -45,7 +45,8 @@ def spgemm_f64(m, k, n, a_ptr, a_col, a_val, b_ptr, b_col, b_val, reps: int = 3,
                                   bc.ctypes.data, bv.ctypes.data, reps, C.byref(ms), C.byref(nnz),
                                   None, None, None, err, 256)
     if rc != 0:
-        raise RuntimeError("rocSPARSE SpGEMM failed: " + err.value.decode())
+        # -22: an invalid operand, rejected on the host before any device work
+        raise (ValueError if rc == -22 else RuntimeError)("rocSPARSE SpGEMM failed: " + err.value.decode())
     if not want_result:
         return ms.value, nnz.value, None
     cc = np.zeros(max(nnz.value, 1), np.int32)

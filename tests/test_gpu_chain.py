@@ -219,3 +219,93 @@ def test_cli_cost_providers(tmp_path, prov):
     got = [line.split() for line in out.read_text().splitlines()]
     want = TG.ref_lines(name, "resmcm")
     assert [(g[0], g[1]) for g in got] == [(w[0], w[1]) for w in want]
+
+
+@pytest.mark.parametrize("plan_cost", [0, 1])
+def test_products_count_is_the_oracles(ctx, cfg1, plan_cost):
+    """The executed product count (last_stats()['products'], the benchmark's numerator) equals
+    the oracle's count for the executed plan: sum over its SpGEMMs of sum_{(u,k) in A} |row_k(B)|,
+    on every fixture meta-path and on config 1; with plan_cost 0 the plan is also the oracle DP's."""
+    import test_oracle_golden as TG
+    from resmcm_ari_amd.hin import MetaPath
+
+    cases = []
+    for name in ["edge", "bib_tiny", "bib_tiny2", "loc_tiny"]:
+        z, hg = TG.load(name)
+        cases.append((hg, smcm.Graph(ctx, hg), [MetaPath(str(p)) for p in z["paths"]]))
+    hg1, G1 = cfg1
+    cases.append((hg1, G1, [configs.metapath_from_letters(p, "bib")
+                            for p in ["A-P-V-P-A", "A-P-T-P-A", "P-T-P-A-P-V", "T-P-A-P-T"]]))
+    ctx.set_option("plan_cost", plan_cost)
+    try:
+        checked = 0
+        for hg, G, paths in cases:
+            for mp in paths:
+                if mp.pathLen < 2:
+                    continue
+                r, _ = G.run(mp)
+                st = ctx.last_stats()
+                r.free()
+                pairs = np.asarray(st["pairs"], np.int32).reshape(-1, 2)
+                mats = O.chain_matrices(hg, mp)
+                if plan_cost == 0:
+                    c, _ = O.estimate(mats, O.coldims(hg, mp))
+                    np.testing.assert_array_equal(pairs, O.dp(c)[3], err_msg=mp.toString())
+                assert st["products"] == O.plan_products(mats, pairs), mp.toString()
+                checked += 1
+        assert checked > 20
+    finally:
+        ctx.set_option("plan_cost", 0)
+
+
+def test_res_program_matches_reference_lines(tmp_path):
+    """The estimate-only drop-in (smcm_res, accuracy/res.cpp:612-709) writes the reference
+    program's lines exactly: meta-path string and estimate with 6 significant digits (:697),
+    against the lines the reference's own res program wrote at one thread on the same inputs."""
+    import os
+    import subprocess
+
+    from resmcm_ari_amd import _build
+    import test_oracle_golden as TG
+
+    for name in TG.GRAPHS:
+        z, hg = TG.load(name)
+        files = [str(tmp_path / f"{name}_{x}.txt") for x in ("graph", "vertex", "edge")]
+        hg.write_text(*files)
+        mpf = tmp_path / f"{name}_mp.txt"
+        mpf.write_text("".join(str(p) + "\n" for p in z["paths"]))
+        out = tmp_path / f"{name}_res.txt"
+        subprocess.run([os.path.join(_build.LIBDIR, "smcm_res"), "8", "1", *files, str(mpf), str(out)],
+                       check=True, stdout=subprocess.DEVNULL)
+        got = [line.split() for line in out.read_text().splitlines()]
+        assert got == TG.ref_lines(name, "res"), name
+
+
+def test_estimate_is_the_oracles(ctx, tiny_graphs, cfg1):
+    """smcm_estimate (accuracy mode, c_[0][L-1] of accuracy/res.cpp:494-523) equals the oracle's
+    one-thread row-order sum bit for bit under exact_cost_sum, within C_RTOL otherwise; also on
+    source subsets."""
+    hg1, G1 = cfg1
+    cases = [(tiny_graphs["bib"], "bib", ["A-P-V-P-A", "P-T-P-A-P-V", "T-P-A"]),
+             (tiny_graphs["loc"], "loc", ["U-L-C-L-Y-L-U-L-S-L-U", "L-U-L"]),
+             ((hg1, G1), "bib", ["A-P-V-P-A", "A-P-T-P-A", "V-P-T-P-V"])]
+    rng = np.random.default_rng(11)
+    for exact in (1, 0):
+        ctx.set_option("exact_cost_sum", exact)
+        try:
+            for (hg, G), schema, paths in cases:
+                for p in paths:
+                    mp = configs.metapath_from_letters(p, schema)
+                    t0 = mp.vertexTypes[0]
+                    for src in (None, np.sort(rng.choice(hg.bins[t0], size=max(1, len(hg.bins[t0]) // 3),
+                                                         replace=False))):
+                        local = None if src is None else hg.local_ids()[src]
+                        mats = O.chain_matrices(hg, mp, local)
+                        c, _ = O.estimate(mats, O.coldims(hg, mp))
+                        got = G.estimate(mp, src_rows=src)
+                        if exact:
+                            assert got == c[0, -1], (p, got, c[0, -1])
+                        else:
+                            np.testing.assert_allclose(got, c[0, -1], rtol=C_RTOL, atol=0, err_msg=p)
+        finally:
+            ctx.set_option("exact_cost_sum", 0)

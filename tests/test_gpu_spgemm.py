@@ -258,3 +258,40 @@ def test_window_major_wraparound_later_window(ctx):
         assert list(col) == [5, 70, 200, 70]
         assert list(val) == [1, 0, 3, 7]
         assert_same_csr(ptr, col, val, ref, what=f"later-window wrap mode={mode}")
+
+
+@pytest.mark.parametrize("bad", ["a_ptr0", "a_decreasing", "a_null_col", "b_decreasing", "b_unsorted", "b_range"])
+def test_spgemm_csr_rejects_malformed_operands(ctx, bad):
+    """Malformed host operands are rejected with EINVAL before any upload or launch: a first row
+    pointer above 0, decreasing row pointers, a missing column array, unsorted or out-of-range
+    right-operand columns."""
+    import resmcm_ari_amd.smcm as S
+
+    ap = np.array([0, 2, 3], np.int64)
+    ac = np.array([0, 1, 1], np.int32)
+    bp = np.array([0, 1, 3], np.int64)
+    bc = np.array([2, 0, 3], np.int32)
+    if bad == "a_ptr0":
+        ap = np.array([1, 2, 3], np.int64)
+    elif bad == "a_decreasing":
+        ap = np.array([0, 3, 2], np.int64)
+    elif bad == "b_decreasing":
+        bp = np.array([0, 3, 1], np.int64)
+    elif bad == "b_unsorted":
+        bc = np.array([2, 3, 0], np.int32)
+    elif bad == "b_range":
+        bc = np.array([2, 0, 4], np.int32)
+    if bad == "a_null_col":
+        h = S.C.c_void_p()
+        rc = S.load().smcm_spgemm_csr(ctx._h, 2, 2, 4, ap.ctypes.data, None, None, bp.ctypes.data, bc.ctypes.data,
+                                      None, S.C.byref(h))
+        assert rc == S.SMCM_EINVAL
+        return
+    with pytest.raises(S.SmcmError) as e:
+        ctx.spgemm_csr(2, 2, 4, ap, ac, None, bp, bc, None)
+    assert e.value.code == S.SMCM_EINVAL
+    # the context still works afterwards
+    r = ctx.spgemm_csr(2, 2, 4, np.array([0, 2, 3], np.int64), np.array([0, 1, 1], np.int32), None,
+                       np.array([0, 1, 3], np.int64), np.array([2, 0, 3], np.int32), None)
+    assert r.nnz == 5  # rows {0,2,3} and {0,3}
+    r.free()

@@ -197,3 +197,39 @@ def test_estimator_exact_on_functional_relations():
         for _ in range(deg):
             rho *= (1 - 1.0 / nV)
         assert R[0][u, 1] == 1 - rho
+
+
+@pytest.mark.parametrize("name", ["bib_tiny", "loc_tiny", "edge"])
+def test_plan_digest_equals_full_execution(name):
+    """orc_plan_digest (row blocks of M_0, the root reduced to digests without materialising it)
+    equals the digests of the fully executed plan, for the DP plan and left to right, at block
+    sizes 1, 7 and all rows; its product count equals the count over a full execution, which is
+    checked against a brute-force count from dense operand structures."""
+    import test_oracle_golden as TG
+    from resmcm_ari_amd.hin import MetaPath
+
+    z, hg = TG.load(name)
+    for p in z["paths"]:
+        mp = MetaPath(str(p))
+        if mp.pathLen < 2:
+            continue
+        mats = O.chain_matrices(hg, mp)
+        c, _ = O.estimate(mats, O.coldims(hg, mp))
+        for plan in (O.dp(c)[3], O.left_to_right(mp.pathLen)):
+            full = O.execute_plan(mats, plan)
+            want = O.row_digest(full)
+            count = O.plan_products(mats, plan)
+            # brute force: products of a step = sum over A's entries of B's row lengths; a base
+            # matrix's entries are counted with their duplicates, a product's are distinct
+            mult = [m.to_dense().astype(np.int64) for m in mats]  # values 1: entry multiplicities
+            stack, brute = [], 0
+            for k, l in plan:
+                A = mult[k] if k >= 0 else stack.pop(-2 if l < 0 else -1)
+                B = mult[l] if l >= 0 else stack.pop()
+                brute += int((A @ B.sum(axis=1)).sum())
+                stack.append(((A @ B) != 0).astype(np.int64))
+            assert count == brute, p
+            for block in (1, 7, mats[0].nrows + 1):
+                d, nnz, prods = O.plan_digest(mats, plan, block)
+                np.testing.assert_array_equal(d, want, err_msg=f"{p} block {block}")
+                assert nnz == full.nnz and prods == count, p
