@@ -144,6 +144,18 @@ int smcm_chain_products_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vt
 /* Estimated nnz of the whole chain, c_[0][L-1] (accuracy/res.cpp:494-523, 564-600). */
 int smcm_estimate(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
                   int32_t L, const int32_t* src_rows, int64_t n_src, double* est_nnz);
+/* Allocation-free variant of smcm_estimate (no source subset).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Writes c_[0][L-1] to the
+ * device double d_est_nnz; the work is queued on the context's stream and the call returns
+ * without waiting (smcm_ctx_sync, or the caller's own stream order, sees the result).  It
+ * allocates and frees nothing: the estimator's scratch comes from one 64 MiB device array per
+ * GPU, so chains whose row densities and pass temporaries exceed it are rejected (SMCM_EINVAL)
+ * before any kernel is queued for the excess; the meta-path's typed matrices must already be on
+ * the device (any earlier smcm_estimate / smcm_chain_run over it on this graph builds them).
+ * Calls that overlap on one GPU (from other contexts or threads) race on the shared array. */
+int smcm_estimate_noalloc(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes, const int32_t* etypes,
+                          int32_t L, double* d_est_nnz);
 /* The reference's other DP cost providers over the whole chain (no source subset), selected by
  * `provider`: 2 MetaAC (efficient/meta.cpp:448-566), 3 MNC sketches (efficient/mnc.cpp:313-800),
  * 4 DMap density maps (efficient/dm.cpp:52-620), 5 LGraph rank propagation

@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "scan.hip",
     "batch.hip",
     "providers.hip",
+    "noalloc.hip",
     "smcm_api.cpp",
     "batch_api.cpp",
     "server.cpp",

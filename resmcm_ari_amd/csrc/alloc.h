@@ -11,6 +11,7 @@
 
 #include <map>
 #include <unordered_map>
+#include <vector>
 
 #include "common.h"
 
@@ -19,6 +20,14 @@ namespace smcm {
 struct DevAlloc {
   static constexpr size_t kCacheMin = (size_t)64 << 20;
   hipStream_t stream = nullptr;
+  // Arena mode (arena != nullptr): every block is carved from the caller's device region
+  // [arena, arena + arena_cap) and nothing is ever allocated or freed on the device.  Blocks are
+  // handed out in order; freeing the most recent block returns its bytes (scoped buffers are
+  // freed in reverse order), other frees keep their bytes until the allocator is dropped.  An
+  // allocation beyond arena_cap throws (EINVAL) before any kernel could use the block.
+  unsigned char* arena = nullptr;
+  size_t arena_cap = 0, arena_top = 0, arena_high = 0;
+  std::vector<size_t> arena_stack;
   size_t live_bytes = 0;
   size_t peak_bytes = 0;
   size_t cached_bytes = 0;
@@ -29,6 +38,16 @@ struct DevAlloc {
   void* alloc(size_t bytes) {
     bytes = (bytes + 255) / 256 * 256;
     if (bytes == 0) bytes = 256;
+    if (arena) {
+      if (bytes > arena_cap - arena_top)
+        throw HipError(-22, "scratch exhausted: " + std::to_string(arena_top + bytes) + " bytes needed, " +
+                                std::to_string(arena_cap) + " available");
+      arena_stack.push_back(arena_top);
+      void* q = arena + arena_top;
+      arena_top += bytes;
+      if (arena_top > arena_high) arena_high = arena_top;
+      return q;
+    }
     void* p = nullptr;
     size_t got = bytes;
     if (bytes >= kCacheMin) {
@@ -67,6 +86,13 @@ struct DevAlloc {
   }
   void free(void* p) {
     if (!p) return;
+    if (arena) {
+      if (!arena_stack.empty() && (unsigned char*)p == arena + arena_stack.back()) {
+        arena_top = arena_stack.back();
+        arena_stack.pop_back();
+      }
+      return;
+    }
     auto it = sizes.find(p);
     const size_t sz = it != sizes.end() ? it->second : 0;
     if (it != sizes.end()) sizes.erase(it);

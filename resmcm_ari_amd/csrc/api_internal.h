@@ -65,6 +65,10 @@ inline int guarded(F&& f) {
   }
 }
 
+// device address of smcm_estimate_noalloc's scratch array on `device` (the current device),
+// looked up with hipGetSymbolAddress on first use per device (noalloc.hip)
+void* noalloc_scratch(int device);
+
 inline double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }

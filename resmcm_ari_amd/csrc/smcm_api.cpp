@@ -76,6 +76,8 @@ int smcm_ctx_create(int device, uint64_t hbm_budget, smcm_ctx** out) {
     size_t freeb = 0, totalb = 0;
     SMCM_HIP(hipMemGetInfo(&freeb, &totalb));
     ctx->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)freeb);
+    // the allocation-free entry point's scratch on this device: its address, looked up once
+    (void)noalloc_scratch(device);
     *out = ctx.release();
   });
 }

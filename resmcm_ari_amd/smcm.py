@@ -28,7 +28,8 @@ LIB_PATH = os.path.join(_build.LIBDIR, "libsmcm.so")
 EXPORTS = [
     "smcm_ctx_create", "smcm_ctx_destroy", "smcm_set_option", "smcm_get_option", "smcm_last_error",
     "smcm_last_stats", "smcm_ctx_sync", "smcm_graph_upload", "smcm_graph_free", "smcm_graph_dims",
-    "smcm_chain_run", "smcm_chain_run_plan", "smcm_chain_plan", "smcm_estimate", "smcm_estimator_rows",
+    "smcm_chain_run", "smcm_chain_run_plan", "smcm_chain_plan", "smcm_estimate", "smcm_estimate_noalloc",
+    "smcm_estimator_rows",
     "smcm_result_nrows", "smcm_result_nnz", "smcm_result_copy", "smcm_result_copy_rows", "smcm_result_row_ids",
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
     "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth", "smcm_chain_products_estimate",
@@ -105,6 +106,7 @@ def load() -> C.CDLL:
             "smcm_chain_run_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp],
             "smcm_chain_plan": [vp, vp, vp, vp, i32, vp, i64, vp, vp, vp, vp],
             "smcm_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
+            "smcm_estimate_noalloc": [vp, vp, vp, vp, i32, vp],
             "smcm_cost_estimate": [vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp],
             "smcm_chain_products_estimate": [vp, vp, vp, vp, i32, vp, i64, vp],
             "smcm_batch_run": [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
@@ -307,6 +309,13 @@ class Graph:
         x = C.c_double()
         _check(load().smcm_estimate(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen, _ptr(s), ns, C.byref(x)))
         return float(x.value)
+
+    def estimate_noalloc(self, mp: MetaPath, d_out: int) -> None:
+        """Allocation-free estimate: c_[0][L-1] written to the device double at address d_out
+        (queued on the context's stream; sync the context before reading it)."""
+        vt, et = self._path(mp)
+        _check(load().smcm_estimate_noalloc(self.ctx._h, self._h, _ptr(vt), _ptr(et), mp.pathLen,
+                                            C.c_void_p(int(d_out))))
 
     def cost_estimate(self, mp: MetaPath, provider, accuracy: bool = False):
         """One of the reference's other DP cost providers ("metaac", "mnc", "dmap", "lgraph" or
