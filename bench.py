@@ -239,8 +239,10 @@ def bench_vendor(hg1, hg2=None) -> dict:
     return out
 
 
-def bench_batched(hg1, n_chains: int = 4096) -> dict:
-    """Workload 4: the batched form against the per-chain path on the same chains."""
+def bench_batched(hg1, n_chains: int = 4096, ref_sample: int = 1024) -> dict:
+    """Workload 4: the batched form against the per-chain path on the same chains: with the
+    product-aware plans the batch uses (all chains, every result's digests compared) and with the
+    reference DP's plans (the drop-in; a sample of the first chains, it is slower)."""
     ctx = smcm.Context(0)
     G = smcm.Graph(ctx, hg1)
     batch = configs.cfg4_batch(hg1.dimensions, n_chains)
@@ -255,52 +257,81 @@ def bench_batched(hg1, n_chains: int = 4096) -> dict:
     nnz, dig, _, st = G.run_batch(paths, srcs, digests=True)
     tb = time.perf_counter() - t0
     print(f"[bench] batched: {n_chains} chains in {tb:.2f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
+    per_chain = {}
     bad = 0
-    for i, mp in enumerate(paths):
-        r, _ = G.run(mp, srcs[i])
-        d = r.row_digest()
-        if r.nnz != nnz[i] or not np.array_equal(d, dig[i]):
-            bad += 1
-        r.free()
-        if i % 1024 == 1023:
-            print(f"[bench] per-chain path: {i + 1} chains, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-                  flush=True)
-    tp = time.perf_counter() - t0
+    for pc, count in ((1, n_chains), (0, min(ref_sample, n_chains))):
+        ctx.set_option("plan_cost", pc)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for i in range(count):
+            r, _ = G.run(paths[i], srcs[i])
+            d = r.row_digest()
+            if r.nnz != nnz[i] or not np.array_equal(d, dig[i]):
+                bad += 1
+            r.free()
+            if i % 1024 == 1023:
+                print(f"[bench] per-chain path, plan_cost={pc}: {i + 1} chains, {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
+        per_chain[pc] = (count, time.perf_counter() - t0)
+    ctx.set_option("plan_cost", 0)
     G.close()
     ctx.close()
+    (n1, t1), (n0, t0s) = per_chain[1], per_chain[0]
+    qps, qps1, qps0 = n_chains / tb, n1 / t1, n0 / t0s
     return {"workload": f"{n_chains} chains, L 2-6 over A/P/V/T, sources log-uniform 16..65536 "
                         f"(seed 0x5EED0004), graph A=20k P=30k V=200 T=20k",
-            "queries_per_s": round(n_chains / tb, 2), "plain_queries_per_s": round(n_chains / tp, 2),
-            "speedup": round(tp / tb, 2), "seconds": round(tb, 3), "plain_seconds": round(tp, 3),
+            "queries_per_s": round(qps, 2), "seconds": round(tb, 3),
+            "product_plan_queries_per_s": round(qps1, 2), "product_plan_seconds": round(t1, 3),
+            "speedup_vs_product_plan": round(qps / qps1, 2),
+            "plain_queries_per_s": round(qps0, 2), "plain_sample_chains": n0, "plain_sample_seconds": round(t0s, 3),
+            "speedup": round(qps / qps0, 2),
             "mismatching_chains": bad, "jobs": st["jobs"], "executor_jobs": st["small_jobs"],
-            "autotuned": {k: tune[k] for k in ("small_products", "threads", "sort_cap", "tile_rows", "trials")},
-            "result_nnz": int(nnz.sum())}
+            "tiered_jobs": st["large_jobs"], "plan_s": round(st["plan_s"], 3), "executor_s": round(st["exec_s"], 3),
+            "tiered_s": round(st["large_s"], 3),
+            "autotuned": {k: tune[k] for k in ("streams", "small_products", "threads", "sort_cap", "tile_rows",
+                                               "trials")},
+            "result_nnz": int(nnz.sum()),
+            "note": "product_plan = the per-chain path with the batch's product-aware plans (plan_cost 1), all "
+                    "chains; plain = the per-chain path with the reference DP's plans (the drop-in), on the "
+                    f"first {n0} chains; every per-chain result's digests are compared with the batch's"}
 
 
-def bench_served(hg1, n_queries: int = 128, qps: float = 5.0) -> dict:
-    """Workload 5: the server against one-at-a-time execution on the same Poisson stream."""
-    hg3 = configs.CFG3.build()
-    ctx = smcm.Context(0)
-    G = [smcm.Graph(ctx, hg1), smcm.Graph(ctx, hg3)]
+def served_queries(hg1, hg3, n_queries: int, qps: float):
     hgs = [hg1, hg3]
     stream = configs.cfg5_stream(hg1.dimensions, hg3.dimensions, n_queries=n_queries, qps=qps)
     queries = [(q.graph, q.path, (q.src_rows + hgs[q.graph].type_offset[q.path.vertexTypes[0]]).astype(np.int32))
                for q in stream]
-    arr = np.array([q.arrival for q in stream])
+    return queries, np.array([q.arrival for q in stream])
+
+
+def latencies(outs, arr):
+    lat = np.array([o["t_done"] - a for o, a in zip(outs, arr)])
+    span = max(o["t_done"] for o in outs) - float(np.min(arr))
+    return lat, span
+
+
+def bench_served(hg1, n_queries: int = 128, qps: float = 5.0) -> dict:
+    """Workload 5: the server against one-at-a-time execution on the same Poisson stream, and the
+    server's capacity (every query submitted at once)."""
+    hg3 = configs.CFG3.build()
+    ctx = smcm.Context(0)
+    G = [smcm.Graph(ctx, hg1), smcm.Graph(ctx, hg3)]
+    queries, arr = served_queries(hg1, hg3, n_queries, qps)
     res = {}
-    for name, kw in (("served", dict(slots=4, plan_cost=1)), ("plain", dict(slots=1, plan_cost=0, cache=0))):
+    for name, kw, at in (("served", dict(plan_cost=1), arr),
+                         ("plain", dict(slots=1, plan_cost=0, cache=0), arr),
+                         ("saturated", dict(plan_cost=1), np.zeros(n_queries))):
         srv = smcm.Server(ctx, G, **kw)
-        outs, digs = srv.replay(queries, arr)
+        outs, digs = srv.replay(queries, at)
         stats = srv.stats()
         srv.close()
-        lat = np.array([o["t_done"] - a for o, a in zip(outs, arr)])
-        span = max(o["t_done"] for o in outs) - arr[0]
+        lat, span = latencies(outs, at)
         res[name] = (outs, digs, lat, span, stats)
         print(f"[bench] {name}: {n_queries} queries, span {span:.1f} s", file=sys.stderr, flush=True)
-    (o1, d1, l1, s1, st1), (o0, d0, l0, s0, _) = res["served"], res["plain"]
-    bad = sum(1 for i in range(n_queries) if o1[i]["status"] or o0[i]["status"] or o1[i]["nnz"] != o0[i]["nnz"]
-              or not np.array_equal(d1[i], d0[i]))
+    (o1, d1, l1, s1, st1), (o0, d0, l0, s0, _), (o2, d2, l2, s2, _) = res["served"], res["plain"], res["saturated"]
+    bad = sum(1 for i in range(n_queries)
+              if o1[i]["status"] or o0[i]["status"] or o2[i]["status"] or o1[i]["nnz"] != o0[i]["nnz"]
+              or not np.array_equal(d1[i], d0[i]) or not np.array_equal(d2[i], d0[i]))
     for G_ in G:
         G_.close()
     ctx.close()
@@ -312,9 +343,63 @@ def bench_served(hg1, n_queries: int = 128, qps: float = 5.0) -> dict:
             "plain_queries_per_s": round(n_queries / s0, 3),
             "plain_p50_ms": round(float(np.percentile(l0, 50)) * 1e3, 1),
             "plain_p99_ms": round(float(np.percentile(l0, 99)) * 1e3, 1),
+            "capacity_queries_per_s": round(n_queries / s2, 3),
+            "capacity_p50_ms": round(float(np.percentile(l2, 50)) * 1e3, 1),
             "mismatching_queries": bad, "cache_hits": st1["cache_hits"],
-            "server": "4 slots (2 for large queries), products plan, 200 GiB admission budget, 16 GiB prefix cache",
-            "plain": "one query at a time, reference DP plan, no cache"}
+            "server": "6 slots (2 for large queries, their persistent kernels sized for 50% of the CUs), products "
+                      "plan, 200 GiB admission budget, 16 GiB prefix cache",
+            "plain": "one query at a time, reference DP plan, no cache",
+            "capacity": "the same queries submitted at once to the same server configuration"}
+
+
+def bench_served_sweep(hg1, n_queries: int = 512, rates=(5.0, 10.0, 20.0)) -> dict:
+    """Serving capacity: n_queries of workload 5 at several arrival rates and all at once, each
+    query's latency in the mix against its own time when queries run one at a time through the
+    same server configuration (one slot; consecutive completions), results compared."""
+    hg3 = configs.CFG3.build()
+    ctx = smcm.Context(0)
+    G = [smcm.Graph(ctx, hg1), smcm.Graph(ctx, hg3)]
+    queries, _ = served_queries(hg1, hg3, n_queries, rates[0])
+    srv = smcm.Server(ctx, G, slots=1, large_slots=0, plan_cost=1)
+    alone, dig0 = [], []
+    for c in range(0, n_queries, 64):  # chunks of one server (its cache kept), progress per chunk
+        outs, digs = srv.replay(queries[c:c + 64], np.zeros(len(queries[c:c + 64])))
+        done = np.array([o["t_done"] for o in outs])
+        alone.extend(np.diff(np.concatenate([[0.0], done])))  # FIFO on one slot: completion gaps
+        dig0.extend(digs)
+        print(f"[bench] served sweep: one at a time, {c + len(outs)} queries", file=sys.stderr, flush=True)
+    srv.close()
+    alone = np.array(alone)
+    out = {"workload": f"{n_queries} queries of workload 5 (seed 0x5EED0005), same queries at every rate",
+           "alone_seconds": round(float(alone.sum()), 3), "runs": []}
+    for rate in list(rates) + [None]:
+        if rate is None:
+            q2, arr = queries, np.zeros(n_queries)
+        else:
+            q2, arr = served_queries(hg1, hg3, n_queries, rate)
+            assert all(a[1].vertexTypes == b[1].vertexTypes and np.array_equal(a[2], b[2]) for a, b in zip(q2, queries))
+        srv = smcm.Server(ctx, G, plan_cost=1)
+        o, d = srv.replay(q2, arr)
+        hits = srv.stats()["cache_hits"]
+        srv.close()
+        lat, span = latencies(o, arr)
+        bad = sum(1 for i in range(n_queries) if o[i]["status"] or not np.array_equal(d[i], dig0[i]))
+        slow = lat / np.maximum(alone, 1e-4)
+        worst = np.argsort(-slow)[:5]
+        out["runs"].append({
+            "offered_qps": rate if rate is not None else "all at once", "queries_per_s": round(n_queries / span, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1), "p90_ms": round(float(np.percentile(lat, 90)) * 1e3, 1),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1), "max_ms": round(float(lat.max()) * 1e3, 1),
+            "slowdown_p50": round(float(np.percentile(slow, 50)), 2), "slowdown_p90": round(float(np.percentile(slow, 90)), 2),
+            "slowdown_max": round(float(slow.max()), 2), "cache_hits": hits, "mismatching_queries": bad,
+            "worst": [{"query": int(i), "path": queries[i][1].toString(), "graph": int(queries[i][0]),
+                       "sources": int(len(queries[i][2])), "latency_ms": round(float(lat[i]) * 1e3, 1),
+                       "alone_ms": round(float(alone[i]) * 1e3, 1)} for i in worst]})
+        print(f"[bench] served sweep: rate {rate}: span {span:.1f} s", file=sys.stderr, flush=True)
+    for G_ in G:
+        G_.close()
+    ctx.close()
+    return out
 
 
 def main() -> int:
@@ -331,7 +416,7 @@ def main() -> int:
     ap.add_argument("--batch-chains", type=int, default=4096)
     ap.add_argument("--serve-queries", type=int, default=128)
     ap.add_argument("--serve-qps", type=float, default=5.0)
-    ap.add_argument("--only", choices=["config1", "config3", "batched", "served", "vendor"],
+    ap.add_argument("--only", choices=["config1", "config3", "batched", "served", "served_sweep", "vendor"],
                     help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -343,7 +428,8 @@ def main() -> int:
         res = {"config1": lambda: bench_config1(hg1), "config3": lambda: bench_config3(),
                "batched": lambda: bench_batched(hg1, args.batch_chains),
                "vendor": lambda: bench_vendor(hg1, configs.CFG2.build()),
-               "served": lambda: bench_served(hg1, args.serve_queries, args.serve_qps)}[args.only]()
+               "served": lambda: bench_served(hg1, args.serve_queries, args.serve_qps),
+               "served_sweep": lambda: bench_served_sweep(hg1)}[args.only]()
         print(json.dumps({args.only: res}))
         return 0
     chains = args.chains.split(",")

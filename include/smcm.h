@@ -178,7 +178,8 @@ int smcm_estimator_rows(smcm_ctx* ctx, smcm_graph* g, const int32_t* vtypes,
 /* ---- batched form (no counterpart in the reference) ------------------------------------- */
 typedef struct smcm_batch_stats {
   int32_t chains, jobs;                    /* jobs: distinct meta-paths */
-  int32_t small_jobs, large_jobs, rerun_jobs, groups;
+  int32_t small_jobs, large_jobs, rerun_jobs, groups;  /* large jobs run on "batch_streams"
+                                              concurrent worker streams */
   int64_t nodes, tiles, products_est;
   double plan_s, exec_s, large_s, total_s; /* host wall-clock seconds */
 } smcm_batch_stats;
@@ -202,11 +203,14 @@ typedef struct smcm_batch_tuning {
   double baseline_s, best_s;     /* sample time with the settings on entry / the chosen ones */
   int64_t small_products;        /* chosen options (set on the context) */
   int32_t threads, sort_cap, tile_rows, trials;
+  int32_t streams;               /* concurrent worker streams of the tiered-path jobs */
+  int32_t pad;
 } smcm_batch_tuning;
 /* Autotuner of the batched form: runs smcm_batch_run on the given chains (a sample of the
- * workload), best of reps runs per setting: first the executor threshold (batch_small_products),
- * then the executor shape (batch_threads, batch_sort_cap, batch_tile_rows).  Leaves the fastest
- * settings on ctx and reports them. */
+ * workload), best of reps runs per setting: first the number of worker streams
+ * (batch_streams), then the executor threshold (batch_small_products), then the executor shape
+ * (batch_threads, batch_sort_cap, batch_tile_rows).  Leaves the fastest settings on ctx and
+ * reports them. */
 int smcm_batch_autotune(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t* path_len,
                         const int32_t* vtypes, const int32_t* etypes, const int64_t* src_off,
                         const int32_t* src_rows, int32_t reps, smcm_batch_tuning* out);
@@ -245,8 +249,8 @@ typedef struct smcm_query_out {
   int32_t pad;
   char error[160];                  /* message when status != SMCM_OK */
 } smcm_query_out;
-/* graphs: uploaded to ctx; opts NULL = 4 slots (2 of them for large queries, above 4 GiB
- * predicted, with kernels sized for 75% of the CUs), products plan, 200 GiB budget, 16 GiB cache (entries up to 2 GiB), 32 GiB row
+/* graphs: uploaded to ctx; opts NULL = 6 slots (2 of them for large queries, above 4 GiB
+ * predicted, with kernels sized for 50% of the CUs), products plan, 200 GiB budget, 16 GiB cache (entries up to 2 GiB), 32 GiB row
  * blocks.  Small-lane workers take arrivals in order and hand a query they find large to the
  * large-lane workers, so short queries do not queue behind long ones. */
 int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graphs,

@@ -98,7 +98,10 @@ struct smcm_ctx {
     int tile_rows = 32;
     int log2w = 13;                        // executor window width (columns)
     int64_t arena_entries = (int64_t)1 << 31;  // executor output arena (12 bytes per entry)
+    int streams = 2;                       // jobs of the tiered path run concurrently on this many
+                                           // worker contexts (own stream and allocator each)
   } batch;
+  std::vector<smcm_ctx*> batch_workers;  // created on first use, destroyed with the context
   smcm_stats last{};
   int64_t live = 0;  // graphs and results created from this context and not yet freed
   smcm::KernelTimer timer;

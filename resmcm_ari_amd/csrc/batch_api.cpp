@@ -12,10 +12,14 @@
 //    gather), intermediate rows stay in the executor's arena.  Their results are digested (and,
 //    if asked for, compacted into result handles) straight from the arena.
 // 3. The other chains, and any small chain whose products outgrew the arena, run through the
-//    per-chain tiered path with the same plan.
+//    per-chain tiered path with the same plan, several jobs at a time on worker contexts (own
+//    stream and allocator each), largest first, each admitted when its estimator-predicted bytes
+//    fit the context's budget next to the running ones.  A job that still runs out of device
+//    memory is run again alone on the calling context.
 #include "smcm.h"
 
 #include "api_internal.h"
+#include "server_sched.h"
 
 using namespace smcm_host;
 
@@ -41,6 +45,8 @@ struct BatchChain {
   std::vector<int32_t> h_len;
   Csr C;
   bool have_C = false;
+  smcm_ctx* c_owner = nullptr;   // the context whose allocator holds C
+  std::vector<double> ctab;      // estimated sizes c_ij (L*L)
 };
 
 // one chain of the call: its job and its rows' positions in the job's rows
@@ -190,8 +196,142 @@ void run_large(smcm_ctx* ctx, smcm_graph* g, BatchChain& bc, bool keep) {
   if (keep) {
     bc.C = C;
     bc.have_C = true;
+    bc.c_owner = ctx;
   } else {
     free_csr(al, C);
+  }
+}
+
+// estimator-predicted device bytes of a job on the tiered path: 12 per estimated entry of every
+// product its plan materialises, the root included (it is kept or digested whole)
+double job_bytes(const BatchChain& b, smcm_graph* g) {
+  const int32_t L = b.L;
+  if (L == 1) return 12.0 * (double)typed(g, b.vts[0], b.ets[0], b.vts[1]).nnz;
+  double pred = 0;
+  std::vector<std::pair<int32_t, int32_t>> stk;
+  for (size_t x = 0; 2 * x < b.pairs.size(); ++x) {
+    const int32_t k = b.pairs[2 * x], l = b.pairs[2 * x + 1];
+    std::pair<int32_t, int32_t> A, B;
+    if (k >= 0 && l >= 0) {
+      A = {k, k};
+      B = {l, l};
+    } else if (k < 0 && l >= 0) {
+      A = stk.back();
+      stk.pop_back();
+      B = {l, l};
+    } else if (k >= 0 && l < 0) {
+      B = stk.back();
+      stk.pop_back();
+      A = {k, k};
+    } else {
+      B = stk.back();
+      stk.pop_back();
+      A = stk.back();
+      stk.pop_back();
+    }
+    pred += 12.0 * std::max(0.0, b.ctab[(size_t)A.first * L + B.second]);
+    stk.push_back({A.first, B.second});
+  }
+  return pred;
+}
+
+// the worker contexts of the batched form (created on first use, options of the caller's context)
+std::vector<smcm_ctx*>& batch_workers(smcm_ctx* ctx, int n) {
+  while ((int)ctx->batch_workers.size() < n) {
+    smcm_ctx* w = nullptr;
+    const int rc = smcm_ctx_create(ctx->device, ctx->budget, &w);
+    if (rc != SMCM_OK) fail(rc, "batch worker context: " + std::string(smcm_last_error()));
+    ctx->batch_workers.push_back(w);
+  }
+  for (smcm_ctx* w : ctx->batch_workers) {
+    w->opts = ctx->opts;
+    w->exact_cost_sum = ctx->exact_cost_sum;
+    w->budget = ctx->budget;
+  }
+  return ctx->batch_workers;
+}
+
+struct LargeJob : smcm_sched::Job {
+  int32_t idx = 0;
+  uint64_t bytes = 0;
+  bool oom = false;
+};
+
+// the tiered-path jobs `ids`, several at a time on worker contexts; jobs that ran out of device
+// memory next to others run again alone on ctx
+void run_large_jobs(smcm_ctx* ctx, smcm_graph* g, std::vector<BatchChain>& ch, std::vector<int32_t> ids,
+                    bool keep) {
+  const int nw = std::min<int>(ctx->batch.streams, (int)ids.size());
+  if (nw <= 1) {
+    for (int32_t i : ids) run_large(ctx, g, ch[i], keep);
+    return;
+  }
+  std::vector<smcm_ctx*>& workers = batch_workers(ctx, nw);
+  std::stable_sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) { return ch[a].est_products > ch[b].est_products; });
+  std::vector<int> status(ch.size(), SMCM_OK);
+  std::vector<std::string> errors(ch.size());
+  {
+    smcm_sched::Scheduler sched(nw, 0, ctx->budget);
+    sched.start([&](smcm_sched::Job& j, int wi, bool) {
+      LargeJob& job = static_cast<LargeJob&>(j);
+      smcm_ctx* w = workers[wi];
+      try {
+        SMCM_HIP(hipSetDevice(w->device));
+        sched.admit(job, job.bytes);
+        struct Release {
+          smcm_sched::Scheduler& s;
+          LargeJob& j;
+          ~Release() { s.release(j); }
+        } rel{sched, job};
+        run_large(w, g, ch[job.idx], keep);
+      } catch (const ApiError& e) {
+        job.status = e.code;
+        job.error = e.msg;
+      } catch (const HipError& e) {
+        job.oom = e.code == (int)hipErrorOutOfMemory;
+        job.status = job.oom ? SMCM_ENOMEM : SMCM_EDEVICE;
+        job.error = e.what();
+        (void)hipGetLastError();
+      } catch (const std::exception& e) {
+        job.status = SMCM_EDEVICE;
+        job.error = e.what();
+      }
+      return true;
+    });
+    std::vector<int64_t> tickets;
+    for (int32_t i : ids) {
+      auto j = std::make_unique<LargeJob>();
+      j->idx = i;
+      j->bytes = (uint64_t)std::min(job_bytes(ch[i], g), 1.8e19);
+      tickets.push_back(sched.submit(std::move(j)));
+    }
+    for (int64_t t : tickets) {
+      std::unique_ptr<smcm_sched::Job> out;
+      if (sched.poll(t, true, &out) != 0 || !out) continue;
+      const LargeJob& j = static_cast<const LargeJob&>(*out);
+      status[j.idx] = j.oom ? 1 : j.status;  // 1: out of memory next to others
+      errors[j.idx] = j.error;
+    }
+    sched.stop();
+  }
+  bool released = false;
+  for (smcm_ctx* w : workers) SMCM_HIP(hipStreamSynchronize(w->stream));
+  for (int32_t i : ids) {
+    if (status[i] == 1) {
+      if (!released) {  // the workers' cached blocks go back to the device pool first
+        for (smcm_ctx* w : workers) {
+          w->al.release_cache();
+          SMCM_HIP(hipStreamSynchronize(w->stream));
+        }
+        smcm::DevAlloc::trim_pool();
+        released = true;
+      }
+      // a partial result of the failed attempt is not kept (run_large frees C on errors)
+      ch[i].have_C = false;
+      run_large(ctx, g, ch[i], keep);
+    } else if (status[i] != SMCM_OK) {
+      fail(status[i], "batch job: " + errors[i]);
+    }
   }
 }
 
@@ -337,6 +477,7 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
       c.ptab.assign(htab.begin() + toff[i] + LL, htab.begin() + toff[i] + LL + LL * b.L);
       c.dp();
       b.pairs = c.pairs;
+      b.ctab = c.c;
       // estimated products of the chosen plan
       double tot = 0;
       std::vector<std::pair<int32_t, int32_t>> stk;
@@ -495,6 +636,7 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
         if (keep) {
           smcm::padded_to_csr(R.out_start, R.out_len, acol.p, aval.p, R.nrows, R.ncols, b.C, al);
           b.have_C = true;
+          b.c_owner = ctx;
         }
       }
       SMCM_HIP(hipStreamSynchronize(st));
@@ -509,10 +651,8 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
       if (!ch[i].small) large.push_back(i);
     large.insert(large.end(), rerun.begin(), rerun.end());
     bs.rerun_jobs = (int32_t)rerun.size();
-    for (int32_t i : large) {
-      run_large(ctx, g, ch[i], keep);
-      bs.large_jobs++;
-    }
+    run_large_jobs(ctx, g, ch, large, keep);
+    bs.large_jobs = (int32_t)large.size();
     bs.large_s = now_s() - t_large;
 
     // ---- every chain: its rows of its job's result ----------------------------------------
@@ -546,12 +686,14 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
         }
       }
     } catch (...) {
+      SMCM_HIP(hipStreamSynchronize(st));
       for (auto& b : ch)
-        if (b.have_C) free_csr(al, b.C);
+        if (b.have_C) free_csr(b.c_owner->al, b.C);
       throw;
     }
+    // the gathers above ran on this context's stream and are complete (synchronised per chain)
     for (auto& b : ch)
-      if (b.have_C) free_csr(al, b.C);
+      if (b.have_C) free_csr(b.c_owner->al, b.C);
     bs.jobs = n_jobs;
     for (int32_t i = 0; i < n_jobs; ++i) bs.products_est += (int64_t)ch[i].est_products;
     bs.total_s = now_s() - t_all;
@@ -585,6 +727,22 @@ int smcm_batch_autotune(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const in
     };
     smcm_batch_tuning t{};
     t.trials = 0;
+    // 0. how many tiered-path jobs run at once (worker streams)
+    {
+      const double base = time_it();
+      double bs = base;
+      int best_streams = ctx->batch.streams;
+      for (int ns : {1, 2, 4, 8}) {
+        ctx->batch.streams = ns;
+        const double s = time_it();
+        t.trials++;
+        if (s >= 0 && (bs < 0 || s < bs)) {
+          bs = s;
+          best_streams = ns;
+        }
+      }
+      ctx->batch.streams = best_streams;
+    }
     // 1. which jobs go to the executor
     const int64_t thresholds[] = {0, 1 << 20, 1 << 22, 1 << 24, 1 << 25, 1 << 27};
     double best = time_it();
@@ -626,6 +784,7 @@ int smcm_batch_autotune(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const in
     t.threads = ctx->batch.threads;
     t.sort_cap = ctx->batch.sort_cap;
     t.tile_rows = ctx->batch.tile_rows;
+    t.streams = ctx->batch.streams;
     if (out) *out = t;
   });
 }

@@ -422,7 +422,7 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
       s->graphs.push_back(graphs[i]);
     }
     smcm_server_opts o{};
-    o.slots = 4;
+    o.slots = 6;
     o.plan_cost = 1;
     o.budget_bytes = 200ull << 30;
     o.cache_bytes = 16ull << 30;
@@ -430,7 +430,7 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
     o.block_bytes = 32ull << 30;
     o.large_slots = 2;
     o.large_bytes = 4ull << 30;
-    o.large_cu_percent = 75;
+    o.large_cu_percent = 50;
     if (opts) o = *opts;
     if (o.large_slots < 0 || o.large_slots > o.slots) fail(SMCM_EINVAL, "large_slots must be in [0, slots]");
     if (o.large_cu_percent < 0 || o.large_cu_percent > 100) fail(SMCM_EINVAL, "large_cu_percent must be in [0, 100]");

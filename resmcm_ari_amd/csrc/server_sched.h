@@ -103,8 +103,10 @@ class Scheduler {
     return 0;
   }
 
-  // blocks until `bytes` fit the budget next to the admitted jobs (or nothing else is admitted)
+  // blocks until `bytes` fit the budget next to the admitted jobs (or nothing else is admitted);
+  // every job counts at least one byte, so a budget of b bytes admits at most b jobs at once
   void admit(Job& j, uint64_t bytes) {
+    bytes = bytes > 0 ? bytes : 1;
     std::unique_lock<std::mutex> lk(mu_);
     cv_budget_.wait(lk, [&] { return admitted_ == 0 || inflight_ + bytes <= budget_; });
     inflight_ += bytes;

@@ -88,6 +88,8 @@ int smcm_ctx_destroy(smcm_ctx* ctx) {
     if (ctx->live != 0)
       fail(SMCM_EINVAL, std::to_string(ctx->live) + " graph/result handle(s) still alive; free them first");
     (void)hipSetDevice(ctx->device);
+    for (smcm_ctx* w : ctx->batch_workers) smcm_ctx_destroy(w);
+    ctx->batch_workers.clear();
     ctx->al.release_cache();
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
@@ -186,6 +188,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "batch_arena_mb") {
       if (v < 1) fail(SMCM_EINVAL, "batch_arena_mb must be >= 1");
       ctx->batch.arena_entries = v * ((int64_t)1 << 20) / 12;
+    } else if (k == "batch_streams") {
+      if (v < 1 || v > 16) fail(SMCM_EINVAL, "batch_streams must be in [1, 16]");
+      ctx->batch.streams = (int)v;
     } else if (k == "kernel_timing") {
       ctx->timer.enabled = v != 0;
     } else if (k == "budget") {
@@ -235,6 +240,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "batch_tile_rows") *v = ctx->batch.tile_rows;
     else if (k == "batch_log2w") *v = ctx->batch.log2w;
     else if (k == "batch_arena_mb") *v = ctx->batch.arena_entries * 12 / ((int64_t)1 << 20);
+    else if (k == "batch_streams") *v = ctx->batch.streams;
     else if (k == "kernel_timing") *v = ctx->timer.enabled;
     else if (k == "lds_bytes") *v = o.lds_bytes;
     else if (k == "cus") *v = o.cus;

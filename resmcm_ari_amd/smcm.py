@@ -60,7 +60,8 @@ class BatchStats(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("baseline_s", C.c_double), ("best_s", C.c_double), ("small_products", C.c_int64),
-                ("threads", C.c_int32), ("sort_cap", C.c_int32), ("tile_rows", C.c_int32), ("trials", C.c_int32)]
+                ("threads", C.c_int32), ("sort_cap", C.c_int32), ("tile_rows", C.c_int32), ("trials", C.c_int32),
+                ("streams", C.c_int32), ("pad", C.c_int32)]
 
 
 class ServerOpts(C.Structure):
@@ -476,14 +477,14 @@ def build_pair(graph: Graph, path: MetaPath, src_rows=None) -> tuple[Result, Pai
 class Server:
     """Serving form: worker slots over graphs resident in HBM (see smcm_server_create)."""
 
-    def __init__(self, ctx: Context, graphs: list, slots: int = 4, plan_cost: int = 1, budget: int = 200 << 30,
+    def __init__(self, ctx: Context, graphs: list, slots: int = 6, plan_cost: int = 1, budget: int = 200 << 30,
                  cache: int = 16 << 30, cache_entry_max: int = 2 << 30, block: int = 32 << 30,
-                 large_slots: int | None = None, large_bytes: int = 4 << 30, large_cu_percent: int = 75):
+                 large_slots: int | None = None, large_bytes: int = 4 << 30, large_cu_percent: int = 50):
         self.ctx = ctx
         self.graphs = list(graphs)
         arr = (C.c_void_p * len(graphs))(*[G._h for G in graphs])
         if large_slots is None:
-            large_slots = slots // 2 if slots > 1 else 0
+            large_slots = min(2, slots // 2) if slots > 1 else 0
         o = ServerOpts(slots, plan_cost, budget, cache, cache_entry_max, block, large_slots, large_cu_percent,
                        large_bytes)
         h = C.c_void_p()

@@ -92,6 +92,7 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     sc.admit(q, q.bytes);
     const auto c = sc.counters();
     if (c.admitted > 1 && c.inflight > budget) over++;
+    if (budget == 1 && c.admitted > 1) over++;  // a one-byte budget admits one job at a time
     sleep_us(q.exec_us);  // device work
     sc.release(q);
     q.status = 0;
@@ -107,7 +108,7 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     j->plan_us = (int)(u(rng) * plan_max_us);
     j->exec_us = (int)(u(rng) * exec_max_us);
     j->big = u(rng) < p_big;
-    j->bytes = (uint64_t)(u(rng) * (double)budget * (j->big ? 0.9 : 0.3));
+    j->bytes = u(rng) < 0.1 ? 0 : (uint64_t)(u(rng) * (double)budget * (j->big ? 0.9 : 0.3));
     j->fail = u(rng) < 0.03;
     want_fail.push_back(j->fail);
     tickets.push_back(sc.submit(std::move(j)));
@@ -196,6 +197,7 @@ int main(int argc, char** argv) {
     run_scenario("stop_slow_plan", 8, 3, 200, 40, 1.0, true, seed + 29, 3000, 50);
     run_scenario("stop_slow_plan", 4, 2, 200, 40, 1.0, true, seed + 31, 3000, 50);
     run_scenario("tight", 4, 2, 10, 60, 0.3, false, seed + 19);
+    run_scenario("one_byte", 4, 2, 1, 40, 0.3, false, seed + 37);  // one job at a time
     run_scenario("nolane", 3, 0, 200, 60, 0.3, false, seed + 23);
   }
   g_finished = true;

@@ -158,7 +158,7 @@ def test_config5_stream(ctx, cfg1, cfg3):
     for gi, mp, src in queries:
         n, d, _ = Gs[gi].run_blocked(mp, 32 << 30, src_rows=src, digest=True)
         alone.append((n, d))
-    srv = smcm.Server(ctx, Gs, slots=4, plan_cost=1)
+    srv = smcm.Server(ctx, Gs, plan_cost=1)  # the default serving configuration
     try:
         outs, digs = srv.replay(queries, np.array([q.arrival for q in stream]))
         hits = srv.stats()["cache_hits"]
