@@ -261,6 +261,7 @@ __device__ __forceinline__ int64_t block_exscan_i64(int64_t x, int64_t* red, int
 // row (a hub) is shared by the whole workgroup, and there is no per-product search: a piece's
 // segment is found once per piece (wave-uniform LDS reads).
 constexpr int kChunk = 512;  // A entries per traversal round
+constexpr int64_t kSegMaxBytes = (int64_t)1 << 30;  // largest segment table of B per product
 constexpr int kTiny = 2;
 constexpr int kQuarter = 16;
 
@@ -1051,6 +1052,23 @@ __global__ void __launch_bounds__(kThreads) window_bounds_kernel(const int64_t* 
   }
 }
 
+// Segment table of B per group of G numeric windows: seg[g * nrows + k] = absolute [first, end)
+// of the elements of row k of B with columns in windows [g*G, min((g+1)*G, nwin)).  A task then
+// gets its segment with one 16-byte load; from the bound table it takes three (two offsets in
+// different rows of the table, then the row start).
+__global__ void __launch_bounds__(kThreads) window_segs_kernel(const int64_t* __restrict__ bptr,
+                                                               const int32_t* __restrict__ bnd, int32_t nrows,
+                                                               int32_t nwin, int32_t G, int32_t ngroups,
+                                                               longlong2* __restrict__ seg) {
+  const int64_t total = (int64_t)ngroups * nrows;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kThreads) {
+    const int32_t g = (int32_t)(i / nrows), k = (int32_t)(i - (int64_t)g * nrows);
+    const int32_t wa = g * G, wb = min(wa + G, nwin);
+    const int64_t b = bptr[k];
+    seg[i] = make_longlong2(b + bnd[(size_t)wa * nrows + k], b + bnd[(size_t)wb * nrows + k]);
+  }
+}
+
 template <bool AV, bool BV>
 __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol,
                                    const uint64_t* __restrict__ aval, const int64_t* __restrict__ bptr,
@@ -1241,7 +1259,8 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
                                     const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
                                     const uint64_t* __restrict__ bval, const WinRow* __restrict__ info,
                                     const int32_t* __restrict__ pre, int32_t nrows, int32_t ncols, int log2w,
-                                    int32_t nwin, const int32_t* __restrict__ bnd, int32_t nbrows,
+                                    int32_t nwin, const int32_t* __restrict__ bnd,
+                                    const longlong2* __restrict__ seg, int32_t nbrows,
                                     int32_t* __restrict__ queue, int32_t* __restrict__ ccol,
                                     uint64_t* __restrict__ cval) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1323,6 +1342,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
     const int64_t w0 = (int64_t)w << log2w;
     const int32_t* b_lo = bnd + (size_t)w * nbrows;
     const int32_t* b_hi = b_lo + nbrows;
+    const longlong2* segw = seg ? seg + (size_t)w * nbrows : nullptr;  // one load per segment
     // PRES: presence bits from the start (one pass; the compaction touches only occupied slots)
     for (int pass = PRES ? 1 : 0; pass < 2; ++pass) {
       const bool with_pres = pass == 1;
@@ -1349,9 +1369,15 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
           }
         }
         if (valid) {
-          const int32_t s0 = b_lo[k];
-          cnt = b_hi[k] - s0;
-          st = bptr[k] + s0;
+          if (segw) {
+            const longlong2 sg = segw[k];
+            st = sg.x;
+            cnt = sg.y - sg.x;
+          } else {
+            const int32_t s0 = b_lo[k];
+            cnt = b_hi[k] - s0;
+            st = bptr[k] + s0;
+          }
         }
         trav_round<BV>(st, cnt, a, bcol, bval, L, acc);
       }
@@ -1457,7 +1483,8 @@ template <bool MERGE>
 __global__ void window_symbolic_kernel(const int32_t* __restrict__ acol, const int64_t* __restrict__ bptr,
                                        const int32_t* __restrict__ bcol, const SymRow* __restrict__ info,
                                        int32_t nrows, int log2w, int sub, int32_t nwin,
-                                       const int32_t* __restrict__ bnd, int32_t nbrows, int32_t* __restrict__ queue,
+                                       const int32_t* __restrict__ bnd, const longlong2* __restrict__ seg,
+                                       int32_t nbrows, int32_t* __restrict__ queue,
                                        int32_t* __restrict__ wcnt, int64_t* __restrict__ row_nnz) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* bm = (uint32_t*)smem;
@@ -1495,14 +1522,21 @@ __global__ void window_symbolic_kernel(const int32_t* __restrict__ acol, const i
     const int64_t w0 = (int64_t)wa << log2w;
     const int32_t* b_lo = bnd + (size_t)wa * nbrows;
     const int32_t* b_hi = bnd + (size_t)wb * nbrows;
+    const longlong2* segw = seg ? seg + (size_t)ws * nbrows : nullptr;  // one load per segment
     for (int64_t base = rs; base < re; base += chunk) {
       int64_t st = 0, cnt = 0;
       const int64_t e = base + threadIdx.x;
       if ((int)threadIdx.x < chunk && e < re) {
         const int32_t k = acol[e];
-        const int32_t s0 = b_lo[k];
-        cnt = b_hi[k] - s0;
-        st = bptr[k] + s0;
+        if (segw) {
+          const longlong2 sg = segw[k];
+          st = sg.x;
+          cnt = sg.y - sg.x;
+        } else {
+          const int32_t s0 = b_lo[k];
+          cnt = b_hi[k] - s0;
+          st = bptr[k] + s0;
+        }
       }
       if constexpr (MERGE) {
         // very dense rows: lanes whose columns share a 32-bit word are merged (segmented OR over
@@ -1606,6 +1640,9 @@ struct SplitScratch {
   int32_t* counts = nullptr;          // 3: narrow rows, wide rows, widest narrow row
   unsigned long long* sums = nullptr;  // 6
   int32_t* bnd = nullptr;             // window bound table of B (built on first use)
+  longlong2* nseg = nullptr;          // segment tables of B per numeric window and per symbolic
+  longlong2* sseg = nullptr;          // window (built on first use; null above kSegMaxBytes)
+  bool nseg_tried = false, sseg_tried = false;
   int32_t* wcnt = nullptr;            // columns per (row, window) from the symbolic pass, or null
                                       // (then the window tier runs one row at a time)
   WindowLayout wl;
@@ -1613,6 +1650,8 @@ struct SplitScratch {
   struct SpgemmCore* owner = nullptr;
   DevAlloc* al = nullptr;             // the owner's allocator (per-launch task metadata)
   void build_bounds(const Csr& B, hipStream_t st);
+  // segment table of G numeric windows per group (null when it would exceed kSegMaxBytes)
+  longlong2* build_segs(const Csr& B, int32_t G, hipStream_t st);
 };
 
 // algorithmic bytes of a pass over one tier: A and C row pointers, A columns (+values) with
@@ -1696,8 +1735,12 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
     heavy.a_nnz[0] = tc.a_nnz[kHeavyTier];
     heavy.products[0] = tc.products[kHeavyTier];
     const int tok = T.start("spgemm.symbolic.window", st);
+    if (!split->sseg_tried) {
+      split->sseg = split->build_segs(B, 1 << sub, st);
+      split->sseg_tried = true;
+    }
     skern<<<g, o.d_threads, lds, st>>>(A.col, B.ptr, B.col, info.p, nH, split->wl.log2w, sub, nwin, split->bnd,
-                                       B.nrows, queue, split->wcnt, row_nnz);
+                                       split->sseg, B.nrows, queue, split->wcnt, row_nnz);
     SMCM_LAUNCH_CHECK();
     T.stop(tok, st, tier_bytes(false, heavy, 0, AV, BV));
   }
@@ -1797,8 +1840,13 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
         const bool pres = o.window_presence == 1 || (o.window_presence == 2 && fill < 0.125);
         auto kern = pres ? window_major_kernel<AV, BV, true> : window_major_kernel<AV, BV, false>;
         const unsigned gm = persistent_grid(o, occupancy((const void*)kern, o.w_threads, lds));
+        if (!split->nseg_tried) {
+          split->nseg = split->build_segs(B, 1, st);
+          split->nseg_tried = true;
+        }
         kern<<<gm, o.w_threads, lds, st>>>(A.col, A.val, B.ptr, B.col, B.val, info.p, pre.p, cnt[1], B.ncols,
-                                           split->wl.log2w, nwin, split->bnd, B.nrows, queue, ccol, cval);
+                                           split->wl.log2w, nwin, split->bnd, split->nseg, B.nrows, queue, ccol,
+                                           cval);
       } else {
         window_tier_kernel<AV, BV><<<g, o.w_threads, lds, st>>>(
             A.ptr, A.col, A.val, B.ptr, B.col, B.val, split->wide, cnt[1], B.ncols, split->wl.log2w,
@@ -1941,6 +1989,7 @@ struct SpgemmCore {
   int32_t *counts = nullptr, *queue = nullptr;
   unsigned long long* sums = nullptr;
   DevBuf<int32_t> split_rows, bnd, wcnt;
+  DevBuf<longlong2> segbuf[2];
   SplitScratch split;
   TierCounts tc;
   BitmapLayout lay;
@@ -2058,6 +2107,22 @@ struct SpgemmCore {
     st->rows[3] += tc.rows[kBitmapTier] + tc.rows[kHeavyTier];
   }
 };
+
+longlong2* SplitScratch::build_segs(const Csr& B, int32_t G, hipStream_t st) {
+  const int32_t nwin = (int32_t)wl.nwin;
+  const int32_t ngroups = (nwin + G - 1) / G;
+  const int64_t n = (int64_t)std::max(B.nrows, 1) * ngroups;
+  if (n * (int64_t)sizeof(longlong2) > kSegMaxBytes) return nullptr;
+  if (!bnd) build_bounds(B, st);
+  DevBuf<longlong2>& buf = owner->segbuf[G == 1 ? 0 : 1];
+  buf = DevBuf<longlong2>(*al, n);
+  if (B.nrows > 0) {
+    window_segs_kernel<<<(unsigned)std::min<int64_t>(ceil_div(n, kThreads), 65536), kThreads, 0, st>>>(
+        B.ptr, bnd, B.nrows, nwin, G, ngroups, buf.p);
+    SMCM_LAUNCH_CHECK();
+  }
+  return buf.p;
+}
 
 void SplitScratch::build_bounds(const Csr& B, hipStream_t st) {
   owner->bnd = DevBuf<int32_t>(owner->al, (int64_t)std::max(B.nrows, 1) * (wl.nwin + 1));

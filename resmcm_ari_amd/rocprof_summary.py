@@ -9,6 +9,15 @@
 --bench   a file holding the JSON line bench.py printed in the traced run (HIP-event durations and the
           algorithmic bytes per launch of each timed kernel)
 --kernel  the timed kernel name the traffic line is written for (default: bench's roofline kernel)
+--sq      directory of `rocprofv3 --pmc SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_INSTS_LDS
+          SQ_LDS_BANK_CONFLICT SQ_LDS_IDX_ACTIVE SQ_INSTS_VMEM_RD SQ_INSTS_VALU GRBM_GUI_ACTIVE`: LDS
+          bank conflicts, occupancy and instruction mix per kernel
+
+Counter reductions (as rocprofv3 reports one value per dispatch): GRBM_GUI_ACTIVE is summed over
+the 8 XCDs, so a dispatch's cycles are GRBM_GUI_ACTIVE / 8; SQ_WAVE_CYCLES counts resident waves in
+units of 4 cycles (rocprofv3's OccupancyPercent = 400 * SQ_WAVE_CYCLES / cycles / CUs / 32); bank
+conflicts per LDS access cycle are SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT)
+(rocprofv3's LdsBankConflict).
 
 Traffic calibration: the copy kernel (`copy_kernel`, 16-byte loads and stores, launched by
 `smcm_copy_bandwidth` with a known byte count) runs in the same process; the ratio of its known
@@ -92,6 +101,40 @@ def counter_per_kernel(d: str, counter: str):
     return acc
 
 
+SQ_COUNTERS = ["SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+               "SQ_LDS_IDX_ACTIVE", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VALU", "GRBM_GUI_ACTIVE"]
+XCDS, CUS, MAX_WAVES_PER_CU = 8, 256, 32
+
+
+def sq_table(d: str, top: int):
+    """kernel -> per-dispatch averages and derived LDS / occupancy figures, heaviest first."""
+    acc = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(set)
+    with open(one_csv(d, "_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            k = short_name(r["Kernel_Name"])
+            acc[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            disp[k].add(r["Dispatch_Id"])
+    rows = []
+    for k, c in acc.items():
+        n = len(disp[k])
+        cycles = c["GRBM_GUI_ACTIVE"] / XCDS
+        if cycles <= 0 or c["SQ_WAVES"] <= 0:
+            continue
+        waves_per_cu = 4.0 * c["SQ_WAVE_CYCLES"] / cycles / CUS
+        lds_active = c["SQ_LDS_IDX_ACTIVE"] - c["SQ_LDS_BANK_CONFLICT"]
+        rows.append({"kernel": k, "dispatches": n, "cycles": cycles / n, "waves": c["SQ_WAVES"] / n,
+                     "occupancy_pct": 100.0 * waves_per_cu / MAX_WAVES_PER_CU, "waves_per_cu": waves_per_cu,
+                     "lds_per_wave": c["SQ_INSTS_LDS"] / c["SQ_WAVES"],
+                     "vmem_rd_per_wave": c["SQ_INSTS_VMEM_RD"] / c["SQ_WAVES"],
+                     "valu_per_wave": c["SQ_INSTS_VALU"] / c["SQ_WAVES"],
+                     "conflicts_per_access": c["SQ_LDS_BANK_CONFLICT"] / lds_active if lds_active > 0 else 0.0,
+                     "conflict_stall_pct": 100.0 * c["SQ_LDS_BANK_CONFLICT"] / cycles / CUS,
+                     "wave_cycles": c["SQ_WAVE_CYCLES"]})
+    rows.sort(key=lambda r: -r["wave_cycles"])
+    return rows[:top]
+
+
 def match(pattern: str, names):
     rx = re.compile(pattern)
     return [n for n in names if rx.search(n)]
@@ -103,6 +146,7 @@ def main(argv=None) -> int:
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--bench")
+    ap.add_argument("--sq")
     ap.add_argument("--kernel")
     ap.add_argument("--copy-bytes", type=float, default=float(1 << 32),
                     help="bytes read (and written) by one copy_kernel launch")
@@ -148,6 +192,21 @@ def main(argv=None) -> int:
             out.append(f"| {tname} | {v['launches']} | {ev:.3f} | {tr:.3f} | {gb:.3f} |")
         out += ["", "The trace also covers the warm-up step; estimator.pass spans several kernels (light and heavy "
                 "rows, partial and final sums), so its trace column is a per-kernel average.", ""]
+
+    if a.sq:
+        out += ["## LDS, occupancy and instruction mix (`--pmc` " + " ".join(SQ_COUNTERS) + ")", "",
+                "Per dispatch: cycles = GRBM_GUI_ACTIVE / 8; occupancy = resident waves per CU "
+                "(4 * SQ_WAVE_CYCLES / cycles / 256) as a share of 32; bank conflicts = conflict cycles per "
+                "conflict-free LDS cycle; stall = conflict cycles as a share of all CU cycles.", "",
+                "| kernel | dispatches | Mcycles/dispatch | waves/dispatch | occupancy % (waves/CU) | LDS instr/wave | "
+                "VMEM rd/wave | VALU/wave | bank conflicts/access | conflict stall % |",
+                "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+        for r in sq_table(a.sq, a.top):
+            out.append(f"| `{r['kernel']}` | {r['dispatches']} | {r['cycles'] / 1e6:.3f} | {r['waves']:.0f} | "
+                       f"{r['occupancy_pct']:.1f} ({r['waves_per_cu']:.1f}) | {r['lds_per_wave']:.0f} | "
+                       f"{r['vmem_rd_per_wave']:.0f} | {r['valu_per_wave']:.0f} | {r['conflicts_per_access']:.3f} | "
+                       f"{r['conflict_stall_pct']:.2f} |")
+        out.append("")
 
     traffic_line = None
     if a.fetch and a.write:
