@@ -9,6 +9,7 @@
 // try (no device-wide wait: other contexts' streams keep running).
 #pragma once
 
+#include <functional>
 #include <map>
 #include <unordered_map>
 #include <vector>
@@ -34,6 +35,9 @@ struct DevAlloc {
   size_t cache_cap = SIZE_MAX;
   std::unordered_map<void*, size_t> sizes;
   std::multimap<size_t, void*> cache;
+  // called when an allocation fails, before this allocator's own cache is released: frees what
+  // idle allocators tied to this one (a context's batch workers) keep cached
+  std::function<void()> on_oom;
 
   void* alloc(size_t bytes) {
     bytes = (bytes + 255) / 256 * 256;
@@ -63,6 +67,7 @@ struct DevAlloc {
       hipError_t e = hipMallocAsync(&p, bytes, stream);
       if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
+        if (on_oom) on_oom();
         release_cache();
         (void)hipStreamSynchronize(stream);
         trim_pool();

@@ -243,6 +243,16 @@ std::vector<smcm_ctx*>& batch_workers(smcm_ctx* ctx, int n) {
     if (rc != SMCM_OK) fail(rc, "batch worker context: " + std::string(smcm_last_error()));
     ctx->batch_workers.push_back(w);
   }
+  // the workers keep their cached blocks between calls (they are reused by the next batch); an
+  // allocation of the calling context that fails takes them back first (workers are idle outside
+  // smcm_batch_run, which runs them and waits for them)
+  ctx->al.on_oom = [ctx] {
+    for (smcm_ctx* w : ctx->batch_workers) {
+      (void)hipStreamSynchronize(w->stream);
+      w->al.release_cache();
+      (void)hipStreamSynchronize(w->stream);
+    }
+  };
   for (smcm_ctx* w : ctx->batch_workers) {
     w->opts = ctx->opts;
     w->exact_cost_sum = ctx->exact_cost_sum;
