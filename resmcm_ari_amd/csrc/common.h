@@ -82,6 +82,10 @@ SMCM_DEV int64_t uniform_i64(int64_t x) {
 }
 SMCM_DEV uint64_t uniform_u64(uint64_t x) { return (uint64_t)uniform_i64((int64_t)x); }
 
+// The wave's index in its workgroup, in a scalar register: loops over wave-strided work then run
+// on the scalar unit instead of as exec-masked vector loops.
+SMCM_DEV int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave)); }
+
 SMCM_DEV int64_t wave_sum_i64(int64_t x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);

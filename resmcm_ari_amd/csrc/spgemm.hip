@@ -255,14 +255,16 @@ __device__ __forceinline__ int64_t block_exscan_i64(int64_t x, int64_t* red, int
 //   longer              cut into pieces of 64·E products taken round-robin by the waves, STEP
 //                       pieces per wave step, lanes reading consecutive elements (E·STEP loads
 //                       in flight per lane).  The numeric window kernel uses E = 1, STEP = 8
-//                       (short pieces fill the lanes); the bitmap tier E = 4, STEP = 2 (fewer
-//                       pieces to look up for its cheaper per-product work).
+//                       (short pieces fill the lanes) and four quarter steps in flight; the
+//                       bitmap tier E = 4, STEP = 2 (fewer pieces to look up for its cheaper
+//                       per-product work).  More steps in flight spill registers.
 // All lanes of a step read consecutive elements of one B row (coalesced), a single very long B
 // row (a hub) is shared by the whole workgroup, and there is no per-product search: a piece's
 // segment is found once per piece (wave-uniform LDS reads).
 constexpr int kChunk = 512;  // A entries per traversal round
 constexpr int64_t kSegMaxBytes = (int64_t)1 << 30;  // largest segment table of B per product
 constexpr int kTiny = 2;
+constexpr int kCompactBatch = 4;  // 64-column words per step of the window kernel's compaction
 constexpr int kQuarter = 16;
 
 constexpr int64_t kStaticLds = 16 * 1024;  // static LDS of the bitmap and window kernels (upper bound)
@@ -382,7 +384,10 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
     }
   }
   // pieces of the long segments (E elements per lane), STEP pieces per wave step: E * STEP loads
-  // in flight per lane
+  // in flight per lane.  The segment of piece ii is the number of long segments whose pieces end
+  // at or before ii: with at most 64 long segments one ballot over their ends (read once per
+  // round, a lane per segment) gives it, so the STEP lookups of a step do not wait on each other.
+  const int32_t qend = lane < nlong ? L.Q[lane + 1] : INT32_MAX;
   int t = 0;
   for (int32_t i = wid; i < npieces; i += STEP * nwaves) {
     int32_t n[STEP];
@@ -393,7 +398,11 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
       const int32_t ii = i + h * nwaves;
       n[h] = 0;
       if (ii < npieces) {
-        while (uniform_i32(L.Q[t + 1]) <= ii) ++t;  // wave-uniform; segments are visited in order
+        if (nlong <= kWave) {
+          t = (int)__popcll(__ballot(qend <= ii));
+        } else {
+          while (uniform_i32(L.Q[t + 1]) <= ii) ++t;  // wave-uniform; segments are visited in order
+        }
         const int32_t j = ii - uniform_i32(L.Q[t]);
         const int32_t sidx = nquarter + t;
         n[h] = min(kPieceLen, uniform_i32(L.len[sidx]) - j * kPieceLen);
@@ -1276,7 +1285,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
   __shared__ int64_t slot_m[2][4];
   __shared__ int32_t wcw[1024 / kWave];
   const int nthreads = blockDim.x, nwaves = nthreads / kWave;
-  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int lane = lane_id(), wid = wave_id();
   const int chunk = min(kChunk, nthreads);
   const uint64_t below = (1ull << lane) - 1ull;
   const int64_t ntasks = (int64_t)nrows * nwin;
@@ -1379,17 +1388,26 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
             st = bptr[k] + s0;
           }
         }
-        trav_round<BV>(st, cnt, a, bcol, bval, L, acc);
+        trav_round<BV, 1, 8, 4>(st, cnt, a, bcol, bval, L, acc);
       }
-      // compaction in column order: each wave owns a contiguous run of 64-column words
+      // compaction in column order: each wave owns a contiguous run of 64-column words; word
+      // masks, counts and offsets are wave-uniform (scalar registers)
       const int64_t wend = min(w0 + (int64_t)W, (int64_t)ncols);
       const int32_t nwords = (int32_t)((wend - w0 + 63) / 64);
       const int32_t per = (nwords + nwaves - 1) / nwaves;
       const int32_t ja = min(wid * per, nwords), jb = min(ja + per, nwords);
+      // the words are taken kCompactBatch at a time so that their LDS reads are in flight together
+      // (a read of a word past the wave's run is clamped to its last word and not used)
       int32_t mine = 0;
-      for (int32_t j = ja; j < jb; ++j) {
-        const uint64_t m = with_pres ? pres64[j] : __ballot(vals[j * 64 + lane] != 0ull);
-        mine += __popcll(m);
+      for (int32_t j = ja; j < jb; j += kCompactBatch) {
+        uint64_t m[kCompactBatch];
+#pragma unroll
+        for (int u = 0; u < kCompactBatch; ++u) {
+          const int32_t jj = min(j + u, jb - 1);
+          m[u] = with_pres ? uniform_u64(pres64[jj]) : __ballot(vals[jj * 64 + lane] != 0ull);
+        }
+#pragma unroll
+        for (int u = 0; u < kCompactBatch; ++u) mine += j + u < jb ? __popcll(m[u]) : 0;
       }
       if (lane == 0) wcw[wid] = mine;
       if (threadIdx.x == 0) {
@@ -1400,7 +1418,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
       __syncthreads();
       int32_t woff = 0, wtot = 0;
       for (int v2 = 0; v2 < nwaves; ++v2) {
-        const int32_t cw = wcw[v2];
+        const int32_t cw = uniform_i32(wcw[v2]);
         woff += v2 < wid ? cw : 0;
         wtot += cw;
       }
@@ -1417,34 +1435,48 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
           pa = AV ? aval[e] : 1ull;
         }
       }
-      int64_t pos0 = woff;
-      for (int32_t j = ja; j < jb; ++j) {
-        const int32_t xi = j * 64 + lane;
-        if (with_pres) {
-          const uint64_t m = pres64[j];
+      // this task's output run; slots are below the window's width, so 32-bit offsets from it
+      int32_t* const cc = ccol + rowbase;
+      uint64_t* const cv = cval + rowbase;
+      const int32_t c0 = (int32_t)w0;
+      int32_t pos0 = woff;
+      if (with_pres) {
+        for (int32_t j = ja; j < jb; ++j) {
+          const int32_t xi = j * 64 + lane;
+          const uint64_t m = uniform_u64(pres64[j]);
           if ((m >> lane) & 1ull) {
             const uint64_t v = vals[xi];
-            const int64_t slot = pos0 + __popcll(m & below);
-            if (write && slot < target) {
-              ccol[rowbase + slot] = (int32_t)(w0 + xi);
-              cval[rowbase + slot] = v;
+            const uint32_t slot = (uint32_t)(pos0 + lanes_below(m));
+            if ((int32_t)slot < target) {
+              cc[slot] = c0 + xi;
+              cv[slot] = v;
             }
             vals[xi] = 0ull;
           }
           if (lane == 0) pres64[j] = 0ull;
           pos0 += __popcll(m);
-        } else {
-          const uint64_t v = vals[xi];
-          const uint64_t m = __ballot(v != 0ull);
-          if (write && ((m >> lane) & 1ull)) {
-            const int64_t slot = pos0 + __popcll(m & below);
-            if (slot < target) {
-              ccol[rowbase + slot] = (int32_t)(w0 + xi);
-              cval[rowbase + slot] = v;
+        }
+      } else {
+        // written only when the nonzero slots number exactly `target`, so every slot is in range
+        for (int32_t j = ja; j < jb; j += kCompactBatch) {
+          uint64_t v[kCompactBatch];
+#pragma unroll
+          for (int u = 0; u < kCompactBatch; ++u) v[u] = vals[min(j + u, jb - 1) * 64 + lane];
+#pragma unroll
+          for (int u = 0; u < kCompactBatch; ++u) {
+            if (j + u < jb) {
+              const int32_t xi = (j + u) * 64 + lane;
+              const bool nz = v[u] != 0ull;
+              const uint64_t m = __ballot(nz);
+              if (write && nz) {
+                const uint32_t slot = (uint32_t)(pos0 + lanes_below(m));
+                __builtin_nontemporal_store(c0 + xi, &cc[slot]);
+                __builtin_nontemporal_store(v[u], &cv[slot]);
+              }
+              if (m) vals[xi] = 0ull;
+              pos0 += __popcll(m);
             }
           }
-          if (m) vals[xi] = 0ull;
-          pos0 += __popcll(m);
         }
       }
       // no barrier after a written window: the next task's first LDS writes (the traversal
