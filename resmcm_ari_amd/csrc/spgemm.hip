@@ -225,26 +225,29 @@ __device__ __forceinline__ int32_t block_exscan_i32(int32_t x, int32_t* red, int
   return off + incl - x;
 }
 
-__device__ __forceinline__ int64_t block_exscan_i64(int64_t x, int64_t* red, int64_t* total) {
-  const int lane = lane_id(), wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
+// The traversal's scan: one barrier.  Its caller guarantees that every wave passed a barrier
+// after the previous reads of red and passes another before the next write to it (the traversal
+// round's own barriers do both).
+__device__ __forceinline__ int64_t block_exscan_i64_once(int64_t x, int64_t* red, int64_t* total) {
+  const int lane = lane_id(), wid = wave_id(), nw = blockDim.x / kWave;
   int64_t incl = x;
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
     int64_t y = __shfl_up(incl, o);
     if (lane >= o) incl += y;
   }
-  __syncthreads();
   if (lane == kWave - 1) red[wid] = incl;
   __syncthreads();
   int64_t off = 0, tot = 0;
   for (int w = 0; w < nw; ++w) {
-    if (w < wid) off += red[w];
-    tot += red[w];
+    const int64_t r = uniform_i64(red[w]);
+    if (w < wid) off += r;
+    tot += r;
   }
-  __syncthreads();
   *total = tot;
   return off + incl - x;
 }
+
 
 // ------------------------------------------------------------------------------------------
 // Balanced product traversal for the workgroup-per-row tiers.  For one row of A, every entry e
@@ -340,7 +343,7 @@ __device__ __forceinline__ void trav_round(int64_t st, int64_t cnt, uint64_t a, 
     }
   }
   int64_t total;
-  const int64_t ex = block_exscan_i64(packed, L.red, &total);
+  const int64_t ex = block_exscan_i64_once(packed, L.red, &total);
 #pragma unroll
   for (int x = 0; x < kTiny; ++x) call(x < tn, tc[x], BV ? a * tv[x] : a);
   const int32_t nquarter = (int32_t)(total & 0xffff), nlong = (int32_t)((total >> 16) & 0xffff);
