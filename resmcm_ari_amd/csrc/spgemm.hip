@@ -1526,6 +1526,7 @@ __global__ void window_symbolic_kernel(const int32_t* __restrict__ acol, const i
   const int32_t R = 1 << sub;                   // numeric windows per symbolic window
   const int32_t nw32 = (1 << (log2w + sub)) / 32;
   const int32_t sub32 = (1 << log2w) / 32;      // bitmap words per numeric window
+  const int32_t lw = log2w - 5, wmask = sub32 - 1;
   const int32_t nwin_s = (nwin + R - 1) / R;
   __shared__ TravLds L;
   __shared__ int32_t claim;
@@ -1590,9 +1591,12 @@ __global__ void window_symbolic_kernel(const int32_t* __restrict__ acol, const i
           if (valid && (lane == kWave - 1 || nw != word)) atomicOr(&bm[word], bit);
         });
       } else {
+        // bit layout inside a numeric window: column x sets bit x >> lw of word x & (words - 1), so
+        // the consecutive columns of a dense B row fall in consecutive words (distinct banks)
+        // instead of sharing one word; a window's words stay contiguous for its count
         trav_round<false, 4, 2>(st, cnt, 1ull, bcol, nullptr, L, [&](int32_t c, uint64_t) {
           const int32_t x = (int32_t)(c - w0);
-          atomicOr(&bm[x >> 5], 1u << (x & 31));
+          atomicOr(&bm[((x >> log2w) << lw) | (x & wmask)], 1u << ((x >> lw) & 31));
         });
       }
     }
