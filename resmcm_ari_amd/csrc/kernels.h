@@ -149,8 +149,9 @@ struct SpgemmOpts {
                             // start (1), take the nonzero values as the columns (0), or choose by
                             // the expected window fill (2: presence below 1/8)
   int64_t sym_window_cols = 262144;  // heavy rows' symbolic pass: columns per window-major task
-  int sym_merge = 2;        // ... merges lanes sharing a bitmap word: 0 never, 1 always, 2 when the
-                            // heavy rows average at least 8 products per column
+  int sym_merge = 0;        // ... merges lanes sharing a bitmap word: 0 never (default: its bitmap
+                            // layout spreads a dense run of columns over the banks instead), 1
+                            // always, 2 when the heavy rows average at least 8 products per column
   int64_t window_min_products = 128;   // rows wider than the bitmap tier's LDS value slots use the
                                        // window kernel when they average at least this many
                                        // products per (row, window); otherwise the bitmap kernel
