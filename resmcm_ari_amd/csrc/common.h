@@ -19,7 +19,8 @@ struct Csr {
   int64_t* ptr = nullptr;
   int32_t* col = nullptr;
   uint64_t* val = nullptr;
-  bool sorted = true;  // every row ascending (non-strict)
+  bool sorted = true;     // every row ascending (non-strict)
+  bool distinct = false;  // no column repeated within a row (products; strictly sorted host input)
 };
 
 struct HipError : std::runtime_error {

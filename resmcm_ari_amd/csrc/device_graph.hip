@@ -206,6 +206,7 @@ void build_typed_csr(const int32_t* vtype, const int64_t* adj_ptr, const int32_t
   out.nrows = nrows;
   out.ncols = ncols;
   out.val = nullptr;
+  out.distinct = false;  // adjacency duplicates are kept
   out.ptr = al.alloc_n<int64_t>((int64_t)nrows + 1);
   const int wpb = kThreads / kWave;
   if (nrows == 0) {
@@ -240,6 +241,7 @@ void csr_gather_rows(const Csr& in, const int32_t* rows_sel, int32_t nsel, Csr& 
   out.nrows = nsel;
   out.ncols = in.ncols;
   out.sorted = in.sorted;
+  out.distinct = in.distinct;
   out.ptr = al.alloc_n<int64_t>((int64_t)nsel + 1);
   DevBuf<int64_t> len(al, nsel);
   if (nsel > 0) {

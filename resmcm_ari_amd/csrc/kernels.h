@@ -145,6 +145,8 @@ struct SpgemmOpts {
                                        // column bitmap (cols 0 = never)
   int64_t wave_bitmap_cap = 2048;      // LDS value slots per wave; wider rows accumulate in HBM
   int window_major = 1;     // window tier in window-major task order (0 = one row at a time)
+  int copy_tier = 1;        // rows of one entry over a right operand without repeated columns are
+                            // copied (scaled) from B when they have more than s_max products
   int window_presence = 2;  // window-major tasks track occupied columns with presence bits from the
                             // start (1), take the nonzero values as the columns (0), or choose by
                             // the expected window fill (2: presence below 1/8)

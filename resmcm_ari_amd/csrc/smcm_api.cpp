@@ -138,6 +138,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "sym_merge") {
       if (v < 0 || v > 2) fail(SMCM_EINVAL, "sym_merge must be 0, 1 or 2");
       o.sym_merge = (int)v;
+    } else if (k == "copy_tier") {
+      if (v != 0 && v != 1) fail(SMCM_EINVAL, "copy_tier must be 0 or 1");
+      o.copy_tier = (int)v;
     } else if (k == "sym_window_cols") {
       if (v < 64) fail(SMCM_EINVAL, "sym_window_cols must be >= 64");
       o.sym_window_cols = v;
@@ -226,6 +229,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "window_all_nwin") *v = o.window_all_nwin;
     else if (k == "sym_window_cols") *v = o.sym_window_cols;
     else if (k == "sym_merge") *v = o.sym_merge;
+    else if (k == "copy_tier") *v = o.copy_tier;
     else if (k == "window_presence") *v = o.window_presence;
     else if (k == "force_tier") *v = o.force_tier;
     else if (k == "exact_cost_sum") *v = ctx->exact_cost_sum;
@@ -586,7 +590,8 @@ int smcm_result_free(smcm_result* r) {
 namespace {
 // Host CSR operand of smcm_spgemm_csr: ptr[0] == 0, non-decreasing row pointers, a column array
 // whenever there are entries, columns in [0, ncols) and, for a right operand, ascending rows.
-void check_host_csr(const char* name, int32_t rows, int32_t ncols, const int64_t* ptr, const int32_t* col,
+// returns whether every row is strictly ascending (no column repeated within a row)
+bool check_host_csr(const char* name, int32_t rows, int32_t ncols, const int64_t* ptr, const int32_t* col,
                     bool sorted) {
   using smcm_host::fail;
   const std::string nm(name);
@@ -594,11 +599,16 @@ void check_host_csr(const char* name, int32_t rows, int32_t ncols, const int64_t
   for (int32_t r = 0; r < rows; ++r)
     if (ptr[r + 1] < ptr[r]) fail(SMCM_EINVAL, nm + ": row pointers decrease at row " + std::to_string(r));
   if (ptr[rows] > 0 && !col) fail(SMCM_EINVAL, nm + ": null column array");
+  bool strict = true;
   for (int32_t r = 0; r < rows; ++r)
     for (int64_t x = ptr[r]; x < ptr[r + 1]; ++x) {
       if (col[x] < 0 || col[x] >= ncols) fail(SMCM_EINVAL, nm + ": column out of range");
-      if (sorted && x > ptr[r] && col[x] < col[x - 1]) fail(SMCM_EINVAL, nm + ": rows must be sorted");
+      if (x > ptr[r] && col[x] <= col[x - 1]) {
+        strict = false;
+        if (sorted && col[x] < col[x - 1]) fail(SMCM_EINVAL, nm + ": rows must be sorted");
+      }
     }
+  return strict;
 }
 }  // namespace
 
@@ -614,7 +624,7 @@ int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_
     if (m < 0 || k < 0 || n < 0) fail(SMCM_EINVAL, "negative dimension");
     // every check runs before the first upload, so a malformed operand launches no kernel
     check_host_csr("A", m, k, a_ptr, a_col, false);
-    check_host_csr("B", k, n, b_ptr, b_col, true);
+    const bool b_distinct = check_host_csr("B", k, n, b_ptr, b_col, true);
     const int64_t annz = a_ptr[m], bnnz = b_ptr[k];
     DevAlloc& al = ctx->al;
     auto up = [&](const void* h, size_t bytes) {
@@ -635,6 +645,7 @@ int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_
     B.ptr = (int64_t*)up(b_ptr, sizeof(int64_t) * (k + 1));
     B.col = (int32_t*)up(b_col, sizeof(int32_t) * bnnz);
     B.val = b_val ? (uint64_t*)up(b_val, sizeof(uint64_t) * bnnz) : nullptr;
+    B.distinct = b_distinct;
     smcm::SpgemmStats ss;
     try {
       smcm::spgemm(A, B, nullptr, 0, C, ctx->opts, &ss, al, true, ctx->timer.enabled ? &ctx->timer : nullptr);

@@ -61,7 +61,7 @@ __global__ void entry_len_kernel(const int32_t* __restrict__ acol, int64_t nnz, 
   }
 }
 
-enum Tier { kEmpty = 0, kWaveTier = 1, kSortTier = 2, kBitmapTier = 3, kHeavyTier = 4, kNumBins = 5 };
+enum Tier { kEmpty = 0, kWaveTier = 1, kSortTier = 2, kBitmapTier = 3, kHeavyTier = 4, kCopyTier = 5, kNumBins = 6 };
 
 
 __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __restrict__ aptr, int32_t r0, int32_t r1,
@@ -69,7 +69,8 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
                                                             const double* __restrict__ est_r, int est_stride,
                                                             double ncols, int s_max, int m_max, int dense_div,
                                                             int64_t heavy_ub, int force_tier, int wave_bitmap_max_ub,
-                                                            int32_t* __restrict__ bins, int32_t* __restrict__ counts,
+                                                            int copy_ok, int32_t* __restrict__ bins,
+                                                            int32_t* __restrict__ counts,
                                                             unsigned long long* __restrict__ sums) {
   const int32_t u = r0 + (int32_t)(blockIdx.x * kThreads + threadIdx.x);
   int tier = -1;
@@ -79,6 +80,8 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
     d = aptr[u + 1] - aptr[u];
     if (ub == 0) {
       tier = kEmpty;
+    } else if (copy_ok && d == 1 && ub > s_max) {
+      tier = kCopyTier;  // one entry: the row is a scaled copy of one row of B
     } else if (force_tier >= 0) {
       tier = force_tier == 0 ? kWaveTier : force_tier == 1 ? kSortTier : kBitmapTier;
       if (tier == kWaveTier && ub > 64) tier = kSortTier;
@@ -1664,12 +1667,92 @@ __global__ void __launch_bounds__(kThreads) split_wide_kernel(const int32_t* __r
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// copy tier: a row of A with a single entry a (column k) whose B row repeats no column.  Row u of
+// C is then a * B[k] with B's columns in B's (ascending) order, so its width is |B[k]| (no
+// symbolic work) and its numeric pass is a streaming copy: the copy rows' outputs are laid end to
+// end and cut into kCopyChunk-element chunks, one workgroup per chunk, consecutive threads on
+// consecutive elements.
+constexpr int64_t kCopyChunk = 8192;
+
+__global__ void __launch_bounds__(kThreads) copy_width_kernel(const int32_t* __restrict__ rows, int32_t n,
+                                                              const int64_t* __restrict__ aptr,
+                                                              const int64_t* __restrict__ S,
+                                                              int64_t* __restrict__ row_nnz) {
+  for (int32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const int32_t u = rows[i];
+    row_nnz[u] = S[aptr[u] + 1] - S[aptr[u]];
+  }
+}
+
+// per copy row: its length, first B element, first output slot and A value
+__global__ void __launch_bounds__(kThreads) copy_prep_kernel(const int32_t* __restrict__ rows, int32_t n,
+                                                             const int64_t* __restrict__ aptr,
+                                                             const int32_t* __restrict__ acol,
+                                                             const uint64_t* __restrict__ aval,
+                                                             const int64_t* __restrict__ bptr,
+                                                             const int64_t* __restrict__ cptr,
+                                                             int64_t* __restrict__ len, int64_t* __restrict__ src,
+                                                             int64_t* __restrict__ dst, uint64_t* __restrict__ av) {
+  for (int32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const int32_t u = rows[i];
+    const int64_t e = aptr[u];
+    const int32_t k = acol[e];
+    len[i] = bptr[k + 1] - bptr[k];
+    src[i] = bptr[k];
+    dst[i] = cptr[u];
+    av[i] = aval ? aval[e] : 1ull;
+  }
+}
+
+// first copy row of every chunk: chunk c starts inside exactly one row (every copy row is non-empty)
+__global__ void __launch_bounds__(kThreads) copy_chunks_kernel(const int64_t* __restrict__ pre, int32_t n,
+                                                               int32_t* __restrict__ first) {
+  for (int32_t r = blockIdx.x * kThreads + threadIdx.x; r < n; r += gridDim.x * kThreads) {
+    const int64_t lo = pre[r], hi = pre[r + 1];
+    for (int64_t c = (lo + kCopyChunk - 1) / kCopyChunk; c * kCopyChunk < hi; ++c) first[c] = r;
+  }
+}
+
+template <bool BV>
+__global__ void __launch_bounds__(kThreads) copy_tier_kernel(const int64_t* __restrict__ pre, int64_t total,
+                                                             const int32_t* __restrict__ first,
+                                                             const int64_t* __restrict__ src,
+                                                             const int64_t* __restrict__ dst,
+                                                             const uint64_t* __restrict__ av,
+                                                             const int32_t* __restrict__ bcol,
+                                                             const uint64_t* __restrict__ bval,
+                                                             int32_t* __restrict__ ccol, uint64_t* __restrict__ cval) {
+  const int64_t nch = (total + kCopyChunk - 1) / kCopyChunk;
+  for (int64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    int32_t r = first[c];
+    int64_t lo = pre[r], hi = pre[r + 1];
+    int64_t s0 = src[r] - lo, d0 = dst[r] - lo;  // element i of the run: B[s0 + i] -> C[d0 + i]
+    uint64_t a = av[r];
+    const int64_t end = min((c + 1) * kCopyChunk, total);
+    for (int64_t i = c * kCopyChunk + threadIdx.x; i < end; i += kThreads) {
+      while (i >= hi) {  // the next copy row of the chunk
+        ++r;
+        lo = hi;
+        hi = pre[r + 1];
+        s0 = src[r] - lo;
+        d0 = dst[r] - lo;
+        a = av[r];
+      }
+      const int32_t col = bcol[s0 + i];
+      const uint64_t v = BV ? a * bval[s0 + i] : a;
+      __builtin_nontemporal_store(col, &ccol[d0 + i]);
+      __builtin_nontemporal_store(v, &cval[d0 + i]);
+    }
+  }
+}
+
 // per-tier accounting of one pass: rows, A entries, products, output entries
 struct TierCounts {
-  int32_t rows[kNumBins] = {0, 0, 0, 0, 0};
-  int64_t a_nnz[kNumBins] = {0, 0, 0, 0, 0};
-  int64_t products[kNumBins] = {0, 0, 0, 0, 0};
-  int64_t c_nnz[kNumBins] = {0, 0, 0, 0, 0};
+  int32_t rows[kNumBins] = {};
+  int64_t a_nnz[kNumBins] = {};
+  int64_t products[kNumBins] = {};
+  int64_t c_nnz[kNumBins] = {};
 };
 
 // scratch of the numeric bitmap/window split (device)
@@ -1730,6 +1813,34 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
                                                              cval);
     SMCM_LAUNCH_CHECK();
     T.stop(tok, st, tier_bytes(NUMERIC, tc, kWaveTier, AV, BV));
+  }
+  const int32_t nC = tc.rows[kCopyTier];
+  if (nC > 0) {
+    const int32_t* crow = bins + (size_t)kCopyTier * m;
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(nC, kThreads), 65536);
+    if constexpr (!NUMERIC) {
+      const int tok = T.start("spgemm.symbolic.copy", st);
+      copy_width_kernel<<<g, kThreads, 0, st>>>(crow, nC, A.ptr, S, row_nnz);
+      SMCM_LAUNCH_CHECK();
+      T.stop(tok, st, 16.0 * (nC + 1) + 20.0 * (double)tc.a_nnz[kCopyTier]);
+    } else {
+      DevAlloc& al = *split->al;
+      const int64_t total = tc.products[kCopyTier];
+      DevBuf<int64_t> len(al, nC), pre(al, (int64_t)nC + 1), src(al, nC), dst(al, nC);
+      DevBuf<uint64_t> av(al, nC);
+      const int64_t nch = ceil_div(total, kCopyChunk);
+      DevBuf<int32_t> first(al, std::max<int64_t>(nch, 1));
+      const int tok = T.start("spgemm.numeric.copy", st);
+      copy_prep_kernel<<<g, kThreads, 0, st>>>(crow, nC, A.ptr, A.col, A.val, B.ptr, cptr, len.p, src.p, dst.p, av.p);
+      SMCM_LAUNCH_CHECK();
+      scan_exclusive_i64(len.p, pre.p, nC, al);
+      copy_chunks_kernel<<<g, kThreads, 0, st>>>(pre.p, nC, first.p);
+      SMCM_LAUNCH_CHECK();
+      const unsigned gc = (unsigned)std::min<int64_t>(std::max<int64_t>(nch, 1), (int64_t)o.cus * 8);
+      copy_tier_kernel<BV><<<gc, kThreads, 0, st>>>(pre.p, total, first.p, src.p, dst.p, av.p, B.col, B.val, ccol, cval);
+      SMCM_LAUNCH_CHECK();
+      T.stop(tok, st, tier_bytes(true, tc, kCopyTier, AV, BV));
+    }
   }
   if (nS > 0 && wave_bitmap_for(o, B.ncols)) {
     const WaveBitmapLayout wl = WaveBitmapLayout::make(B.ncols, (int32_t)o.wave_bitmap_cap, NUMERIC);
@@ -2081,7 +2192,8 @@ struct SpgemmCore {
     if (r1 > r0) {
       bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, kThreads), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
-          o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_max_ub : 0, bins.p, counts, sums);
+          o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_max_ub : 0,
+          (B.distinct && o.copy_tier && o.force_tier < 0) ? 1 : 0, bins.p, counts, sums);
       SMCM_LAUNCH_CHECK();
     }
     unsigned char* hs = host_stage() + kStageBin;
@@ -2141,7 +2253,7 @@ struct SpgemmCore {
   void add_stats(SpgemmStats* st) const {
     if (!st) return;
     st->rows[0] += tc.rows[kEmpty];
-    st->rows[1] += tc.rows[kWaveTier];
+    st->rows[1] += tc.rows[kWaveTier] + tc.rows[kCopyTier];
     st->rows[2] += tc.rows[kSortTier];
     st->rows[3] += tc.rows[kBitmapTier] + tc.rows[kHeavyTier];
   }
@@ -2186,6 +2298,7 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
   C.nrows = m;
   C.ncols = B.ncols;
   C.sorted = true;
+  C.distinct = true;
   C.ptr = al.alloc_n<int64_t>((int64_t)m + 1);
   SpgemmCore core(A, B, est_r, est_stride, o, al, tm);
   core.bin(0, m);
@@ -2292,6 +2405,7 @@ void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_str
     blk.ncols = B.ncols;
     blk.nnz = cnt;
     blk.sorted = true;
+    blk.distinct = true;
     core.bin(b, e);
     auto t1 = std::chrono::steady_clock::now();
     core.numeric(ptr.p, bcol.p - p0, bval.p - p0);

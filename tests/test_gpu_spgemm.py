@@ -124,6 +124,25 @@ def test_spgemm_duplicates_in_A(ctx):
         assert_same_csr(ptr, col, val, ref, what=f"dup tier={tier}")
 
 
+@pytest.mark.parametrize("copy", [0, 1])
+@pytest.mark.parametrize("vals", ["none", "A", "B", "AB"])
+@pytest.mark.parametrize("bdup", [False, True])
+def test_copy_tier(ctx, vals, copy, bdup):
+    """Rows of A with a single entry over a right operand whose rows repeat no column: a scaled
+    copy of one row of B (the copy tier), mixed with longer rows; B rows from empty to 20000
+    columns, so a copy chunk spans many rows or a part of one.  With repeated columns in B the
+    host check leaves the copy tier off."""
+    rng = np.random.default_rng(zlib.crc32(f"copy/{vals}/{copy}/{bdup}".encode()))
+    A = random_csr(rng, 3000, 400, row_len=lambda r: 1 if r.random() < 0.7 else r.integers(0, 6),
+                   with_vals=vals in ("A", "AB"))
+    B = random_csr(rng, 400, 40000, row_len=lambda r: int(r.zipf(1.2)) % 20000, with_vals=vals in ("B", "AB"),
+                   dup=bdup)
+    ref = O.spgemm(A, B)
+    with options(ctx, copy_tier=copy):
+        ptr, col, val = _run(ctx, A, B)
+    assert_same_csr(ptr, col, val, ref, what=f"copy={copy} vals={vals} bdup={bdup}")
+
+
 def test_spgemm_empty(ctx):
     A = O.CSR(3, 4, np.zeros(4, np.int64), np.zeros(0, np.int32))
     B = O.CSR(4, 5, np.zeros(5, np.int64), np.zeros(0, np.int32))
