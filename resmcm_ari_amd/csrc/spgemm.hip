@@ -1402,18 +1402,23 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
       const int32_t nwords = (int32_t)((wend - w0 + 63) / 64);
       const int32_t per = (nwords + nwaves - 1) / nwaves;
       const int32_t ja = min(wid * per, nwords), jb = min(ja + per, nwords);
-      // the words are taken kCompactBatch at a time so that their LDS reads are in flight together
-      // (a read of a word past the wave's run is clamped to its last word and not used)
+      // presence bits: the wave's run of at most 64 words is read at once, a lane per word, and
+      // kept for the write pass.  Values alone: the words are taken kCompactBatch at a time so that
+      // their LDS reads are in flight together (a read past the wave's run is clamped to its last
+      // word and not used).
       int32_t mine = 0;
-      for (int32_t j = ja; j < jb; j += kCompactBatch) {
-        uint64_t m[kCompactBatch];
+      uint64_t pm = 0ull;  // presence mode: bits of word ja + lane
+      if (with_pres) {
+        if (ja + lane < jb) pm = pres64[ja + lane];
+        mine = (int32_t)wave_sum_i64(__popcll(pm));
+      } else {
+        for (int32_t j = ja; j < jb; j += kCompactBatch) {
+          uint64_t m[kCompactBatch];
 #pragma unroll
-        for (int u = 0; u < kCompactBatch; ++u) {
-          const int32_t jj = min(j + u, jb - 1);
-          m[u] = with_pres ? uniform_u64(pres64[jj]) : __ballot(vals[jj * 64 + lane] != 0ull);
+          for (int u = 0; u < kCompactBatch; ++u) m[u] = __ballot(vals[min(j + u, jb - 1) * 64 + lane] != 0ull);
+#pragma unroll
+          for (int u = 0; u < kCompactBatch; ++u) mine += j + u < jb ? __popcll(m[u]) : 0;
         }
-#pragma unroll
-        for (int u = 0; u < kCompactBatch; ++u) mine += j + u < jb ? __popcll(m[u]) : 0;
       }
       if (lane == 0) wcw[wid] = mine;
       if (threadIdx.x == 0) {
@@ -1447,19 +1452,21 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
       const int32_t c0 = (int32_t)w0;
       int32_t pos0 = woff;
       if (with_pres) {
-        for (int32_t j = ja; j < jb; ++j) {
-          const int32_t xi = j * 64 + lane;
-          const uint64_t m = uniform_u64(pres64[j]);
+        // occupied words only, in column order
+        for (uint64_t nzw = __ballot(pm != 0ull); nzw; nzw &= nzw - 1) {
+          const int i = __ffsll((long long)nzw) - 1;
+          const uint64_t m = readlane_u64(pm, i);
+          const int32_t xi = (ja + i) * 64 + lane;
           if ((m >> lane) & 1ull) {
             const uint64_t v = vals[xi];
             const uint32_t slot = (uint32_t)(pos0 + lanes_below(m));
             if ((int32_t)slot < target) {
-              cc[slot] = c0 + xi;
-              cv[slot] = v;
+              __builtin_nontemporal_store(c0 + xi, &cc[slot]);
+              __builtin_nontemporal_store(v, &cv[slot]);
             }
             vals[xi] = 0ull;
           }
-          if (lane == 0) pres64[j] = 0ull;
+          if (lane == 0) pres64[ja + i] = 0ull;
           pos0 += __popcll(m);
         }
       } else {
@@ -1988,6 +1995,9 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
         // slots); dense windows are cheaper with values alone (no per-product bit update)
         const double fill = (double)cls.c_nnz[1] / ((double)cnt[1] * nwin * (double)split->wl.w);
         const bool pres = o.window_presence == 1 || (o.window_presence == 2 && fill < 0.125);
+        // the compaction's presence pass reads a wave's run of words with one lane each
+        if ((split->wl.w / 64 + o.w_threads / kWave - 1) / (o.w_threads / kWave) > kWave)
+          throw HipError(-22, "window tier: more than 64 words per wave in a window");
         auto kern = pres ? window_major_kernel<AV, BV, true> : window_major_kernel<AV, BV, false>;
         const unsigned gm = persistent_grid(o, occupancy((const void*)kern, o.w_threads, lds));
         if (!split->nseg_tried) {
