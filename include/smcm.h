@@ -246,7 +246,8 @@ typedef struct smcm_query_out {
   int32_t status;                   /* SMCM_OK or the error of the query */
   int32_t cache_k;                  /* leading positions 0..cache_k served from the cache (0: none) */
   int32_t blocks;                   /* > 0: run as this many independent blocks of source rows */
-  int32_t pad;
+  int32_t reruns;                   /* times the query ran again by itself after running out of
+                                       device memory next to other queries */
   char error[160];                  /* message when status != SMCM_OK */
 } smcm_query_out;
 /* graphs: uploaded to ctx; opts NULL = 6 slots (2 of them for large queries, above 4 GiB

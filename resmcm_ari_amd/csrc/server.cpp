@@ -41,8 +41,9 @@ struct Query : smcm_sched::Job {
   int32_t L = 0;
   std::vector<int32_t> vts, ets, src;  // src: global ids (empty + whole: all rows)
   bool whole = false;
+  bool alone = false;          // admitted by itself (a rerun after running out of device memory)
   uint64_t* digest = nullptr;  // caller's buffer (rows of the result), may be null
-  smcm_query_out out{};        // nnz, predicted bytes, cache use, blocks
+  smcm_query_out out{};        // nnz, predicted bytes, cache use, blocks, reruns
 };
 
 struct CacheEntry {
@@ -278,7 +279,8 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
     Query* q = nullptr;
     void admit(uint64_t b, Query& qq) {
       q = &qq;
-      sc.admit(qq, b);
+      if (qq.alone) sc.admit_alone(qq);
+      else sc.admit(qq, b);
       qq.out.predicted_bytes = b;
     }
     ~Admission() {
@@ -365,7 +367,22 @@ bool run_job(smcm_server* s, smcm_sched::Job& job, int wi, bool small_lane) {
   bool finished = true;
   try {
     SMCM_HIP(hipSetDevice(w->device));
-    finished = run_query(s, w, q, small_lane);
+    try {
+      finished = run_query(s, w, q, small_lane);
+    } catch (const HipError& e) {
+      // Admission counts predicted bytes, so queries admitted together can still outgrow the
+      // device.  A query that ran out of memory next to others runs once more by itself, after
+      // its worker's cached blocks went back to the device pool (its buffers were freed while
+      // the exception unwound).
+      if (e.code != (int)hipErrorOutOfMemory || q.alone) throw;
+      (void)hipGetLastError();
+      q.alone = true;
+      q.out.reruns++;
+      w->al.release_cache();
+      SMCM_HIP(hipStreamSynchronize(w->stream));
+      smcm::DevAlloc::trim_pool();
+      finished = run_query(s, w, q, small_lane);
+    }
   } catch (const ApiError& e) {
     status = e.code;
     msg = e.msg;
@@ -438,9 +455,10 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
     if (o.plan_cost != 0 && o.plan_cost != 1) fail(SMCM_EINVAL, "plan_cost must be 0 or 1");
     if (o.block_bytes < (1ull << 20)) fail(SMCM_EINVAL, "block_bytes too small");
     s->opts = o;
-    // the workers allocate on their own streams: blocks the main context keeps cached for reuse
-    // (it stays idle while the server runs) go back to the device pool, or they would be out of
-    // the workers' reach
+    // the workers allocate on their own streams: blocks the main context and its batch workers
+    // keep cached for reuse (they stay idle while the server runs) go back to the device pool, or
+    // they would be out of the workers' reach
+    if (ctx->al.on_oom) ctx->al.on_oom();  // the batch workers' caches
     ctx->al.release_cache();
     SMCM_HIP(hipStreamSynchronize(ctx->stream));
     smcm::DevAlloc::trim_pool();

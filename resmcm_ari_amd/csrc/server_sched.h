@@ -7,7 +7,9 @@
 // worker is a large-lane worker).  Every worker takes submitted jobs in arrival order; a
 // small-lane worker that finds a job large hands it to the large lane (hand_over), whose workers
 // take handed-over jobs first.  A job is admitted when its predicted bytes fit the budget next
-// to the admitted jobs, or when no other job is admitted (alone it always runs).
+// to the admitted jobs, or when no other job is admitted (alone it always runs).  A job can also
+// ask to be admitted alone (admit_alone: after it ran out of device memory next to others); no
+// other job is admitted while it waits or runs.
 //
 // Shutdown (stop) lets the workers finish every submitted job.  A small-lane worker exits once
 // no job is waiting; a large-lane worker only once, in addition, no small-lane worker is busy,
@@ -104,17 +106,25 @@ class Scheduler {
   }
 
   // blocks until `bytes` fit the budget next to the admitted jobs (or nothing else is admitted);
-  // every job counts at least one byte, so a budget of b bytes admits at most b jobs at once
+  // every job counts at least one byte, so a budget of b bytes admits at most b jobs at once.
+  // No job is admitted while a job waits for admit_alone().
   void admit(Job& j, uint64_t bytes) {
     bytes = bytes > 0 ? bytes : 1;
     std::unique_lock<std::mutex> lk(mu_);
-    cv_budget_.wait(lk, [&] { return admitted_ == 0 || inflight_ + bytes <= budget_; });
-    inflight_ += bytes;
-    admitted_++;
-    if (admitted_ > 1 && inflight_ > budget_) over_budget_++;  // never: checked by the stress test
-    j.admitted_bytes = bytes;
-    j.admitted = true;
-    j.t_start = now();
+    cv_budget_.wait(lk, [&] { return alone_waiting_ == 0 && (admitted_ == 0 || inflight_ + bytes <= budget_); });
+    take(j, bytes);
+  }
+
+  // blocks until no other job is admitted, then admits j with the whole budget, so nothing else
+  // is admitted until j is released: a job that ran out of device memory next to others runs
+  // again by itself.  Jobs waiting here are admitted one at a time, before any admit() waiter.
+  void admit_alone(Job& j) {
+    std::unique_lock<std::mutex> lk(mu_);
+    alone_waiting_++;
+    cv_budget_.wait(lk, [&] { return admitted_ == 0; });
+    alone_waiting_--;
+    take(j, budget_ > 0 ? budget_ : 1);
+    alone_runs_++;
   }
 
   // the job's device work is done: its budget is freed
@@ -146,19 +156,29 @@ class Scheduler {
 
   struct Counters {
     uint64_t inflight;
-    int admitted, running, small_busy;
-    int64_t completed, over_budget;
+    int admitted, running, small_busy, alone_waiting;
+    int64_t completed, over_budget, alone_runs;
     size_t pending, large_ready, tickets;
   };
   Counters counters() {
     std::lock_guard<std::mutex> lk(mu_);
-    return Counters{inflight_, admitted_, running_, small_busy_, completed_, over_budget_,
-                    pending_.size(), large_ready_.size(), all_.size()};
+    return Counters{inflight_, admitted_, running_, small_busy_, alone_waiting_, completed_, over_budget_,
+                    alone_runs_, pending_.size(), large_ready_.size(), all_.size()};
   }
 
  private:
   static double clock_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  }
+
+  // caller holds mu_
+  void take(Job& j, uint64_t bytes) {
+    inflight_ += bytes;
+    admitted_++;
+    if (admitted_ > 1 && inflight_ > budget_) over_budget_++;  // never: checked by the stress test
+    j.admitted_bytes = bytes;
+    j.admitted = true;
+    j.t_start = now();
   }
 
   void worker(int wi) {
@@ -215,7 +235,8 @@ class Scheduler {
   int admitted_ = 0;
   int running_ = 0;
   int small_busy_ = 0;
-  int64_t completed_ = 0, over_budget_ = 0;
+  int alone_waiting_ = 0;  // jobs blocked in admit_alone()
+  int64_t completed_ = 0, over_budget_ = 0, alone_runs_ = 0;
   bool stop_ = false;
 };
 

@@ -73,7 +73,7 @@ class ServerOpts(C.Structure):
 class QueryOut(C.Structure):
     _fields_ = [("nnz", C.c_int64), ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
                 ("predicted_bytes", C.c_uint64), ("status", C.c_int32), ("cache_k", C.c_int32),
-                ("blocks", C.c_int32), ("pad", C.c_int32), ("error", C.c_char * 160)]
+                ("blocks", C.c_int32), ("reruns", C.c_int32), ("error", C.c_char * 160)]
 
 
 class SmcmError(RuntimeError):

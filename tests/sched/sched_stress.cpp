@@ -12,8 +12,11 @@
 //           longer than it executes, which makes that window wide;
 //   tight : a budget that admits about one job at a time;
 //   nolane: no lanes (every worker takes everything).
+// In every scenario some jobs "run out of memory" once they are admitted and run again through
+// admit_alone(), as the server reruns a query that ran out of device memory next to others.
 // Checks: every job finishes exactly once with its status, no admission beyond the budget
-// (unless alone), handed-over jobs run on large-lane workers, nothing left queued.  A watchdog
+// (unless alone), nothing admitted next to a job admitted alone, handed-over jobs run on
+// large-lane workers, nothing left queued.  A watchdog
 // ends the process with exit status 3 and the scheduler's counters if a scenario hangs.
 #include <atomic>
 #include <cstdio>
@@ -34,7 +37,7 @@ namespace {
 struct StubJob : Job {
   int plan_us = 0, exec_us = 0;
   uint64_t bytes = 0;
-  bool big = false, fail = false;
+  bool big = false, fail = false, oom = false;
   std::atomic<int> runs{0};
   int ran_on = -1;
 };
@@ -72,7 +75,7 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     g_current = &sc;
   }
   const int small_lanes = sc.small_lanes();
-  std::atomic<int64_t> over{0};
+  std::atomic<int64_t> over{0}, not_alone{0};
   sc.start([&](Job& j, int wi, bool small) {
     auto& q = static_cast<StubJob&>(j);
     q.runs++;
@@ -95,6 +98,13 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     if (budget == 1 && c.admitted > 1) over++;  // a one-byte budget admits one job at a time
     sleep_us(q.exec_us);  // device work
     sc.release(q);
+    if (q.oom) {  // ran out of memory: once more, by itself
+      sc.admit_alone(q);
+      if (sc.counters().admitted != 1) not_alone++;
+      sleep_us(q.exec_us);
+      if (sc.counters().admitted != 1) not_alone++;
+      sc.release(q);
+    }
     q.status = 0;
     return true;
   });
@@ -102,6 +112,7 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
   std::uniform_real_distribution<double> u(0.0, 1.0);
   std::vector<int64_t> tickets;
   std::vector<bool> want_fail;
+  int64_t n_oom = 0;
   std::map<int64_t, std::unique_ptr<Job>> early;  // finished jobs taken by a non-blocking poll
   for (int i = 0; i < njobs; ++i) {
     auto j = std::make_unique<StubJob>();
@@ -110,6 +121,8 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     j->big = u(rng) < p_big;
     j->bytes = u(rng) < 0.1 ? 0 : (uint64_t)(u(rng) * (double)budget * (j->big ? 0.9 : 0.3));
     j->fail = u(rng) < 0.03;
+    j->oom = !j->fail && u(rng) < 0.08;
+    n_oom += j->oom ? 1 : 0;
     want_fail.push_back(j->fail);
     tickets.push_back(sc.submit(std::move(j)));
     if (!stop_early) {
@@ -155,6 +168,9 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
   const auto c = sc.counters();
   CHECK(c.over_budget == 0 && over == 0, "admitted beyond the budget (%lld, %lld)", (long long)c.over_budget,
         (long long)over.load());
+  CHECK(not_alone == 0 && c.alone_runs == n_oom && c.alone_waiting == 0,
+        "alone admissions: %lld shared, %lld of %lld run, %d waiting", (long long)not_alone.load(),
+        (long long)c.alone_runs, (long long)n_oom, c.alone_waiting);
   CHECK(c.inflight == 0 && c.admitted == 0 && c.running == 0 && c.small_busy == 0,
         "left: inflight %llu admitted %d running %d small_busy %d", (unsigned long long)c.inflight, c.admitted,
         c.running, c.small_busy);
