@@ -78,7 +78,9 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
  * "lds_bytes", "plan_cost" (the DP's cost: 0 the reference's estimated sizes, 1 estimated
  * products, 2-5 the reference's other providers MetaAC / MNC / DMap / LGraph, see
  * smcm_cost_estimate), "lgraph_seed", "window_major", "window_min_products", "window_cols",
- * "w_threads", "window_all_nwin", "window_presence", "sym_window_cols", "sym_merge", "budget" (device
+ * "w_threads", "window_all_nwin", "window_presence", "sym_window_cols", "sym_merge", "dense_cells"
+ * (products with rows x columns up to this many cells accumulate whole in a dense device array,
+ * default 2^23, 0 = never), "budget" (device
  * bytes the product-aware plan may assume: products whose estimated operands and output exceed it
  * are not planned when another plan avoids them). */
 int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);

@@ -29,6 +29,7 @@ HIP_SOURCES = [
     "device_graph.hip",
     "estimator.hip",
     "spgemm.hip",
+    "dense.hip",
     "scan.hip",
     "batch.hip",
     "providers.hip",

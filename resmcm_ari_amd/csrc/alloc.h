@@ -38,6 +38,38 @@ struct DevAlloc {
   // called when an allocation fails, before this allocator's own cache is released: frees what
   // idle allocators tied to this one (a context's batch workers) keep cached
   std::function<void()> on_oom;
+  // A block kept all-zero between uses (the dense SpGEMM tier's accumulators): a user returns
+  // every byte it changed to zero, and marks the block dirty while it may not have; a dirty block
+  // is cleared before its next use.  Freed with the cache.
+  void* zero_block = nullptr;
+  size_t zero_bytes = 0;
+  bool zero_dirty = false;
+
+  // hipMallocAsync on this stream; on failure the cache is released and the pool trimmed once
+  void* raw_alloc(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMallocAsync(&p, bytes, stream);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      if (on_oom) on_oom();
+      release_cache();
+      (void)hipStreamSynchronize(stream);
+      trim_pool();
+      e = hipMallocAsync(&p, bytes, stream);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw HipError((int)e, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                 hipGetErrorString(e));
+    }
+    return p;
+  }
+  void free_zero_block() {
+    if (zero_block) (void)hipFreeAsync(zero_block, stream);
+    zero_block = nullptr;
+    zero_bytes = 0;
+    zero_dirty = false;
+  }
 
   void* alloc(size_t bytes) {
     bytes = (bytes + 255) / 256 * 256;
@@ -63,23 +95,7 @@ struct DevAlloc {
         cache.erase(it);
       }
     }
-    if (!p) {
-      hipError_t e = hipMallocAsync(&p, bytes, stream);
-      if (e == hipErrorOutOfMemory) {
-        (void)hipGetLastError();
-        if (on_oom) on_oom();
-        release_cache();
-        (void)hipStreamSynchronize(stream);
-        trim_pool();
-        e = hipMallocAsync(&p, bytes, stream);
-      }
-
-      if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw HipError((int)e, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
-                                   hipGetErrorString(e));
-      }
-    }
+    if (!p) p = raw_alloc(bytes);
     sizes[p] = got;
     live_bytes += got;
     if (live_bytes > peak_bytes) peak_bytes = live_bytes;
@@ -120,6 +136,7 @@ struct DevAlloc {
     for (auto& kv : cache) (void)hipFreeAsync(kv.second, stream);
     cache.clear();
     cached_bytes = 0;
+    free_zero_block();
   }
 };
 

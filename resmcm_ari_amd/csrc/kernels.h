@@ -160,6 +160,8 @@ struct SpgemmOpts {
                                        // accumulates their values in HBM (device atomics)
   int64_t window_all_nwin = 4;  // products spanning 2..this many windows send every bitmap-tier row
                                 // to the window kernel (LDS value slots are bypassed)
+  int64_t dense_cells = (int64_t)1 << 23;  // products with rows(A) x cols(B) up to this many cells
+                                           // accumulate whole in a dense device array (0 = never)
 };
 
 struct SpgemmStats {
@@ -178,6 +180,12 @@ struct SpgemmStats {
 void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr& C,
             const SpgemmOpts& o, SpgemmStats* st, DevAlloc& al, bool timing = false,
             KernelTimer* tm = nullptr);
+
+// Dense tier (dense.hip): C = A * B accumulated whole in a dense all-zero device array when
+// rows(A) x cols(B) <= o.dense_cells (and no tier is forced).  Returns false, having done
+// nothing, when the product does not qualify; C.ptr must be allocated (nrows + 1).
+bool spgemm_dense(const Csr& A, const Csr& B, Csr& C, const SpgemmOpts& o, SpgemmStats* st, DevAlloc& al,
+                  KernelTimer* tm);
 
 // Row-blocked C = A * B for outputs larger than memory: the symbolic pass runs once, then
 // rows are cut into blocks of at most block_bytes of output (a single row is always a block);

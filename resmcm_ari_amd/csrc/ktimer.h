@@ -51,6 +51,10 @@ struct KernelTimer {
     pending[tok].bytes = bytes;
     SMCM_HIP(hipEventRecord(pending[tok].b, st));
   }
+  // algorithmic bytes of a timed launch known only after it was enqueued (before the flush)
+  void set_bytes(int tok, double bytes) {
+    if (tok >= 0) pending[tok].bytes = bytes;
+  }
   // call after the stream has been synchronised
   void flush() {
     for (auto& r : pending) {

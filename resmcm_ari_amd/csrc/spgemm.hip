@@ -64,6 +64,11 @@ __global__ void entry_len_kernel(const int32_t* __restrict__ acol, int64_t nnz, 
 enum Tier { kEmpty = 0, kWaveTier = 1, kSortTier = 2, kBitmapTier = 3, kHeavyTier = 4, kCopyTier = 5, kNumBins = 6 };
 
 
+constexpr int kBinRowsPerThread = 4;  // a block bins kBinRowsPerThread * kThreads rows
+
+// Tier of every row; rows are appended to their tier's list.  Counts and sums are aggregated per
+// wave, then per block in LDS, so each block makes one global atomic per tier and counter: the
+// same few counters updated by every wave would serialise the launch.
 __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __restrict__ aptr, int32_t r0, int32_t r1,
                                                             int32_t m, const int64_t* __restrict__ S,
                                                             const double* __restrict__ est_r, int est_stride,
@@ -72,48 +77,75 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
                                                             int copy_ok, int32_t* __restrict__ bins,
                                                             int32_t* __restrict__ counts,
                                                             unsigned long long* __restrict__ sums) {
-  const int32_t u = r0 + (int32_t)(blockIdx.x * kThreads + threadIdx.x);
-  int tier = -1;
-  int64_t ub = 0, d = 0;
-  if (u < r1) {
-    ub = S[aptr[u + 1]] - S[aptr[u]];
-    d = aptr[u + 1] - aptr[u];
-    if (ub == 0) {
-      tier = kEmpty;
-    } else if (copy_ok && d == 1 && ub > s_max) {
-      tier = kCopyTier;  // one entry: the row is a scaled copy of one row of B
-    } else if (force_tier >= 0) {
-      tier = force_tier == 0 ? kWaveTier : force_tier == 1 ? kSortTier : kBitmapTier;
-      if (tier == kWaveTier && ub > 64) tier = kSortTier;
-      if (tier == kSortTier && ub > (wave_bitmap_max_ub ? wave_bitmap_max_ub : m_max)) tier = kBitmapTier;
-    } else if (ub <= s_max) {
-      tier = kWaveTier;
-    } else if (wave_bitmap_max_ub) {
-      tier = ub <= wave_bitmap_max_ub ? kSortTier : kBitmapTier;
-    } else {
-      bool dense = false;
-      if (est_r) dense = est_r[(size_t)u * est_stride] * ncols * (double)dense_div >= ncols;
-      tier = (ub <= m_max && !dense) ? kSortTier : kBitmapTier;
-    }
-    if (tier == kBitmapTier && ub > heavy_ub) tier = kHeavyTier;
-  }
-  // wave-aggregated append per bin (empty rows are only counted); per-tier sums of A entries
-  // and products feed the algorithmic-byte accounting
+  __shared__ int32_t lcnt[kNumBins], lbase[kNumBins];
+  __shared__ unsigned long long lsum[2 * kNumBins];
+  if (threadIdx.x < kNumBins) lcnt[threadIdx.x] = 0;
+  if (threadIdx.x < 2 * kNumBins) lsum[threadIdx.x] = 0;
+  __syncthreads();
   const int lane = lane_id();
-  for (int t = 0; t < kNumBins; ++t) {
-    const uint64_t mask = __ballot(tier == t);
-    if (!mask) continue;
-    const int leader = __ffsll((long long)mask) - 1;
-    int32_t base = 0;
-    const int64_t sd = wave_sum_i64(tier == t ? d : 0);
-    const int64_t su = wave_sum_i64(tier == t ? ub : 0);
-    if (lane == leader) {
-      base = atomicAdd(&counts[t], (int32_t)__popcll(mask));
-      atomicAdd(&sums[2 * t], (unsigned long long)sd);
-      atomicAdd(&sums[2 * t + 1], (unsigned long long)su);
+  int tiers[kBinRowsPerThread], slot[kBinRowsPerThread];
+#pragma unroll
+  for (int i = 0; i < kBinRowsPerThread; ++i) {
+    const int64_t uu = (int64_t)r0 + ((int64_t)blockIdx.x * kBinRowsPerThread + i) * kThreads + threadIdx.x;
+    const int32_t u = (int32_t)(uu < r1 ? uu : r1);
+    int tier = -1;
+    int64_t ub = 0, d = 0;
+    if (uu < r1) {
+      ub = S[aptr[u + 1]] - S[aptr[u]];
+      d = aptr[u + 1] - aptr[u];
+      if (ub == 0) {
+        tier = kEmpty;
+      } else if (copy_ok && d == 1 && ub > s_max) {
+        tier = kCopyTier;  // one entry: the row is a scaled copy of one row of B
+      } else if (force_tier >= 0) {
+        tier = force_tier == 0 ? kWaveTier : force_tier == 1 ? kSortTier : kBitmapTier;
+        if (tier == kWaveTier && ub > 64) tier = kSortTier;
+        if (tier == kSortTier && ub > (wave_bitmap_max_ub ? wave_bitmap_max_ub : m_max)) tier = kBitmapTier;
+      } else if (ub <= s_max) {
+        tier = kWaveTier;
+      } else if (wave_bitmap_max_ub) {
+        tier = ub <= wave_bitmap_max_ub ? kSortTier : kBitmapTier;
+      } else {
+        bool dense = false;
+        if (est_r) dense = est_r[(size_t)u * est_stride] * ncols * (double)dense_div >= ncols;
+        tier = (ub <= m_max && !dense) ? kSortTier : kBitmapTier;
+      }
+      if (tier == kBitmapTier && ub > heavy_ub) tier = kHeavyTier;
     }
-    base = __shfl(base, leader);
-    if (tier == t && t != kEmpty) bins[(size_t)t * m + base + lanes_below(mask)] = u;
+    tiers[i] = tier;
+    slot[i] = 0;
+    // wave-aggregated: a slot in the block's part of the tier list; per-tier sums of A entries
+    // and products feed the algorithmic-byte accounting
+    for (int t = 0; t < kNumBins; ++t) {
+      const uint64_t mask = __ballot(tier == t);
+      if (!mask) continue;
+      const int leader = __ffsll((long long)mask) - 1;
+      int32_t base = 0;
+      const int64_t sd = wave_sum_i64(tier == t ? d : 0);
+      const int64_t su = wave_sum_i64(tier == t ? ub : 0);
+      if (lane == leader) {
+        base = atomicAdd(&lcnt[t], (int32_t)__popcll(mask));
+        atomicAdd(&lsum[2 * t], (unsigned long long)sd);
+        atomicAdd(&lsum[2 * t + 1], (unsigned long long)su);
+      }
+      base = __shfl(base, leader);
+      if (tier == t) slot[i] = base + lanes_below(mask);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumBins) {
+    const int32_t c = lcnt[threadIdx.x];
+    lbase[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0;
+  }
+  if (threadIdx.x < 2 * kNumBins && lsum[threadIdx.x]) atomicAdd(&sums[threadIdx.x], lsum[threadIdx.x]);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kBinRowsPerThread; ++i) {
+    const int t = tiers[i];
+    if (t > kEmpty) {  // empty rows are only counted (t is -1 past r1)
+      const int64_t u = (int64_t)r0 + ((int64_t)blockIdx.x * kBinRowsPerThread + i) * kThreads + threadIdx.x;
+      bins[(size_t)t * m + lbase[t] + slot[i]] = (int32_t)u;
+    }
   }
 }
 
@@ -2200,7 +2232,7 @@ struct SpgemmCore {
     fill_i32(counts, kNumBins + 1, 0, st);
     fill_i64((int64_t*)sums, 3 * kNumBins, 0, st);
     if (r1 > r0) {
-      bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, kThreads), kThreads, 0, st>>>(
+      bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, (int64_t)kThreads * kBinRowsPerThread), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
           o.force_tier, wave_bitmap_for(o, B.ncols) ? (int)o.wave_bitmap_max_ub : 0,
           (B.distinct && o.copy_tier && o.force_tier < 0) ? 1 : 0, bins.p, counts, sums);
@@ -2310,6 +2342,15 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
   C.sorted = true;
   C.distinct = true;
   C.ptr = al.alloc_n<int64_t>((int64_t)m + 1);
+  if (spgemm_dense(A, B, C, o, stats, al, tm)) {
+    if (timing || tm || trace) SMCM_HIP(hipStreamSynchronize(st));
+    if (tm) tm->flush();
+    if (stats) stats->ms_numeric = ms_since(t0);
+    if (trace)
+      fprintf(stderr, "[smcm] spgemm %dx%d * %dx%d: dense tier %.3f ms, nnz %lld\n", A.nrows, A.ncols, B.nrows,
+              B.ncols, ms_since(t0), (long long)C.nnz);
+    return;
+  }
   SpgemmCore core(A, B, est_r, est_stride, o, al, tm);
   core.bin(0, m);
   if (stats) {

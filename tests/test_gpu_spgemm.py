@@ -59,8 +59,10 @@ CASES = [
 ]
 
 
-# tier "1s": the mid tier through the workgroup LDS sort instead of the wave bitmap
-@pytest.mark.parametrize("tier", [-1, 0, 1, "1s", 2])
+# tier -1: automatic (these products all fit the dense tier); "t": automatic row binning over the
+# row tiers (dense tier off); "1s": the mid tier through the workgroup LDS sort instead of the
+# wave bitmap
+@pytest.mark.parametrize("tier", [-1, "t", 0, 1, "1s", 2])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("vals", ["none", "A", "B", "AB"])
 def test_spgemm_tiers(ctx, case, tier, vals):
@@ -69,16 +71,58 @@ def test_spgemm_tiers(ctx, case, tier, vals):
     A = random_csr(rng, m, k, row_len=la, with_vals=vals in ("A", "AB"))
     B = random_csr(rng, k, n, row_len=lb, with_vals=vals in ("B", "AB"))
     ref = O.spgemm(A, B)
-    wb = ctx.get_option("wave_bitmap_cols")
-    ctx.set_option("force_tier", 1 if tier == "1s" else tier)
+    kv = {"force_tier": {"t": -1, "1s": 1}.get(tier, tier)}
     if tier == "1s":
-        ctx.set_option("wave_bitmap_cols", 0)
-    try:
+        kv["wave_bitmap_cols"] = 0
+    if tier == "t":
+        kv["dense_cells"] = 0
+    with options(ctx, **kv):
         ptr, col, val = _run(ctx, A, B)
-    finally:
-        ctx.set_option("force_tier", -1)
-        ctx.set_option("wave_bitmap_cols", wb)
     assert_same_csr(ptr, col, val, ref, what=f"{name} tier={tier} vals={vals}")
+
+
+DENSE_CASES = [
+    # (name, m, k, n, A row length, B row length, A duplicates)
+    ("one_cell", 1, 1, 1, lambda r: 1, lambda r: 1, False),
+    ("empty_rows", 500, 300, 77, lambda r: 0 if r.random() < 0.6 else r.integers(1, 9), lambda r: r.integers(0, 6), False),
+    ("hub_row", 40, 30000, 200, lambda r: r.integers(15000, 30000) if r.random() < 0.1 else r.integers(0, 4),
+     lambda r: 1, False),
+    ("hub_B", 300, 50, 4000, lambda r: r.integers(0, 6), lambda r: r.integers(2000, 4000) if r.random() < 0.1 else 3, False),
+    ("word_edges", 130, 70, 129, lambda r: r.integers(0, 20), lambda r: r.integers(0, 129), True),
+    ("many_rows", 60000, 400, 10, lambda r: r.integers(0, 5), lambda r: r.integers(0, 10), True),
+]
+
+
+@pytest.mark.parametrize("case", DENSE_CASES, ids=[c[0] for c in DENSE_CASES])
+@pytest.mark.parametrize("vals", ["none", "AB"])
+def test_dense_tier(ctx, case, vals):
+    """Dense tier: hub rows of A spread over waves, long B rows, empty rows, rows that span
+    several row ends per wave, column counts at 64-bit word edges, duplicates in A; two products
+    in a row through the same all-zero block, which each compaction must leave zero again."""
+    name, m, k, n, la, lb, dup = case
+    rng = np.random.default_rng(zlib.crc32(f"dense/{name}/{vals}".encode()))
+    A = random_csr(rng, m, k, row_len=la, with_vals=vals == "AB", dup=dup, sorted_rows=not dup)
+    B = random_csr(rng, k, n, row_len=lb, with_vals=vals == "AB")
+    ref = O.spgemm(A, B)
+    with options(ctx, dense_cells=1 << 28):
+        for rep in range(2):
+            ptr, col, val = _run(ctx, A, B)
+            assert ctx.last_stats()["rows_tier"][1:3] == [0, 0]
+            assert_same_csr(ptr, col, val, ref, what=f"dense {name} vals={vals} rep={rep}")
+
+
+def test_dense_tier_wraps_mod_2_64(ctx):
+    """Counts of the dense tier wrap mod 2^64; an entry whose sum wraps to 0 stays in the
+    structure."""
+    big = np.uint64(2**63)
+    # row 0: 2^63 + 2^63 at column 5 wraps to 0; row 1: 3 * 2^63 = 2^63 mod 2^64
+    A = O.CSR(2, 2, np.array([0, 2, 3]), np.array([0, 1, 0]), np.array([1, 1, 3], np.uint64))
+    B = O.CSR(2, 70, np.array([0, 2, 3]), np.array([0, 5, 5]), np.array([big, big, big], np.uint64))
+    ref = O.spgemm(A, B)
+    with options(ctx, dense_cells=1 << 28):
+        ptr, col, val = _run(ctx, A, B)
+    assert_same_csr(ptr, col, val, ref, what="dense wrap")
+    assert 0 in set(val.tolist())
 
 
 @pytest.mark.parametrize("cap", [64, 1024, 2048, 4096])
