@@ -49,7 +49,9 @@ constexpr int kShortRow = 16;  // B rows up to this length are walked by one thr
 // byte offsets of the readbacks in the thread's pinned host stage
 constexpr size_t kStageBin = 0, kStageTierNnz = 192, kStageNnz = 256, kStageSplit = 320;
 // byte offsets in a product's device counter block
-constexpr size_t kMetaCounts = 0, kMetaQueue = 32, kMetaSums = 64, kMetaSplit = 192, kMetaBytes = 320;
+// (sums: per tier A entries and products, then per tier output entries: 3 * kNumBins values)
+constexpr size_t kMetaCounts = 0, kMetaQueue = 32, kMetaSums = 64, kMetaSplit = 256, kMetaBytes = 384;
+static_assert(kMetaSums + 3 * 8 * 6 <= kMetaSplit, "tier sums overlap the split counters");
 
 // ------------------------------------------------------------------------------------------
 // analysis
@@ -146,6 +148,176 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
       const int64_t u = (int64_t)r0 + ((int64_t)blockIdx.x * kBinRowsPerThread + i) * kThreads + threadIdx.x;
       bins[(size_t)t * m + lbase[t] + slot[i]] = (int32_t)u;
     }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Symbolic pass over a right operand kept as a bitmap (B with few rows: its rows x column words
+// fit a few MB).  Row u's width is the popcount of the OR of the bitmap rows of its A entries;
+// a wave takes a row, lanes take 64-bit words (coalesced loads of each B row's words).  Work is
+// nnz(A row) x words per row instead of the row's products, which the caller checks is cheaper.
+__global__ void __launch_bounds__(kThreads) bbits_kernel(const int64_t* __restrict__ bptr,
+                                                         const int32_t* __restrict__ bcol, int32_t nrows, int32_t nw,
+                                                         unsigned long long* __restrict__ bits) {
+  // bits must be zero; one wave per row of B, a lane per entry.  The row is sorted, so lanes
+  // sharing a word are neighbours: their bits are ORed across the run first and only the run's
+  // last lane makes the atomic (a run cut by the end of a step is finished by the next step's
+  // atomic on the same word).
+  const int lane = lane_id();
+  for (int64_t r = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); r < nrows;
+       r += (int64_t)gridDim.x * (kThreads / kWave)) {
+    const int64_t s0 = bptr[r], s1 = bptr[r + 1];
+    for (int64_t g0 = s0; g0 < s1; g0 += 8 * kWave) {
+      int32_t cs[8];  // eight steps' columns in flight per lane
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int64_t e = g0 + q * kWave + lane;
+        cs[q] = e < s1 ? bcol[e] : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (g0 + q * kWave >= s1) break;  // wave-uniform
+        const bool valid = cs[q] >= 0;
+        const int32_t wd = valid ? cs[q] >> 6 : INT32_MAX;
+        unsigned long long b = valid ? 1ull << (cs[q] & 63) : 0ull;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          const unsigned long long y = __shfl_up(b, o);
+          const int32_t wy = __shfl_up(wd, o);
+          if (lane >= o && wy == wd) b |= y;
+        }
+        const int32_t wn = __shfl_down(wd, 1);
+        if (valid && (lane == kWave - 1 || wn != wd)) atomicOr(&bits[r * nw + wd], b);
+      }
+    }
+  }
+}
+
+constexpr int kBbitsLight = 8;  // rows of A with at most this many entries: a wave per row
+
+// adds a lane's word count c (word w of the row) to the row's per-window counts: windows of
+// 2^wsh words; steps of 64 words starting at w0 (wcnt zeroed by the caller, atomics: several
+// waves may add to one window)
+__device__ __forceinline__ void bbits_window_add(int32_t* __restrict__ wcnt, int64_t u, int32_t nwin, int wsh,
+                                                 int32_t w0, int32_t nw, int32_t w, int32_t c, int32_t sc) {
+  const int lane = lane_id();
+  if (wsh >= 6) {  // the step's 64 words lie in one window
+    if (lane == 0 && sc) atomicAdd(&wcnt[u * nwin + (w0 >> wsh)], sc);
+  } else {  // windows of fewer than 64 words lie inside the step
+    const int32_t first = w0 >> wsh, last = min(w0 + kWave - 1, nw - 1) >> wsh;
+    for (int32_t win = first; win <= last; ++win) {
+      const int32_t wc = (int32_t)wave_sum_i64((w < nw && (w >> wsh) == win) ? c : 0);
+      if (lane == 0 && wc) atomicAdd(&wcnt[u * nwin + win], wc);
+    }
+  }
+}
+
+// Light rows (at most kBbitsLight entries): a wave per row, the row's bitmap rows read eight at
+// a time, widths into row_nnz and per-window counts into wcnt (windows of 2^log2w columns,
+// log2w >= 6) when set.  Heavier rows are appended to heavy[] (count in *nheavy, zeroed).
+__global__ void __launch_bounds__(kThreads) bbits_symbolic_kernel(
+    const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const unsigned long long* __restrict__ bits,
+    int32_t nw, int32_t m, int log2w, int32_t nwin, int64_t* __restrict__ row_nnz, int32_t* __restrict__ wcnt,
+    int32_t* __restrict__ heavy, int32_t* __restrict__ nheavy) {
+  const int lane = lane_id();
+  const int wsh = log2w - 6;  // log2 of the words per window
+  for (int64_t u = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); u < m;
+       u += (int64_t)gridDim.x * (kThreads / kWave)) {
+    const int64_t rs = uniform_i64(aptr[u]), re = uniform_i64(aptr[u + 1]);
+    if (re - rs > kBbitsLight) {
+      if (lane == 0) heavy[atomicAdd(nheavy, 1)] = (int32_t)u;
+      continue;
+    }
+    int32_t k[kBbitsLight];
+#pragma unroll
+    for (int j = 0; j < kBbitsLight; ++j) k[j] = rs + j < re ? acol[rs + j] : -1;
+    int64_t total = 0;
+    int32_t cur_win = -1, cur_cnt = 0;  // windows of 64+ words: the open window's count (lane 0)
+    // four 64-word steps at a time: every load of their words in flight together
+    for (int32_t g0 = 0; g0 < nw; g0 += 4 * kWave) {
+      unsigned long long x[4] = {0ull, 0ull, 0ull, 0ull};
+#pragma unroll
+      for (int j = 0; j < kBbitsLight; ++j) {
+        if (k[j] < 0) continue;  // wave-uniform
+        const unsigned long long* brow = bits + (int64_t)k[j] * nw;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int32_t w = g0 + q * kWave + lane;
+          if (w < nw) x[q] |= brow[w];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int32_t w0 = g0 + q * kWave;
+        if (w0 >= nw) break;  // wave-uniform
+        const int32_t w = w0 + lane;
+        const int32_t c = __popcll(x[q]);
+        const int32_t sc = (int32_t)wave_sum_i64(c);
+        total += sc;
+        if (wcnt) {
+          if (wsh >= 6) {
+            // this wave owns the row's counts: plain stores of whole windows, no atomics
+            // (their completion would hold up the next loads' waits)
+            const int32_t win = w0 >> wsh;
+            if (win != cur_win) {
+              if (lane == 0 && cur_win >= 0 && cur_cnt) wcnt[u * nwin + cur_win] = cur_cnt;
+              cur_win = win;
+              cur_cnt = 0;
+            }
+            cur_cnt += sc;
+          } else {
+            bbits_window_add(wcnt, u, nwin, wsh, w0, nw, w, c, sc);
+          }
+        }
+      }
+    }
+    if (lane == 0) {
+      if (wcnt && cur_win >= 0 && cur_cnt) wcnt[u * nwin + cur_win] = cur_cnt;
+      row_nnz[u] = total;
+    }
+  }
+}
+
+// Heavy rows: a workgroup per (row, 64-word chunk); its waves split the row's entries (eight
+// bitmap rows in flight per lane), OR their words in LDS, and wave 0 counts the chunk into the
+// row's width and window counts (row_nnz zeroed by the caller: chunks add).
+__global__ void __launch_bounds__(kThreads) bbits_heavy_kernel(
+    const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const unsigned long long* __restrict__ bits,
+    int32_t nw, int log2w, int32_t nwin, const int32_t* __restrict__ heavy, const int32_t* __restrict__ nheavy,
+    int64_t* __restrict__ row_nnz, int32_t* __restrict__ wcnt) {
+  constexpr int NWV = kThreads / kWave;
+  __shared__ unsigned long long acc[kWave];
+  const int lane = lane_id(), wid = wave_id();
+  const int32_t nch = (nw + kWave - 1) / kWave;
+  const int64_t tasks = (int64_t)(*nheavy) * nch;
+  for (int64_t t = blockIdx.x; t < tasks; t += gridDim.x) {
+    const int64_t u = heavy[t / nch];
+    const int32_t w0 = (int32_t)(t % nch) * kWave;
+    const int32_t w = w0 + lane;
+    const int64_t rs = aptr[u], re = aptr[u + 1];
+    if (wid == 0) acc[lane] = 0ull;
+    __syncthreads();
+    unsigned long long x = 0;
+    if (w < nw) {
+      int64_t e = rs + wid;
+      for (; e + 7 * NWV < re; e += 8 * NWV) {
+        unsigned long long y[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) y[j] = bits[(int64_t)acol[e + j * NWV] * nw + w];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x |= y[j];
+      }
+      for (; e < re; e += NWV) x |= bits[(int64_t)acol[e] * nw + w];
+    }
+    if (x) atomicOr(&acc[lane], x);
+    __syncthreads();
+    if (wid == 0) {
+      const int32_t c = __popcll(acc[lane]);
+      const int32_t sc = (int32_t)wave_sum_i64(c);
+      if (lane == 0 && sc) atomicAdd((unsigned long long*)&row_nnz[u], (unsigned long long)sc);
+      if (wcnt) bbits_window_add(wcnt, u, nwin, log2w - 6, w0, nw, w, c, sc);
+    }
+    __syncthreads();
   }
 }
 
@@ -2261,10 +2433,45 @@ struct SpgemmCore {
   // widths of the binned rows into row_nnz (other entries untouched).  The per-tier widths are
   // copied to the host stage without a synchronisation: read_symbolic() fills tc.c_nnz after
   // the caller's next stream synchronisation.
+  // the symbolic pass as ORs of B's rows kept as a bitmap: B's bitmap is small and reading a
+  // row's nnz(A row) bitmap rows costs less than its products (checked over the whole product)
+  bool use_bbits() const {
+    if (o.bbits_bytes <= 0 || o.force_tier >= 0 || A.nrows == 0 || B.ncols == 0) return false;
+    const int64_t nw = ceil_div(B.ncols, 64);
+    if ((int64_t)B.nrows * nw * 8 > o.bbits_bytes) return false;
+    return (double)A.nnz * (double)nw <= 4.0 * (double)products;
+  }
   void symbolic(int64_t* row_nnz) {
     hipStream_t st = al.stream;
     if (split.wcnt) fill_i32(split.wcnt, (int64_t)A.nrows * split.wl.nwin, 0, st);
-    dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, &split, queue, st, tm);
+    if (use_bbits()) {
+      const int32_t nw = (int32_t)ceil_div(B.ncols, 64);
+      DevBuf<unsigned long long> bits(al, (int64_t)B.nrows * nw + 1);
+      KernelTimer dummy;
+      KernelTimer& T = tm ? *tm : dummy;
+      DevBuf<int32_t> heavy(al, A.nrows);
+      const int tok = T.start("spgemm.symbolic.bbits", st);
+      // the bitmap and, after it, the heavy-row counter: zeroed by one fill
+      fill_i64((int64_t*)bits.p, (int64_t)B.nrows * nw + 1, 0, st);
+      int32_t* nheavy = (int32_t*)(bits.p + (int64_t)B.nrows * nw);
+      const int wpb = kThreads / kWave;
+      if (B.nrows > 0) {
+        bbits_kernel<<<(unsigned)std::min<int64_t>(ceil_div(B.nrows, wpb), 65536), kThreads, 0, st>>>(
+            B.ptr, B.col, B.nrows, nw, bits.p);
+        SMCM_LAUNCH_CHECK();
+      }
+      bbits_symbolic_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nrows, wpb), 65536), kThreads, 0, st>>>(
+          A.ptr, A.col, bits.p, nw, A.nrows, split.wl.log2w, (int32_t)split.wl.nwin, row_nnz, split.wcnt, heavy.p,
+          nheavy);
+      SMCM_LAUNCH_CHECK();
+      bbits_heavy_kernel<<<(unsigned)(o.cus * 8), kThreads, 0, st>>>(A.ptr, A.col, bits.p, nw, split.wl.log2w,
+                                                                       (int32_t)split.wl.nwin, heavy.p, nheavy,
+                                                                       row_nnz, split.wcnt);
+      SMCM_LAUNCH_CHECK();
+      T.stop(tok, st, 16.0 * ((double)A.nrows + 1) + 20.0 * (double)A.nnz + 4.0 * (double)products);
+    } else {
+      dispatch<false>(A, B, S.p, bins.p, tc, row_nnz, nullptr, nullptr, nullptr, o, lay, &split, queue, st, tm);
+    }
     unsigned long long* tn = sums + 2 * kNumBins;
     fill_i64((int64_t*)tn, kNumBins, 0, st);
     tier_nnz_kernel<<<dim3(64, kNumBins - 1 - kEmpty), kThreads, 0, st>>>(bins.p, A.nrows, counts, row_nnz, tn);

@@ -81,6 +81,38 @@ def test_spgemm_tiers(ctx, case, tier, vals):
     assert_same_csr(ptr, col, val, ref, what=f"{name} tier={tier} vals={vals}")
 
 
+@pytest.mark.parametrize("window", [64, 512, 8192])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_bbits_symbolic(ctx, case, window):
+    """Symbolic pass as ORs of B's bitmap rows: row widths and the per-(row, window) counts the
+    window kernel takes its output offsets from, windows of 1 to 128 bitmap words."""
+    name, m, k, n, la, lb = case
+    rng = np.random.default_rng(zlib.crc32(f"bbits/{name}/{window}".encode()))
+    A = random_csr(rng, m, k, row_len=la, with_vals=True)
+    B = random_csr(rng, k, n, row_len=lb, with_vals=True)
+    ref = O.spgemm(A, B)
+    nw = (n + 63) // 64
+    used = A.nnz * nw <= 4 * O.products(A, B)
+    with options(ctx, dense_cells=0, bbits_bytes=1 << 30, window_cols=window, window_all_nwin=1 << 20,
+                 window_min_products=0, kernel_timing=1):
+        ctx.kernel_stats_reset()
+        ptr, col, val = _run(ctx, A, B)
+        ks = ctx.kernel_stats()
+    assert ("spgemm.symbolic.bbits" in ks) == used, (name, used, sorted(ks))
+    assert_same_csr(ptr, col, val, ref, what=f"bbits {name} window={window}")
+
+
+def test_bbits_symbolic_used():
+    """Some of the cases above take the bitmap symbolic pass."""
+    rng = np.random.default_rng(0)
+    got = []
+    for name, m, k, n, la, lb in CASES:
+        A = random_csr(rng, m, k, row_len=la)
+        B = random_csr(rng, k, n, row_len=lb)
+        got.append(A.nnz * ((n + 63) // 64) <= 4 * O.products(A, B))
+    assert any(got) and not all(got)
+
+
 DENSE_CASES = [
     # (name, m, k, n, A row length, B row length, A duplicates)
     ("one_cell", 1, 1, 1, lambda r: 1, lambda r: 1, False),
