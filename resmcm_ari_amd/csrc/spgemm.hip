@@ -47,11 +47,12 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kShortRow = 16;  // B rows up to this length are walked by one thread
 // byte offsets of the readbacks in the thread's pinned host stage
-constexpr size_t kStageBin = 0, kStageTierNnz = 192, kStageNnz = 256, kStageSplit = 320;
+constexpr size_t kStageBin = 0, kStageTierNnz = 256, kStageNnz = 320, kStageSplit = 384;
 // byte offsets in a product's device counter block
-// (sums: per tier A entries and products, then per tier output entries: 3 * kNumBins values)
+// (sums: per tier A entries and products, per tier output entries, per tier largest product
+// count: 4 * kNumBins values)
 constexpr size_t kMetaCounts = 0, kMetaQueue = 32, kMetaSums = 64, kMetaSplit = 256, kMetaBytes = 384;
-static_assert(kMetaSums + 3 * 8 * 6 <= kMetaSplit, "tier sums overlap the split counters");
+static_assert(kMetaSums + 4 * 8 * 6 <= kMetaSplit, "tier sums overlap the split counters");
 
 // ------------------------------------------------------------------------------------------
 // analysis
@@ -80,8 +81,9 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
                                                             int32_t* __restrict__ counts,
                                                             unsigned long long* __restrict__ sums) {
   __shared__ int32_t lcnt[kNumBins], lbase[kNumBins];
-  __shared__ unsigned long long lsum[2 * kNumBins];
+  __shared__ unsigned long long lsum[2 * kNumBins], lmax[kNumBins];
   if (threadIdx.x < kNumBins) lcnt[threadIdx.x] = 0;
+  if (threadIdx.x < kNumBins) lmax[threadIdx.x] = 0;
   if (threadIdx.x < 2 * kNumBins) lsum[threadIdx.x] = 0;
   __syncthreads();
   const int lane = lane_id();
@@ -125,10 +127,12 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
       int32_t base = 0;
       const int64_t sd = wave_sum_i64(tier == t ? d : 0);
       const int64_t su = wave_sum_i64(tier == t ? ub : 0);
+      const int64_t mu = wave_max_i64(tier == t ? ub : 0);
       if (lane == leader) {
         base = atomicAdd(&lcnt[t], (int32_t)__popcll(mask));
         atomicAdd(&lsum[2 * t], (unsigned long long)sd);
         atomicAdd(&lsum[2 * t + 1], (unsigned long long)su);
+        atomicMax(&lmax[t], (unsigned long long)mu);
       }
       base = __shfl(base, leader);
       if (tier == t) slot[i] = base + lanes_below(mask);
@@ -140,6 +144,7 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
     lbase[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0;
   }
   if (threadIdx.x < 2 * kNumBins && lsum[threadIdx.x]) atomicAdd(&sums[threadIdx.x], lsum[threadIdx.x]);
+  if (threadIdx.x < kNumBins && lmax[threadIdx.x]) atomicMax(&sums[3 * kNumBins + threadIdx.x], lmax[threadIdx.x]);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kBinRowsPerThread; ++i) {
@@ -1958,12 +1963,57 @@ __global__ void __launch_bounds__(kThreads) copy_tier_kernel(const int64_t* __re
   }
 }
 
+// Copy rows of at most kCopyRowMax elements: a wave per row, eight loads in flight per lane
+constexpr int64_t kCopyRowMax = 8192;
+template <bool AV, bool BV>
+__global__ void __launch_bounds__(kThreads) copy_rows_kernel(const int32_t* __restrict__ rows, int32_t n,
+                                                             const int64_t* __restrict__ aptr,
+                                                             const int32_t* __restrict__ acol,
+                                                             const uint64_t* __restrict__ aval,
+                                                             const int64_t* __restrict__ bptr,
+                                                             const int32_t* __restrict__ bcol,
+                                                             const uint64_t* __restrict__ bval,
+                                                             const int64_t* __restrict__ cptr,
+                                                             int32_t* __restrict__ ccol, uint64_t* __restrict__ cval) {
+  constexpr int kIn = 8;
+  const int lane = lane_id();
+  for (int64_t i = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); i < n;
+       i += (int64_t)gridDim.x * (kThreads / kWave)) {
+    const int32_t u = rows[i];
+    const int64_t e = aptr[u];
+    const int32_t k = acol[e];
+    const uint64_t a = AV ? aval[e] : 1ull;
+    const int64_t s0 = bptr[k], len = bptr[k + 1] - s0, d0 = cptr[u];
+    for (int64_t x0 = 0; x0 < len; x0 += kIn * kWave) {
+      int32_t c[kIn];
+      uint64_t v[kIn];
+#pragma unroll
+      for (int q = 0; q < kIn; ++q) {
+        const int64_t x = x0 + q * kWave + lane;
+        if (x < len) {
+          c[q] = bcol[s0 + x];
+          v[q] = BV ? bval[s0 + x] : 1ull;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kIn; ++q) {
+        const int64_t x = x0 + q * kWave + lane;
+        if (x < len) {
+          __builtin_nontemporal_store(c[q], &ccol[d0 + x]);
+          __builtin_nontemporal_store(BV ? a * v[q] : a, &cval[d0 + x]);
+        }
+      }
+    }
+  }
+}
+
 // per-tier accounting of one pass: rows, A entries, products, output entries
 struct TierCounts {
   int32_t rows[kNumBins] = {};
   int64_t a_nnz[kNumBins] = {};
   int64_t products[kNumBins] = {};
   int64_t c_nnz[kNumBins] = {};
+  int64_t max_ub[kNumBins] = {};  // largest product count of a row
 };
 
 // scratch of the numeric bitmap/window split (device)
@@ -2034,7 +2084,14 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
       copy_width_kernel<<<g, kThreads, 0, st>>>(crow, nC, A.ptr, S, row_nnz);
       SMCM_LAUNCH_CHECK();
       T.stop(tok, st, 16.0 * (nC + 1) + 20.0 * (double)tc.a_nnz[kCopyTier]);
+    } else if (tc.max_ub[kCopyTier] <= kCopyRowMax) {
+      const int tok = T.start("spgemm.numeric.copy", st);
+      copy_rows_kernel<AV, BV><<<(unsigned)std::min<int64_t>(ceil_div(nC, kThreads / kWave), 65536), kThreads, 0, st>>>(
+          crow, nC, A.ptr, A.col, A.val, B.ptr, B.col, B.val, cptr, ccol, cval);
+      SMCM_LAUNCH_CHECK();
+      T.stop(tok, st, tier_bytes(true, tc, kCopyTier, AV, BV));
     } else {
+      // rows of any length: elements laid out flat over the copy rows, fixed-size chunks
       DevAlloc& al = *split->al;
       const int64_t total = tc.products[kCopyTier];
       DevBuf<int64_t> len(al, nC), pre(al, (int64_t)nC + 1), src(al, nC), dst(al, nC);
@@ -2402,7 +2459,7 @@ struct SpgemmCore {
   void bin(int32_t r0, int32_t r1) {
     hipStream_t st = al.stream;
     fill_i32(counts, kNumBins + 1, 0, st);
-    fill_i64((int64_t*)sums, 3 * kNumBins, 0, st);
+    fill_i64((int64_t*)sums, 4 * kNumBins, 0, st);
     if (r1 > r0) {
       bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, (int64_t)kThreads * kBinRowsPerThread), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
@@ -2411,17 +2468,19 @@ struct SpgemmCore {
       SMCM_LAUNCH_CHECK();
     }
     unsigned char* hs = host_stage() + kStageBin;
-    // tier row counts at +0, tier sums at +64 in both blocks: one copy
-    SMCM_HIP(hipMemcpyAsync(hs, meta.p + kMetaCounts, kMetaSums + sizeof(unsigned long long) * 2 * kNumBins,
+    // tier row counts at +0, tier sums at +64 (largest product counts after the width sums) in
+    // both blocks: one copy
+    SMCM_HIP(hipMemcpyAsync(hs, meta.p + kMetaCounts, kMetaSums + sizeof(unsigned long long) * 4 * kNumBins,
                             hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipStreamSynchronize(st));
-    unsigned long long sh[2 * kNumBins];
+    unsigned long long sh[4 * kNumBins];
     memcpy(tc.rows, hs, sizeof(int32_t) * kNumBins);
     memcpy(sh, hs + 64, sizeof(sh));
     for (int t = 0; t < kNumBins; ++t) {
       tc.a_nnz[t] = (int64_t)sh[2 * t];
       tc.products[t] = (int64_t)sh[2 * t + 1];
       tc.c_nnz[t] = 0;
+      tc.max_ub[t] = (int64_t)sh[3 * kNumBins + t];
     }
     // every row of A binned: the products of all tiers are the product's total (S[nnz(A)])
     if (r0 == 0 && r1 == A.nrows) {

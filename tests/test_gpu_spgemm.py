@@ -203,16 +203,20 @@ def test_spgemm_duplicates_in_A(ctx):
 @pytest.mark.parametrize("copy", [0, 1])
 @pytest.mark.parametrize("vals", ["none", "A", "B", "AB"])
 @pytest.mark.parametrize("bdup", [False, True])
-def test_copy_tier(ctx, vals, copy, bdup):
+@pytest.mark.parametrize("longest", [8000, 20000])
+def test_copy_tier(ctx, vals, copy, bdup, longest):
     """Rows of A with a single entry over a right operand whose rows repeat no column: a scaled
-    copy of one row of B (the copy tier), mixed with longer rows; B rows from empty to 20000
-    columns, so a copy chunk spans many rows or a part of one.  With repeated columns in B the
-    host check leaves the copy tier off."""
-    rng = np.random.default_rng(zlib.crc32(f"copy/{vals}/{copy}/{bdup}".encode()))
+    copy of one row of B (the copy tier), mixed with longer rows.  B rows up to 8000 columns are
+    copied a wave per row; with rows up to 20000 columns the elements are cut into chunks, so a
+    chunk spans many rows or a part of one.  With repeated columns in B the host check leaves the
+    copy tier off."""
+    rng = np.random.default_rng(zlib.crc32(f"copy/{vals}/{copy}/{bdup}/{longest}".encode()))
     A = random_csr(rng, 3000, 400, row_len=lambda r: 1 if r.random() < 0.7 else r.integers(0, 6),
                    with_vals=vals in ("A", "AB"))
-    B = random_csr(rng, 400, 40000, row_len=lambda r: int(r.zipf(1.2)) % 20000, with_vals=vals in ("B", "AB"),
+    B = random_csr(rng, 400, 40000, row_len=lambda r: int(r.zipf(1.2)) % longest, with_vals=vals in ("B", "AB"),
                    dup=bdup)
+    if longest == 20000:
+        assert np.diff(B.ptr).max() > 8192  # the chunked path
     ref = O.spgemm(A, B)
     with options(ctx, copy_tier=copy):
         ptr, col, val = _run(ctx, A, B)
