@@ -50,6 +50,11 @@ TIMED = {
     "spgemm.numeric.wave": r"wave_tier_kernel<true",
     "spgemm.symbolic.wave": r"wave_tier_kernel<false",
     "spgemm.analysis": r"^entry_len_kernel$",
+    "spgemm.dense.accumulate": r"^dense_(scatter|count)_kernel",
+    "spgemm.dense.compact": r"^dense_compact_kernel$",
+    "spgemm.symbolic.bbits": r"^bbits_(kernel|symbolic_kernel|heavy_kernel)$",
+    "spgemm.numeric.copy": r"^copy_(rows|tier)_kernel<",
+    "spgemm.symbolic.copy": r"^copy_width_kernel$",
     "estimator.pass": r"est_(light|heavy)_kernel<",
     "copy": r"^copy_kernel$",
 }
