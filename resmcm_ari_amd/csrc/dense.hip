@@ -28,7 +28,6 @@ namespace smcm {
 namespace {
 
 constexpr int kDThreads = 256;
-constexpr size_t kStageDense = 512;  // byte offset of this tier's readback in the host stage
 constexpr int kDWaves = kDThreads / kWave;
 
 // row of entry e0: the u with ptr[u] <= e0 < ptr[u+1] (0 <= e0 < ptr[m]).  64 probes per step
@@ -321,7 +320,7 @@ bool spgemm_dense(const Csr& A, const Csr& B, Csr& C, const SpgemmOpts& o, Spgem
     if (tm) tm->stop(acc_tok, st, 0.0);
   }
   scan_exclusive_i64(cc.p, off.p, tasks, al);
-  unsigned char* hs = host_stage() + kStageDense;
+  unsigned char* hs = host_stage() + kHostStageDense;
   SMCM_HIP(hipMemcpyAsync(hs, off.p + tasks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SMCM_HIP(hipMemcpyAsync(hs + 8, counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   SMCM_HIP(hipStreamSynchronize(st));

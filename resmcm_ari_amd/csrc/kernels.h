@@ -124,8 +124,12 @@ void padded_to_csr(const int64_t* start, const int32_t* len, const int32_t* col,
                    int32_t ncols, Csr& out, DevAlloc& al);
 
 // ---- spgemm.hip -------------------------------------------------------------------------
+// byte offset of the dense tier's readback in the thread's host stage (the row-binned path's
+// sections lie below it)
+constexpr size_t kHostStageDense = 576;
+
 struct SpgemmOpts {
-  int s_max = 64;          // rows with ub <= s_max: one wave per row, register sort
+  int s_max = 32;          // rows with ub <= s_max: one wave per row, register sort
   int m_max = 1024;        // rows with ub <= m_max (and sparse estimate): workgroup LDS sort
   int dense_div = 64;      // estimated width >= ncols/dense_div sends the row to the bitmap tier
   int64_t heavy_ub = 1 << 16;  // bitmap-tier rows with ub above this are scheduled first

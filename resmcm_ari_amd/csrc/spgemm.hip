@@ -47,12 +47,13 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kShortRow = 16;  // B rows up to this length are walked by one thread
 // byte offsets of the readbacks in the thread's pinned host stage
-constexpr size_t kStageBin = 0, kStageTierNnz = 256, kStageNnz = 320, kStageSplit = 384;
+constexpr size_t kStageBin = 0, kStageTierNnz = 320, kStageNnz = 384, kStageSplit = 448;
+static_assert(kStageSplit + 112 <= kHostStageDense, "host stage sections overlap");
 // byte offsets in a product's device counter block
 // (sums: per tier A entries and products, per tier output entries, per tier largest product
-// count: 4 * kNumBins values)
-constexpr size_t kMetaCounts = 0, kMetaQueue = 32, kMetaSums = 64, kMetaSplit = 256, kMetaBytes = 384;
-static_assert(kMetaSums + 4 * 8 * 6 <= kMetaSplit, "tier sums overlap the split counters");
+// count, per tier largest row of A: 5 * kNumBins values)
+constexpr size_t kMetaCounts = 0, kMetaQueue = 32, kMetaSums = 64, kMetaSplit = 320, kMetaBytes = 448;
+static_assert(kMetaSums + 5 * 8 * 6 <= kMetaSplit, "tier sums overlap the split counters");
 
 // ------------------------------------------------------------------------------------------
 // analysis
@@ -81,9 +82,9 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
                                                             int32_t* __restrict__ counts,
                                                             unsigned long long* __restrict__ sums) {
   __shared__ int32_t lcnt[kNumBins], lbase[kNumBins];
-  __shared__ unsigned long long lsum[2 * kNumBins], lmax[kNumBins];
+  __shared__ unsigned long long lsum[2 * kNumBins], lmax[2 * kNumBins];  // largest ub, largest d
   if (threadIdx.x < kNumBins) lcnt[threadIdx.x] = 0;
-  if (threadIdx.x < kNumBins) lmax[threadIdx.x] = 0;
+  if (threadIdx.x < 2 * kNumBins) lmax[threadIdx.x] = 0;
   if (threadIdx.x < 2 * kNumBins) lsum[threadIdx.x] = 0;
   __syncthreads();
   const int lane = lane_id();
@@ -128,11 +129,13 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
       const int64_t sd = wave_sum_i64(tier == t ? d : 0);
       const int64_t su = wave_sum_i64(tier == t ? ub : 0);
       const int64_t mu = wave_max_i64(tier == t ? ub : 0);
+      const int64_t md = wave_max_i64(tier == t ? d : 0);
       if (lane == leader) {
         base = atomicAdd(&lcnt[t], (int32_t)__popcll(mask));
         atomicAdd(&lsum[2 * t], (unsigned long long)sd);
         atomicAdd(&lsum[2 * t + 1], (unsigned long long)su);
         atomicMax(&lmax[t], (unsigned long long)mu);
+        atomicMax(&lmax[kNumBins + t], (unsigned long long)md);
       }
       base = __shfl(base, leader);
       if (tier == t) slot[i] = base + lanes_below(mask);
@@ -144,7 +147,8 @@ __global__ void __launch_bounds__(kThreads) bin_rows_kernel(const int64_t* __res
     lbase[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0;
   }
   if (threadIdx.x < 2 * kNumBins && lsum[threadIdx.x]) atomicAdd(&sums[threadIdx.x], lsum[threadIdx.x]);
-  if (threadIdx.x < kNumBins && lmax[threadIdx.x]) atomicMax(&sums[3 * kNumBins + threadIdx.x], lmax[threadIdx.x]);
+  if (threadIdx.x < 2 * kNumBins && lmax[threadIdx.x])
+    atomicMax(&sums[3 * kNumBins + threadIdx.x], lmax[threadIdx.x]);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kBinRowsPerThread; ++i) {
@@ -199,6 +203,7 @@ __global__ void __launch_bounds__(kThreads) bbits_kernel(const int64_t* __restri
 }
 
 constexpr int kBbitsLight = 8;  // rows of A with at most this many entries: a wave per row
+constexpr int64_t kBbitsMaxEntries = 2048;  // longest row of A the bitmap symbolic pass takes
 
 // adds a lane's word count c (word w of the row) to the row's per-window counts: windows of
 // 2^wsh words; steps of 64 words starting at w0 (wcnt zeroed by the caller, atomics: several
@@ -2014,6 +2019,7 @@ struct TierCounts {
   int64_t products[kNumBins] = {};
   int64_t c_nnz[kNumBins] = {};
   int64_t max_ub[kNumBins] = {};  // largest product count of a row
+  int64_t max_d[kNumBins] = {};   // largest row of A (entries)
 };
 
 // scratch of the numeric bitmap/window split (device)
@@ -2459,7 +2465,7 @@ struct SpgemmCore {
   void bin(int32_t r0, int32_t r1) {
     hipStream_t st = al.stream;
     fill_i32(counts, kNumBins + 1, 0, st);
-    fill_i64((int64_t*)sums, 4 * kNumBins, 0, st);
+    fill_i64((int64_t*)sums, 5 * kNumBins, 0, st);
     if (r1 > r0) {
       bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, (int64_t)kThreads * kBinRowsPerThread), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
@@ -2470,10 +2476,10 @@ struct SpgemmCore {
     unsigned char* hs = host_stage() + kStageBin;
     // tier row counts at +0, tier sums at +64 (largest product counts after the width sums) in
     // both blocks: one copy
-    SMCM_HIP(hipMemcpyAsync(hs, meta.p + kMetaCounts, kMetaSums + sizeof(unsigned long long) * 4 * kNumBins,
+    SMCM_HIP(hipMemcpyAsync(hs, meta.p + kMetaCounts, kMetaSums + sizeof(unsigned long long) * 5 * kNumBins,
                             hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipStreamSynchronize(st));
-    unsigned long long sh[4 * kNumBins];
+    unsigned long long sh[5 * kNumBins];
     memcpy(tc.rows, hs, sizeof(int32_t) * kNumBins);
     memcpy(sh, hs + 64, sizeof(sh));
     for (int t = 0; t < kNumBins; ++t) {
@@ -2481,6 +2487,7 @@ struct SpgemmCore {
       tc.products[t] = (int64_t)sh[2 * t + 1];
       tc.c_nnz[t] = 0;
       tc.max_ub[t] = (int64_t)sh[3 * kNumBins + t];
+      tc.max_d[t] = (int64_t)sh[4 * kNumBins + t];
     }
     // every row of A binned: the products of all tiers are the product's total (S[nnz(A)])
     if (r0 == 0 && r1 == A.nrows) {
@@ -2494,10 +2501,15 @@ struct SpgemmCore {
   // the caller's next stream synchronisation.
   // the symbolic pass as ORs of B's rows kept as a bitmap: B's bitmap is small and reading a
   // row's nnz(A row) bitmap rows costs less than its products (checked over the whole product)
+  // A row's entries are split over one workgroup's four waves, so a row of many entries (a hub)
+  // would be one long serial walk: such products keep the tiered pass.
   bool use_bbits() const {
     if (o.bbits_bytes <= 0 || o.force_tier >= 0 || A.nrows == 0 || B.ncols == 0) return false;
     const int64_t nw = ceil_div(B.ncols, 64);
     if ((int64_t)B.nrows * nw * 8 > o.bbits_bytes) return false;
+    int64_t widest = 0;
+    for (int t = 0; t < kNumBins; ++t) widest = std::max(widest, tc.max_d[t]);
+    if (widest > kBbitsMaxEntries) return false;
     return (double)A.nnz * (double)nw <= 4.0 * (double)products;
   }
   void symbolic(int64_t* row_nnz) {
