@@ -75,61 +75,84 @@ __device__ void wave_sort_sum(int32_t& k, uint64_t& v) {
   }
 }
 
-// sum of one value per thread of the block, added to *out by one atomic (same-address atomics
-// from every wave would serialise)
+// sum of one value per thread of the block (NWV waves), added to *out by one atomic: the same
+// address updated by every wave would serialise
+template <int NWV>
 __device__ void block_add(unsigned long long x, unsigned long long* out) {
-  __shared__ unsigned long long part[kDWaves];
+  __shared__ unsigned long long part[NWV];
   x = (unsigned long long)wave_sum_i64((int64_t)x);
   if (lane_id() == 0) part[wave_id()] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
 #pragma unroll
-    for (int w = 0; w < kDWaves; ++w) t += part[w];
+    for (int w = 0; w < NWV; ++w) t += part[w];
     if (t) atomicAdd(out, t);
   }
 }
 
-template <bool AV, bool BV>
-__global__ void __launch_bounds__(kDThreads) dense_scatter_kernel(
+constexpr int kLcombCols = 4096;  // products with at most this many columns combine hub rows in LDS
+
+// LCOMB (products with at most kLcombCols columns): the THREADS / 64 waves of a block take
+// consecutive 64-entry chunks together; when all of the block's products fall in one row (a hub
+// row of A), they are summed in an LDS copy of that row and the block makes one device atomic per
+// distinct column, instead of every wave adding to the row's top columns.
+template <bool AV, bool BV, bool LCOMB, int THREADS>
+__global__ void __launch_bounds__(THREADS) dense_scatter_kernel(
     const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval, int32_t m,
     int64_t nnz_a, const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
     const uint64_t* __restrict__ bval, int32_t ncols, int32_t nw, unsigned long long* __restrict__ D,
     unsigned long long* __restrict__ pres, unsigned long long* __restrict__ counters) {
-  const int lane = lane_id();
-  const int64_t gw = (int64_t)blockIdx.x * kDWaves + wave_id();
-  const int64_t nwaves = (int64_t)gridDim.x * kDWaves;
+  constexpr int NWV = THREADS / kWave;
+  __shared__ unsigned long long lv[LCOMB ? kLcombCols : 1];
+  __shared__ uint32_t lp[LCOMB ? kLcombCols / 32 : 1];
+  __shared__ int32_t s_lo, s_hi;
+  const int lane = lane_id(), wid = wave_id();
+  if constexpr (LCOMB) {
+    for (int i = threadIdx.x; i < kLcombCols; i += THREADS) lv[i] = 0ull;
+    for (int i = threadIdx.x; i < kLcombCols / 32; i += THREADS) lp[i] = 0u;
+    if (threadIdx.x == 0) {
+      s_lo = INT32_MAX;
+      s_hi = -1;
+    }
+    __syncthreads();
+  }
   unsigned long long products = 0;
-  for (int64_t e0 = gw * kWave; e0 < nnz_a; e0 += nwaves * kWave) {
+  // the block's waves take consecutive chunks; the loop is uniform over the block
+  for (int64_t b0 = (int64_t)blockIdx.x * NWV * kWave; b0 < nnz_a; b0 += (int64_t)gridDim.x * NWV * kWave) {
+    const int64_t e0 = b0 + (int64_t)wid * kWave;
     const int64_t e = e0 + lane;
     const bool valid = e < nnz_a;
-    // this lane's row: rows from the chunk's first row on, 64 row ends at a time
-    const int32_t u0 = wave_row_of(aptr, m, e0);
-    int32_t u = u0;
-    bool open = valid;
-    for (int64_t base = u0; __ballot(open); base += kWave) {
-      const int64_t r = base + 1 + lane;
-      const int64_t q = r <= m ? aptr[r] : INT64_MAX;  // end of row base + lane
-      // number of row ends <= e among the 64 (sorted): the lane's row is base + that count
-      int pos = 0;
-#pragma unroll
-      for (int s = 32; s >= 1; s >>= 1) {
-        const int64_t qq = __shfl(q, pos + s - 1);
-        if (open && qq <= e) pos += s;
-      }
-      const int64_t q63 = __shfl(q, kWave - 1);
-      if (open && !(pos == kWave - 1 && q63 <= e)) {
-        u = (int32_t)(base + pos);
-        open = false;
-      }
-    }
+    int32_t u = 0;
     int64_t bs = 0, n = 0;
     uint64_t a = 1;
-    if (valid) {
-      const int32_t k = acol[e];
-      bs = bptr[k];
-      n = bptr[k + 1] - bs;
-      if (AV) a = aval[e];
+    if (e0 < nnz_a) {  // wave-uniform
+      // this lane's row: rows from the chunk's first row on, 64 row ends at a time
+      const int32_t u0 = wave_row_of(aptr, m, e0);
+      u = u0;
+      bool open = valid;
+      for (int64_t base = u0; __ballot(open); base += kWave) {
+        const int64_t r = base + 1 + lane;
+        const int64_t q = r <= m ? aptr[r] : INT64_MAX;  // end of row base + lane
+        // number of row ends <= e among the 64 (sorted): the lane's row is base + that count
+        int pos = 0;
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) {
+          const int64_t qq = __shfl(q, pos + s - 1);
+          if (open && qq <= e) pos += s;
+        }
+        const int64_t q63 = __shfl(q, kWave - 1);
+        if (open && !(pos == kWave - 1 && q63 <= e)) {
+          u = (int32_t)(base + pos);
+          open = false;
+        }
+      }
+      if (valid) {
+        const int32_t k = acol[e];
+        bs = bptr[k];
+        n = bptr[k + 1] - bs;
+        if (AV) a = aval[e];
+      }
     }
     // exclusive prefix of the segment lengths over the wave
     int64_t incl = n;
@@ -141,6 +164,20 @@ __global__ void __launch_bounds__(kDThreads) dense_scatter_kernel(
     const int64_t off = incl - n;
     const int64_t total = __shfl(incl, kWave - 1);
     products += lane == 0 ? (unsigned long long)total : 0ull;
+    bool in_lds = false;
+    int32_t brow = 0;
+    if constexpr (LCOMB) {
+      // do all of the block's products lie in one row?
+      const int64_t lo = wave_max_i64(valid && n > 0 ? -(int64_t)u : INT64_MIN);
+      const int64_t hi = wave_max_i64(valid && n > 0 ? (int64_t)u : -1);
+      if (lane == 0 && total > 0) {
+        atomicMin(&s_lo, (int32_t)(-lo));
+        atomicMax(&s_hi, (int32_t)hi);
+      }
+      __syncthreads();
+      in_lds = s_lo == s_hi;
+      brow = s_lo;
+    }
     for (int64_t p0 = 0; p0 < total; p0 += kWave) {
       const int64_t p = p0 + lane;
       // owner: the last lane whose offset is <= p (zero-length lanes after it start beyond p)
@@ -161,6 +198,13 @@ __global__ void __launch_bounds__(kDThreads) dense_scatter_kernel(
         const int64_t b = bj + (p - oj);
         c = bcol[b];
         v = BV ? aj * bval[b] : aj;
+      }
+      if (in_lds) {
+        if (act) {
+          atomicAdd(&lv[c], (unsigned long long)v);
+          atomicOr(&lp[c >> 5], 1u << (c & 31));
+        }
+        continue;
       }
       // Products of one row (a hub row of A fills whole waves) often repeat a column: sorted
       // across the wave and summed per column, they take one atomic per distinct column.
@@ -183,8 +227,29 @@ __global__ void __launch_bounds__(kDThreads) dense_scatter_kernel(
         if (!(__hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(pw, bit);
       }
     }
+    if constexpr (LCOMB) {
+      __syncthreads();
+      if (in_lds) {  // block-uniform: the block's sums of row brow leave, one atomic per column
+        for (int32_t c = threadIdx.x; c < ncols; c += THREADS) {
+          if ((lp[c >> 5] >> (c & 31)) & 1u) {
+            atomicAdd(&D[(int64_t)brow * ncols + c], lv[c]);
+            unsigned long long* pw = &pres[(int64_t)brow * nw + (c >> 6)];
+            const unsigned long long bit = 1ull << (c & 63);
+            if (!(__hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(pw, bit);
+            lv[c] = 0ull;
+          }
+        }
+        __syncthreads();  // every bit read before the words are cleared
+        for (int32_t w = threadIdx.x; w < (ncols + 31) / 32; w += THREADS) lp[w] = 0u;
+      }
+      if (threadIdx.x == 0) {
+        s_lo = INT32_MAX;
+        s_hi = -1;
+      }
+      __syncthreads();
+    }
   }
-  block_add(products, &counters[0]);
+  block_add<NWV>(products, &counters[0]);
 }
 
 // popcount of every (row, 64-word chunk) of the bitmap: cc[u * nch + c]
@@ -279,9 +344,16 @@ template <bool AV, bool BV>
 void launch_scatter(const Csr& A, const Csr& B, int32_t nw, unsigned long long* D, unsigned long long* pres,
                     unsigned long long* counters, hipStream_t st) {
   const int64_t waves = ceil_div(A.nnz, kWave);
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(waves, kDWaves), 2048));
-  dense_scatter_kernel<AV, BV><<<grid, kDThreads, 0, st>>>(A.ptr, A.col, A.val, A.nrows, A.nnz, B.ptr, B.col, B.val,
-                                                           B.ncols, nw, D, pres, counters);
+  if (B.ncols <= kLcombCols) {
+    constexpr int T = 1024;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(waves, T / kWave), 512));
+    dense_scatter_kernel<AV, BV, true, T><<<grid, T, 0, st>>>(A.ptr, A.col, A.val, A.nrows, A.nnz, B.ptr, B.col,
+                                                              B.val, B.ncols, nw, D, pres, counters);
+  } else {
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(waves, kDWaves), 2048));
+    dense_scatter_kernel<AV, BV, false, kDThreads><<<grid, kDThreads, 0, st>>>(
+        A.ptr, A.col, A.val, A.nrows, A.nnz, B.ptr, B.col, B.val, B.ncols, nw, D, pres, counters);
+  }
   SMCM_LAUNCH_CHECK();
 }
 
