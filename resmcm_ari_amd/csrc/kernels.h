@@ -166,6 +166,8 @@ struct SpgemmOpts {
                                 // to the window kernel (LDS value slots are bypassed)
   int64_t dense_cells = (int64_t)1 << 23;  // products with rows(A) x cols(B) up to this many cells
                                            // accumulate whole in a dense device array (0 = never)
+  int window_u32 = 1;       // window accumulators of 32 bits (twice the columns per window) when the
+                            // operands' largest values bound every count below 2^32
   int64_t bbits_bytes = (int64_t)64 << 20;  // right operands whose column bitmap (rows x words) fits
                                             // this take the symbolic pass as ORs of bitmap rows,
                                             // when that reads fewer words than 4x the products (0 = never)

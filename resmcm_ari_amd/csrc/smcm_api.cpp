@@ -150,6 +150,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "window_all_nwin") {
       if (v < 0) fail(SMCM_EINVAL, "window_all_nwin must be >= 0");
       o.window_all_nwin = v;
+    } else if (k == "window_u32") {
+      if (v != 0 && v != 1) fail(SMCM_EINVAL, "window_u32 must be 0 or 1");
+      o.window_u32 = (int)v;
     } else if (k == "bbits_bytes") {
       if (v < 0) fail(SMCM_EINVAL, "bbits_bytes must be >= 0");
       o.bbits_bytes = v;
@@ -235,6 +238,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "window_all_nwin") *v = o.window_all_nwin;
     else if (k == "dense_cells") *v = o.dense_cells;
     else if (k == "bbits_bytes") *v = o.bbits_bytes;
+    else if (k == "window_u32") *v = o.window_u32;
     else if (k == "sym_window_cols") *v = o.sym_window_cols;
     else if (k == "sym_merge") *v = o.sym_merge;
     else if (k == "copy_tier") *v = o.copy_tier;

@@ -51,9 +51,11 @@ constexpr size_t kStageBin = 0, kStageTierNnz = 320, kStageNnz = 384, kStageSpli
 static_assert(kStageSplit + 112 <= kHostStageDense, "host stage sections overlap");
 // byte offsets in a product's device counter block
 // (sums: per tier A entries and products, per tier output entries, per tier largest product
-// count, per tier largest row of A: 5 * kNumBins values)
+// count, per tier largest row of A: 5 * kNumBins values; then the largest row sum of A's values
+// and the largest value of B)
 constexpr size_t kMetaCounts = 0, kMetaQueue = 32, kMetaSums = 64, kMetaSplit = 320, kMetaBytes = 448;
-static_assert(kMetaSums + 5 * 8 * 6 <= kMetaSplit, "tier sums overlap the split counters");
+static_assert(kMetaSums + (5 * 6 + 2) * 8 <= kMetaSplit, "tier sums overlap the split counters");
+constexpr int64_t kMaxValueScan = (int64_t)1 << 25;  // operands scanned for their largest value
 
 // ------------------------------------------------------------------------------------------
 // analysis
@@ -328,6 +330,63 @@ __global__ void __launch_bounds__(kThreads) bbits_heavy_kernel(
       if (wcnt) bbits_window_add(wcnt, u, nwin, log2w - 6, w0, nw, w, c, sc);
     }
     __syncthreads();
+  }
+}
+
+// largest of n uint64 values into *out (atomicMax; *out zeroed by the caller)
+__global__ void __launch_bounds__(kThreads) max_u64_kernel(const uint64_t* __restrict__ v, int64_t n,
+                                                           unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[kThreads / kWave];
+  unsigned long long mx = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    mx = v[i] > mx ? v[i] : mx;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(mx, o);
+    mx = y > mx ? y : mx;
+  }
+  if (lane_id() == 0) part[threadIdx.x / kWave] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 1; w < kThreads / kWave; ++w) mx = part[w] > mx ? part[w] : mx;
+    if (mx) atomicMax(out, mx);
+  }
+}
+
+// largest row sum of A's values into *out (a wave per row; a sum that overflows saturates)
+__global__ void __launch_bounds__(kThreads) row_sum_max_kernel(const int64_t* __restrict__ aptr,
+                                                               const uint64_t* __restrict__ aval, int32_t m,
+                                                               unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[kThreads / kWave];
+  const int lane = lane_id();
+  unsigned long long mx = 0;
+  for (int64_t u = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); u < m;
+       u += (int64_t)gridDim.x * (kThreads / kWave)) {
+    const int64_t rs = aptr[u], re = aptr[u + 1];
+    unsigned long long sum = 0;
+    bool sat = false;
+    for (int64_t e = rs + lane; e < re; e += kWave) {
+      const unsigned long long y = sum + aval[e];
+      sat |= y < sum;
+      sum = y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long y = __shfl_xor(sum, o);
+      const unsigned long long z = sum + y;
+      sat |= z < sum;
+      sum = z;
+    }
+    sat = __ballot(sat) != 0;
+    const unsigned long long r = sat ? ~0ull : sum;
+    mx = r > mx ? r : mx;
+  }
+  if (lane == 0) part[wave_id()] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / kWave; ++w) mx = part[w] > mx ? part[w] : mx;
+    if (mx) atomicMax(out, mx);
   }
 }
 
@@ -1255,7 +1314,8 @@ struct WindowLayout {
   int64_t w = 0;     // window width in columns (power of two >= 64)
   int log2w = 6;
   int64_t nwin = 0;  // windows covering the columns of B
-  size_t bytes() const { return (size_t)w * 8 + (size_t)w / 8 + (size_t)w / 16; }
+  int vbytes = 8;    // LDS accumulator width: 8, or 4 when no count of the product reaches 2^32
+  size_t bytes() const { return (size_t)w * vbytes + (size_t)w / 8 + (size_t)w / 16; }
 };
 
 // Window-major bound table: bnd[w * nrows + k] = offset (within row k of B) of the first element
@@ -1483,7 +1543,9 @@ __global__ void __launch_bounds__(kThreads) window_prep_kernel(const int32_t* __
 // loaded, while the current task runs; the next task's first-round A entries are loaded while the
 // current task writes its output.  Only the B-row bounds and the gathers are left on a task's
 // critical path.
-template <bool AV, bool BV, bool PRES>
+// VT: the LDS accumulator (uint32_t only when the host has bounded every count of the product
+// below 2^32: twice the columns per window for the same LDS, so half the tasks)
+template <bool AV, bool BV, bool PRES, class VT = uint64_t>
 __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint64_t* __restrict__ aval,
                                     const int64_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
                                     const uint64_t* __restrict__ bval, const WinRow* __restrict__ info,
@@ -1494,8 +1556,8 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
                                     uint64_t* __restrict__ cval) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int32_t W = 1 << log2w;
-  uint64_t* vals = (uint64_t*)smem;
-  uint32_t* pres = (uint32_t*)(smem + (size_t)W * 8);
+  VT* vals = (VT*)smem;
+  uint32_t* pres = (uint32_t*)(smem + (size_t)W * sizeof(VT));
   uint64_t* pres64 = (uint64_t*)pres;
   __shared__ TravLds L;
   // task records by parity: slot p holds the current task, slot p ^ 1 receives the next one
@@ -1524,7 +1586,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
     m[2] = r.base + p0;
     m[3] = p1 - p0;
   };
-  for (int i = threadIdx.x; i < W; i += nthreads) vals[i] = 0ull;
+  for (int i = threadIdx.x; i < W; i += nthreads) vals[i] = 0;
   for (int i = threadIdx.x; i < W / 32; i += nthreads) pres[i] = 0u;
   if (threadIdx.x == 0) {
     const int32_t t0 = atomicAdd(queue, 1);
@@ -1577,7 +1639,8 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
       const bool with_pres = pass == 1;
       auto acc = [&](int32_t c, uint64_t v) {
         const int32_t x = (int32_t)(c - w0);
-        atomicAdd((unsigned long long*)&vals[x], (unsigned long long)v);
+        if constexpr (sizeof(VT) == 8) atomicAdd((unsigned long long*)&vals[x], (unsigned long long)v);
+        else atomicAdd((unsigned int*)&vals[x], (unsigned int)v);
         if (with_pres) atomicOr(&pres[x >> 5], 1u << (x & 31));
       };
       for (int64_t base = rs; base < re; base += chunk) {
@@ -1629,7 +1692,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
         for (int32_t j = ja; j < jb; j += kCompactBatch) {
           uint64_t m[kCompactBatch];
 #pragma unroll
-          for (int u = 0; u < kCompactBatch; ++u) m[u] = __ballot(vals[min(j + u, jb - 1) * 64 + lane] != 0ull);
+          for (int u = 0; u < kCompactBatch; ++u) m[u] = __ballot(vals[min(j + u, jb - 1) * 64 + lane] != 0);
 #pragma unroll
           for (int u = 0; u < kCompactBatch; ++u) mine += j + u < jb ? __popcll(m[u]) : 0;
         }
@@ -1672,13 +1735,13 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
           const uint64_t m = readlane_u64(pm, i);
           const int32_t xi = (ja + i) * 64 + lane;
           if ((m >> lane) & 1ull) {
-            const uint64_t v = vals[xi];
+            const uint64_t v = (uint64_t)vals[xi];
             const uint32_t slot = (uint32_t)(pos0 + lanes_below(m));
             if ((int32_t)slot < target) {
               __builtin_nontemporal_store(c0 + xi, &cc[slot]);
               __builtin_nontemporal_store(v, &cv[slot]);
             }
-            vals[xi] = 0ull;
+            vals[xi] = 0;
           }
           if (lane == 0) pres64[ja + i] = 0ull;
           pos0 += __popcll(m);
@@ -1688,7 +1751,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
         for (int32_t j = ja; j < jb; j += kCompactBatch) {
           uint64_t v[kCompactBatch];
 #pragma unroll
-          for (int u = 0; u < kCompactBatch; ++u) v[u] = vals[min(j + u, jb - 1) * 64 + lane];
+          for (int u = 0; u < kCompactBatch; ++u) v[u] = (uint64_t)vals[min(j + u, jb - 1) * 64 + lane];
 #pragma unroll
           for (int u = 0; u < kCompactBatch; ++u) {
             if (j + u < jb) {
@@ -1700,7 +1763,7 @@ __global__ void window_major_kernel(const int32_t* __restrict__ acol, const uint
                 __builtin_nontemporal_store(c0 + xi, &cc[slot]);
                 __builtin_nontemporal_store(v[u], &cv[slot]);
               }
-              if (m) vals[xi] = 0ull;
+              if (m) vals[xi] = 0;
               pos0 += __popcll(m);
             }
           }
@@ -2265,7 +2328,9 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
         // the compaction's presence pass reads a wave's run of words with one lane each
         if ((split->wl.w / 64 + o.w_threads / kWave - 1) / (o.w_threads / kWave) > kWave)
           throw HipError(-22, "window tier: more than 64 words per wave in a window");
-        auto kern = pres ? window_major_kernel<AV, BV, true> : window_major_kernel<AV, BV, false>;
+        auto kern = split->wl.vbytes == 4
+                        ? (pres ? window_major_kernel<AV, BV, true, uint32_t> : window_major_kernel<AV, BV, false, uint32_t>)
+                        : (pres ? window_major_kernel<AV, BV, true> : window_major_kernel<AV, BV, false>);
         const unsigned gm = persistent_grid(o, occupancy((const void*)kern, o.w_threads, lds));
         if (!split->nseg_tried) {
           split->nseg = split->build_segs(B, 1, st);
@@ -2334,6 +2399,8 @@ void set_lds_attr(size_t bytes) {
     lds_attr((const void*)window_tier_kernel<AV, BV>, "window_tier_kernel", bytes);
     lds_attr((const void*)window_major_kernel<AV, BV, false>, "window_major_kernel", bytes);
     lds_attr((const void*)window_major_kernel<AV, BV, true>, "window_major_kernel", bytes);
+    lds_attr((const void*)window_major_kernel<AV, BV, false, uint32_t>, "window_major_kernel", bytes);
+    lds_attr((const void*)window_major_kernel<AV, BV, true, uint32_t>, "window_major_kernel", bytes);
   }
 }
 
@@ -2374,19 +2441,24 @@ BitmapLayout choose_layout(int64_t ncols, const SpgemmOpts& o) {
   return l;
 }
 
-WindowLayout choose_window(int64_t ncols, const SpgemmOpts& o) {
+// widest window whose accumulators (vbytes each) fit the LDS budget; window_cols caps the width
+// of 8-byte windows, and of 4-byte ones at twice that (the same LDS)
+WindowLayout choose_window(int64_t ncols, const SpgemmOpts& o, int vbytes = 8) {
   const int64_t budget = o.lds_bytes - kStaticLds;
   WindowLayout l;
   l.w = 64;
   l.log2w = 6;
+  l.vbytes = vbytes;
   while (l.w < (1 << 20)) {
     WindowLayout n2;
     n2.w = l.w * 2;
+    n2.vbytes = vbytes;
     if ((int64_t)n2.bytes() > budget || n2.w > std::max<int64_t>(64, 2 * ncols)) break;
     l.w = n2.w;
     l.log2w += 1;
   }
-  while (o.window_cols > 0 && l.w > o.window_cols && l.w > 64) {
+  const int64_t cap = o.window_cols > 0 ? o.window_cols * (8 / vbytes) : 0;
+  while (cap > 0 && l.w > cap && l.w > 64) {
     l.w >>= 1;
     l.log2w -= 1;
   }
@@ -2465,7 +2537,22 @@ struct SpgemmCore {
   void bin(int32_t r0, int32_t r1) {
     hipStream_t st = al.stream;
     fill_i32(counts, kNumBins + 1, 0, st);
-    fill_i64((int64_t*)sums, 5 * kNumBins, 0, st);
+    fill_i64((int64_t*)sums, 5 * kNumBins + 2, 0, st);
+    // the operands' largest values (a bound on every count, for 32-bit window accumulators):
+    // scanned with the first binning of the whole product when the operands are small enough
+    const bool whole = r0 == 0 && r1 == A.nrows;
+    const bool scan_a = whole && A.val && A.nnz > 0 && A.nnz <= kMaxValueScan;
+    const bool scan_b = whole && B.val && B.nnz > 0 && B.nnz <= kMaxValueScan;
+    if (scan_a) {
+      row_sum_max_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nrows, kThreads / kWave), 1024), kThreads, 0,
+                           st>>>(A.ptr, A.val, A.nrows, sums + 5 * kNumBins);
+      SMCM_LAUNCH_CHECK();
+    }
+    if (scan_b) {
+      max_u64_kernel<<<(unsigned)std::min<int64_t>(ceil_div(B.nnz, kThreads), 1024), kThreads, 0, st>>>(
+          B.val, B.nnz, sums + 5 * kNumBins + 1);
+      SMCM_LAUNCH_CHECK();
+    }
     if (r1 > r0) {
       bin_rows_kernel<<<(unsigned)ceil_div(r1 - r0, (int64_t)kThreads * kBinRowsPerThread), kThreads, 0, st>>>(
           A.ptr, r0, r1, A.nrows, S.p, est_r, est_stride, (double)B.ncols, o.s_max, o.m_max, o.dense_div, o.heavy_ub,
@@ -2476,10 +2563,10 @@ struct SpgemmCore {
     unsigned char* hs = host_stage() + kStageBin;
     // tier row counts at +0, tier sums at +64 (largest product counts after the width sums) in
     // both blocks: one copy
-    SMCM_HIP(hipMemcpyAsync(hs, meta.p + kMetaCounts, kMetaSums + sizeof(unsigned long long) * 5 * kNumBins,
+    SMCM_HIP(hipMemcpyAsync(hs, meta.p + kMetaCounts, kMetaSums + sizeof(unsigned long long) * (5 * kNumBins + 2),
                             hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipStreamSynchronize(st));
-    unsigned long long sh[5 * kNumBins];
+    unsigned long long sh[5 * kNumBins + 2];
     memcpy(tc.rows, hs, sizeof(int32_t) * kNumBins);
     memcpy(sh, hs + 64, sizeof(sh));
     for (int t = 0; t < kNumBins; ++t) {
@@ -2490,9 +2577,24 @@ struct SpgemmCore {
       tc.max_d[t] = (int64_t)sh[4 * kNumBins + t];
     }
     // every row of A binned: the products of all tiers are the product's total (S[nnz(A)])
-    if (r0 == 0 && r1 == A.nrows) {
+    if (whole) {
       products = 0;
       for (int t = 0; t < kNumBins; ++t) products += tc.products[t];
+      // 32-bit window accumulators when no count can reach 2^32: a count of row u is
+      // sum_e A[u,e] * B[k_e, c] <= (row sum of A's values) x max(B); without values a row sums
+      // to its length.  Unscanned operands bound nothing.
+      int64_t widest = 0;
+      for (int t = 0; t < kNumBins; ++t) widest = std::max(widest, tc.max_d[t]);
+      const long double arow = A.val ? (scan_a ? (long double)sh[5 * kNumBins] : 0x1p64L) : (long double)widest;
+      const long double bmax = B.val ? (scan_b ? (long double)sh[5 * kNumBins + 1] : 0x1p64L) : 1.0L;
+      const bool window_rows = tc.rows[kBitmapTier] + tc.rows[kHeavyTier] > 0;
+      if (o.window_u32 && split.wcnt && window_rows && arow * bmax < 4294967296.0L) {
+        const WindowLayout w4 = choose_window(B.ncols, o, 4);
+        if (w4.w > split.wl.w) {
+          split.wl = w4;
+          set_all_lds_attrs(split.wl.bytes());
+        }
+      }
     }
   }
 
@@ -2665,9 +2767,12 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
     stats->nnz = nnz;
   }
   if (trace)
-    fprintf(stderr, "[smcm] spgemm %dx%d * %dx%d: analysis %.3f symbolic %.3f numeric(+alloc) %.3f ms, nnz %lld\n",
+    fprintf(stderr,
+            "[smcm] spgemm %dx%d * %dx%d: analysis %.3f symbolic %.3f numeric(+alloc) %.3f ms, nnz %lld, "
+            "windows of %lld columns with %d-byte accumulators\n",
             A.nrows, A.ncols, B.nrows, B.ncols, std::chrono::duration<double, std::milli>(t1 - t0).count(),
-            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), (long long)nnz);
+            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), (long long)nnz,
+            (long long)core.split.wl.w, core.split.wl.vbytes);
 }
 
 void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_stride, int64_t block_bytes,

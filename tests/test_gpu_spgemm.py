@@ -113,6 +113,33 @@ def test_bbits_symbolic_used():
     assert any(got) and not all(got)
 
 
+@pytest.mark.parametrize("window", [64, 512, 8192])
+@pytest.mark.parametrize("mode", ["value", "presence"])
+@pytest.mark.parametrize("vmax", [60, 1 << 40])
+def test_window_u32(ctx, capfd, monkeypatch, window, mode, vmax):
+    """Window accumulators of 32 bits when every count is bounded below 2^32 by (row sum of A's
+    values) x max(B) (twice the columns per window), 64 bits otherwise; rows of a few to many
+    windows."""
+    rng = np.random.default_rng(zlib.crc32(f"u32/{window}/{mode}/{vmax}".encode()))
+    A = random_csr(rng, 120, 300, row_len=lambda r: r.integers(1, 60))
+    B = random_csr(rng, 300, 20000, row_len=lambda r: r.integers(50, 900))
+    A.val = rng.integers(1, vmax, size=A.nnz, dtype=np.uint64)
+    B.val = rng.integers(1, vmax, size=B.nnz, dtype=np.uint64)
+    ref = O.spgemm(A, B)
+    monkeypatch.setenv("SMCM_TRACE", "1")
+    with options(ctx, force_tier=2, bitmap_vcap=0, window_cols=window, window_min_products=0,
+                 window_presence=1 if mode == "presence" else 0):
+        capfd.readouterr()
+        ptr, col, val = _run(ctx, A, B)
+        err = capfd.readouterr().err
+    rowsum = max(int(A.val[A.ptr[i]:A.ptr[i + 1]].astype(object).sum()) for i in range(A.nrows))
+    small = rowsum * int(B.val.max()) < 1 << 32
+    assert small == (vmax < 1 << 16)
+    want = f"windows of {window * 2 if small else window} columns with {4 if small else 8}-byte accumulators"
+    assert want in err, err[-400:]
+    assert_same_csr(ptr, col, val, ref, what=f"u32 window={window} {mode} vmax={vmax}")
+
+
 DENSE_CASES = [
     # (name, m, k, n, A row length, B row length, A duplicates)
     ("one_cell", 1, 1, 1, lambda r: 1, lambda r: 1, False),
