@@ -10,7 +10,7 @@
 //      (device atomics) and the presence bit of (u, col) is set.  Work is split by products, not
 //      by rows, so a hub row of A (tens of thousands of entries) is spread over many waves
 //      instead of serialising one workgroup;
-//   2. count: a wave per (row, 64-word chunk of the bitmap) pops its words; one scan over all
+//   2. count: a wave per (row, 16-word chunk of the bitmap) pops its words; one scan over all
 //      chunks gives every chunk its output offset (and the row pointers of C);
 //   3. compact: a wave per (row, chunk) writes its columns in order (lane = bit, 8 words of
 //      loads in flight) with their counts, and returns every touched word of D and of the bitmap
@@ -29,6 +29,7 @@ namespace {
 
 constexpr int kDThreads = 256;
 constexpr int kDWaves = kDThreads / kWave;
+constexpr int kDChunk = 16;  // presence words per count / compaction task
 
 // row of entry e0: the u with ptr[u] <= e0 < ptr[u+1] (0 <= e0 < ptr[m]).  64 probes per step
 // narrow [lo, hi) by 64x, so m rows take ceil(log64 m) dependent loads.
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(THREADS) dense_scatter_kernel(
   block_add<NWV>(products, &counters[0]);
 }
 
-// popcount of every (row, 64-word chunk) of the bitmap: cc[u * nch + c]
+// popcount of every (row, kDChunk-word chunk) of the bitmap: cc[u * nch + c]
 __global__ void __launch_bounds__(kDThreads) dense_count_kernel(int32_t m, int32_t nw, int32_t nch,
                                                                 const unsigned long long* __restrict__ pres,
                                                                 int64_t* __restrict__ cc) {
@@ -260,16 +261,17 @@ __global__ void __launch_bounds__(kDThreads) dense_count_kernel(int32_t m, int32
   const int64_t tasks = (int64_t)m * nch;
   for (int64_t t = (int64_t)blockIdx.x * kDWaves + wave_id(); t < tasks; t += (int64_t)gridDim.x * kDWaves) {
     const int64_t u = t / nch;
-    const int32_t w = (int32_t)(t - u * nch) * kWave + lane;
-    int64_t s = w < nw ? __popcll(pres[u * nw + w]) : 0;
+    const int32_t w = (int32_t)(t - u * nch) * kDChunk + lane;
+    int64_t s = (lane < kDChunk && w < nw) ? __popcll(pres[u * nw + w]) : 0;
     s = wave_sum_i64(s);
     if (lane == 0) cc[t] = s;
   }
 }
 
-// Task t = (row u, chunk c of 64 presence words) writes its columns in order from off[t] (the
-// exclusive prefix of the chunk counts): lane = bit, 8 words per step with their loads of D in
-// flight together; D and the bitmap are cleared behind.  Chunk 0 of a row writes cptr[u].
+// Task t = (row u, chunk c of kDChunk presence words) writes its columns in order from off[t]
+// (the exclusive prefix of the chunk counts): lane = bit, 8 words per step with their loads of D
+// in flight together; D and the bitmap are cleared behind.  Chunk 0 of a row writes cptr[u].
+// Short chunks keep a dense row's words spread over many waves.
 __global__ void __launch_bounds__(kDThreads) dense_compact_kernel(
     int32_t m, int32_t ncols, int32_t nw, int32_t nch, unsigned long long* __restrict__ pres,
     unsigned long long* __restrict__ D, const int64_t* __restrict__ off, int64_t* __restrict__ cptr,
@@ -284,10 +286,10 @@ __global__ void __launch_bounds__(kDThreads) dense_compact_kernel(
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
   for (int64_t t = (int64_t)blockIdx.x * kDWaves + wave_id(); t < tasks; t += (int64_t)gridDim.x * kDWaves) {
     const int64_t u = t / nch;
-    const int32_t w0 = (int32_t)(t - u * nch) * kWave;
+    const int32_t w0 = (int32_t)(t - u * nch) * kDChunk;
     const int64_t base = off[t];
     if (w0 == 0 && lane == 0) cptr[u] = base;
-    const int32_t wn = nw - w0 < kWave ? nw - w0 : kWave;
+    const int32_t wn = nw - w0 < kDChunk ? nw - w0 : kDChunk;
     unsigned long long* prow = pres + u * nw + w0;
     unsigned long long* drow = D + u * ncols + (int64_t)w0 * kWave;
     const uint64_t x = lane < wn ? prow[lane] : 0ull;
@@ -367,7 +369,7 @@ bool spgemm_dense(const Csr& A, const Csr& B, Csr& C, const SpgemmOpts& o, Spgem
     return false;
   hipStream_t st = al.stream;
   const int32_t nw = (int32_t)ceil_div(ncols, kWave);
-  const int32_t nch = (int32_t)ceil_div(nw, kWave);
+  const int32_t nch = (int32_t)ceil_div(nw, kDChunk);
   const int64_t tasks = m * nch;
   const size_t dbytes = (size_t)cells * 8, pbytes = (size_t)m * nw * 8;
   unsigned char* z = zero_block(al, dbytes + pbytes + 64);

@@ -227,14 +227,24 @@ __device__ __forceinline__ void bbits_window_add(int32_t* __restrict__ wcnt, int
 // Light rows (at most kBbitsLight entries): a wave per row, the row's bitmap rows read eight at
 // a time, widths into row_nnz and per-window counts into wcnt (windows of 2^log2w columns,
 // log2w >= 6) when set.  Heavier rows are appended to heavy[] (count in *nheavy, zeroed).
+// the non-empty rows of a product: the binning's tier lists, concatenated by index
+struct RowLists {
+  const int32_t* rows[kNumBins];
+  int32_t start[kNumBins + 1];  // exclusive prefix of the list lengths
+};
+
 __global__ void __launch_bounds__(kThreads) bbits_symbolic_kernel(
     const int64_t* __restrict__ aptr, const int32_t* __restrict__ acol, const unsigned long long* __restrict__ bits,
-    int32_t nw, int32_t m, int log2w, int32_t nwin, int64_t* __restrict__ row_nnz, int32_t* __restrict__ wcnt,
+    int32_t nw, RowLists lists, int log2w, int32_t nwin, int64_t* __restrict__ row_nnz, int32_t* __restrict__ wcnt,
     int32_t* __restrict__ heavy, int32_t* __restrict__ nheavy) {
   const int lane = lane_id();
   const int wsh = log2w - 6;  // log2 of the words per window
-  for (int64_t u = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); u < m;
-       u += (int64_t)gridDim.x * (kThreads / kWave)) {
+  const int32_t n = lists.start[kNumBins];
+  for (int64_t i = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); i < n;
+       i += (int64_t)gridDim.x * (kThreads / kWave)) {
+    int t = 0;
+    while (i >= lists.start[t + 1]) ++t;  // wave-uniform, at most kNumBins steps
+    const int64_t u = lists.rows[t][i - lists.start[t]];
     const int64_t rs = uniform_i64(aptr[u]), re = uniform_i64(aptr[u + 1]);
     if (re - rs > kBbitsLight) {
       if (lane == 0) heavy[atomicAdd(nheavy, 1)] = (int32_t)u;
@@ -2633,10 +2643,21 @@ struct SpgemmCore {
             B.ptr, B.col, B.nrows, nw, bits.p);
         SMCM_LAUNCH_CHECK();
       }
-      bbits_symbolic_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nrows, wpb), 65536), kThreads, 0, st>>>(
-          A.ptr, A.col, bits.p, nw, A.nrows, split.wl.log2w, (int32_t)split.wl.nwin, row_nnz, split.wcnt, heavy.p,
-          nheavy);
-      SMCM_LAUNCH_CHECK();
+      // rows of every tier but the empty one (their widths stay 0)
+      RowLists lists{};
+      int32_t total = 0;
+      for (int t = 0; t < kNumBins; ++t) {
+        lists.rows[t] = bins.p + (size_t)t * A.nrows;
+        lists.start[t] = total;
+        total += t == kEmpty ? 0 : tc.rows[t];
+      }
+      lists.start[kNumBins] = total;
+      if (total > 0) {
+        bbits_symbolic_kernel<<<(unsigned)std::min<int64_t>(ceil_div(total, wpb), 65536), kThreads, 0, st>>>(
+            A.ptr, A.col, bits.p, nw, lists, split.wl.log2w, (int32_t)split.wl.nwin, row_nnz, split.wcnt, heavy.p,
+            nheavy);
+        SMCM_LAUNCH_CHECK();
+      }
       bbits_heavy_kernel<<<(unsigned)(o.cus * 8), kThreads, 0, st>>>(A.ptr, A.col, bits.p, nw, split.wl.log2w,
                                                                        (int32_t)split.wl.nwin, heavy.p, nheavy,
                                                                        row_nnz, split.wcnt);
