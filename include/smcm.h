@@ -80,7 +80,11 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
  * smcm_cost_estimate), "lgraph_seed", "window_major", "window_min_products", "window_cols",
  * "w_threads", "window_all_nwin", "window_presence", "sym_window_cols", "sym_merge", "dense_cells"
  * (products with rows x columns up to this many cells accumulate whole in a dense device array,
- * default 2^23, 0 = never), "budget" (device
+ * default 2^23, 0 = never), "bbits_bytes" (largest right operand kept as a column bitmap for the
+ * symbolic pass, default 64 MiB, 0 = never), "window_u32" (1 = 32-bit window accumulators when
+ * every count of a product is bounded below 2^32, the default; 0 = always 64-bit), "oom_inject"
+ * (tests of the out-of-memory recovery: the n-th device allocation from now reports out of
+ * memory once, then the allocator releases its cache and tries again), "budget" (device
  * bytes the product-aware plan may assume: products whose estimated operands and output exceed it
  * are not planned when another plan avoids them). */
 int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);

@@ -41,14 +41,22 @@ struct DevAlloc {
   // A block kept all-zero between uses (the dense SpGEMM tier's accumulators): a user returns
   // every byte it changed to zero, and marks the block dirty while it may not have; a dirty block
   // is cleared before its next use.  Freed with the cache.
+  // While a product holds pointers into the block (zero_pins > 0), release_cache() keeps it: an
+  // allocation failing in the middle of that product releases the cache and trims the device
+  // pool, which would otherwise unmap the block under the kernels still to run on it.
   void* zero_block = nullptr;
   size_t zero_bytes = 0;
   bool zero_dirty = false;
+  int zero_pins = 0;
+  // Failure injection for tests of the recovery path: when oom_inject > 0, the oom_inject-th
+  // device allocation from now on takes the out-of-memory path once (as if hipMallocAsync had
+  // failed) before its real attempt.
+  int64_t oom_inject = 0;
 
   // hipMallocAsync on this stream; on failure the cache is released and the pool trimmed once
   void* raw_alloc(size_t bytes) {
     void* p = nullptr;
-    hipError_t e = hipMallocAsync(&p, bytes, stream);
+    hipError_t e = (oom_inject > 0 && --oom_inject == 0) ? hipErrorOutOfMemory : hipMallocAsync(&p, bytes, stream);
     if (e == hipErrorOutOfMemory) {
       (void)hipGetLastError();
       if (on_oom) on_oom();
@@ -136,7 +144,7 @@ struct DevAlloc {
     for (auto& kv : cache) (void)hipFreeAsync(kv.second, stream);
     cache.clear();
     cached_bytes = 0;
-    free_zero_block();
+    if (zero_pins == 0) free_zero_block();
   }
 };
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <chrono>
@@ -102,6 +103,9 @@ struct smcm_ctx {
                                            // worker contexts (own stream and allocator each)
   } batch;
   std::vector<smcm_ctx*> batch_workers;  // created on first use, destroyed with the context
+  // set while the batch workers run jobs: their allocators are then in use by their own threads
+  // and an allocation failing on this context must not release their caches
+  std::atomic<bool> batch_running{false};
   smcm_stats last{};
   int64_t live = 0;  // graphs and results created from this context and not yet freed
   smcm::KernelTimer timer;

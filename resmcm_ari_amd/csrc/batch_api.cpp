@@ -244,9 +244,10 @@ std::vector<smcm_ctx*>& batch_workers(smcm_ctx* ctx, int n) {
     ctx->batch_workers.push_back(w);
   }
   // the workers keep their cached blocks between calls (they are reused by the next batch); an
-  // allocation of the calling context that fails takes them back first (workers are idle outside
-  // smcm_batch_run, which runs them and waits for them)
+  // allocation of the calling context that fails takes them back first, unless they are running
+  // jobs (a typed matrix built for one of their jobs allocates on the calling context)
   ctx->al.on_oom = [ctx] {
+    if (ctx->batch_running.load()) return;
     for (smcm_ctx* w : ctx->batch_workers) {
       (void)hipStreamSynchronize(w->stream);
       w->al.release_cache();
@@ -281,6 +282,12 @@ void run_large_jobs(smcm_ctx* ctx, smcm_graph* g, std::vector<BatchChain>& ch, s
   std::vector<int> status(ch.size(), SMCM_OK);
   std::vector<std::string> errors(ch.size());
   {
+    struct Running {
+      smcm_ctx* c;
+      explicit Running(smcm_ctx* x) : c(x) { c->batch_running.store(true); }
+      ~Running() { c->batch_running.store(false); }
+    } running(ctx);
+    // declared after `running`: the workers are joined before the flag is cleared
     smcm_sched::Scheduler sched(nw, 0, ctx->budget);
     sched.start([&](smcm_sched::Job& j, int wi, bool) {
       LargeJob& job = static_cast<LargeJob&>(j);

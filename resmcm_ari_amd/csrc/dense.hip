@@ -342,6 +342,16 @@ unsigned char* zero_block(DevAlloc& al, size_t bytes) {
   return (unsigned char*)al.zero_block;
 }
 
+// holds the all-zero block for one product: an allocation that fails and releases the cache in
+// the meantime leaves the block mapped
+struct ZeroPin {
+  DevAlloc& al;
+  explicit ZeroPin(DevAlloc& a) : al(a) { al.zero_pins++; }
+  ~ZeroPin() { al.zero_pins--; }
+  ZeroPin(const ZeroPin&) = delete;
+  ZeroPin& operator=(const ZeroPin&) = delete;
+};
+
 template <bool AV, bool BV>
 void launch_scatter(const Csr& A, const Csr& B, int32_t nw, unsigned long long* D, unsigned long long* pres,
                     unsigned long long* counters, hipStream_t st) {
@@ -373,6 +383,7 @@ bool spgemm_dense(const Csr& A, const Csr& B, Csr& C, const SpgemmOpts& o, Spgem
   const int64_t tasks = m * nch;
   const size_t dbytes = (size_t)cells * 8, pbytes = (size_t)m * nw * 8;
   unsigned char* z = zero_block(al, dbytes + pbytes + 64);
+  ZeroPin pin(al);  // D, pres and counters point into the block until the compaction is enqueued
   auto* D = (unsigned long long*)z;
   auto* pres = (unsigned long long*)(z + dbytes);
   auto* counters = (unsigned long long*)(z + dbytes + pbytes);

@@ -205,6 +205,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
       ctx->batch.streams = (int)v;
     } else if (k == "kernel_timing") {
       ctx->timer.enabled = v != 0;
+    } else if (k == "oom_inject") {
+      if (v < 0) fail(SMCM_EINVAL, "oom_inject must be >= 0");
+      ctx->al.oom_inject = v;
     } else if (k == "budget") {
       if (v <= 0) fail(SMCM_EINVAL, "budget must be > 0 bytes");
       ctx->budget = (uint64_t)v;
@@ -258,6 +261,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "batch_arena_mb") *v = ctx->batch.arena_entries * 12 / ((int64_t)1 << 20);
     else if (k == "batch_streams") *v = ctx->batch.streams;
     else if (k == "kernel_timing") *v = ctx->timer.enabled;
+    else if (k == "oom_inject") *v = ctx->al.oom_inject;
     else if (k == "lds_bytes") *v = o.lds_bytes;
     else if (k == "cus") *v = o.cus;
     else if (k == "budget") *v = (int64_t)ctx->budget;

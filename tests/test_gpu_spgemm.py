@@ -184,6 +184,38 @@ def test_dense_tier_wraps_mod_2_64(ctx):
     assert 0 in set(val.tolist())
 
 
+@pytest.mark.parametrize("route", ["dense", "window_major"])
+def test_out_of_memory_recovery(ctx, route):
+    """Every device allocation of a product in turn reports out of memory once (injected on the
+    host): the allocator releases its cached blocks and trims the device pool, then tries again.
+    The dense tier keeps its all-zero block through that (its kernels still write into it), and
+    every result stays bit-exact."""
+    rng = np.random.default_rng(zlib.crc32(f"oom/{route}".encode()))
+    if route == "dense":
+        A = random_csr(rng, 300, 50, row_len=lambda r: r.integers(0, 6), with_vals=True)
+        B = random_csr(rng, 50, 4000, row_len=lambda r: r.integers(2000, 4000) if r.random() < 0.1 else 3,
+                       with_vals=True)
+        opts = {"dense_cells": 1 << 28}
+    else:
+        A = random_csr(rng, 64, 400, row_len=lambda r: r.integers(0, 200), with_vals=True)
+        B = random_csr(rng, 400, 30000, row_len=lambda r: r.integers(50, 400), with_vals=True)
+        opts = dict(WIDE_MODES["window_major"], dense_cells=0)
+    ref = O.spgemm(A, B)
+    injected = 0
+    with options(ctx, **opts):
+        _run(ctx, A, B)  # the all-zero block and code objects exist before the injections
+        for k in range(1, 64):
+            ctx.set_option("oom_inject", k)
+            ptr, col, val = _run(ctx, A, B)
+            left = ctx.get_option("oom_inject")
+            ctx.set_option("oom_inject", 0)
+            assert_same_csr(ptr, col, val, ref, what=f"{route}, out of memory at allocation {k}")
+            if left > 0:  # fewer than k allocations in the call: every one has been covered
+                break
+            injected += 1
+    assert injected >= 4, injected
+
+
 @pytest.mark.parametrize("cap", [64, 1024, 2048, 4096])
 @pytest.mark.parametrize("n", [1, 31, 33, 4096, 32768])
 @pytest.mark.parametrize("shape", ["long", "short"])
