@@ -269,13 +269,13 @@ struct LargeJob : smcm_sched::Job {
 };
 
 // the tiered-path jobs `ids`, several at a time on worker contexts; jobs that ran out of device
-// memory next to others run again alone on ctx
-void run_large_jobs(smcm_ctx* ctx, smcm_graph* g, std::vector<BatchChain>& ch, std::vector<int32_t> ids,
+// memory next to others run again alone on ctx (their number is returned)
+int32_t run_large_jobs(smcm_ctx* ctx, smcm_graph* g, std::vector<BatchChain>& ch, std::vector<int32_t> ids,
                     bool keep) {
   const int nw = std::min<int>(ctx->batch.streams, (int)ids.size());
   if (nw <= 1) {
     for (int32_t i : ids) run_large(ctx, g, ch[i], keep);
-    return;
+    return 0;
   }
   std::vector<smcm_ctx*>& workers = batch_workers(ctx, nw);
   std::stable_sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) { return ch[a].est_products > ch[b].est_products; });
@@ -332,9 +332,11 @@ void run_large_jobs(smcm_ctx* ctx, smcm_graph* g, std::vector<BatchChain>& ch, s
     sched.stop();
   }
   bool released = false;
+  int32_t alone = 0;
   for (smcm_ctx* w : workers) SMCM_HIP(hipStreamSynchronize(w->stream));
   for (int32_t i : ids) {
     if (status[i] == 1) {
+      alone++;
       if (!released) {  // the workers' cached blocks go back to the device pool first
         for (smcm_ctx* w : workers) {
           w->al.release_cache();
@@ -350,6 +352,7 @@ void run_large_jobs(smcm_ctx* ctx, smcm_graph* g, std::vector<BatchChain>& ch, s
       fail(status[i], "batch job: " + errors[i]);
     }
   }
+  return alone;
 }
 
 }  // namespace
@@ -668,7 +671,7 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
       if (!ch[i].small) large.push_back(i);
     large.insert(large.end(), rerun.begin(), rerun.end());
     bs.rerun_jobs = (int32_t)rerun.size();
-    run_large_jobs(ctx, g, ch, large, keep);
+    bs.alone_jobs = run_large_jobs(ctx, g, ch, large, keep);
     bs.large_jobs = (int32_t)large.size();
     bs.large_s = now_s() - t_large;
 
