@@ -303,7 +303,7 @@ def bench_batched(hg1, n_chains: int = 4096, ref_sample: int = 1024) -> dict:
             "plain_queries_per_s": round(qps0, 2), "plain_sample_chains": n0, "plain_sample_seconds": round(t0s, 3),
             "speedup": round(qps / qps0, 2),
             "mismatching_chains": bad, "jobs": st["jobs"], "executor_jobs": st["small_jobs"],
-            "tiered_jobs": st["large_jobs"], "alone_jobs": st["alone_jobs"], "plan_s": round(st["plan_s"], 3), "executor_s": round(st["exec_s"], 3),
+            "tiered_jobs": st["large_jobs"], "alone_jobs": st["alone_jobs"], "oom_retries": st["oom_retries"], "plan_s": round(st["plan_s"], 3), "executor_s": round(st["exec_s"], 3),
             "tiered_s": round(st["large_s"], 3),
             "autotuned": {k: tune[k] for k in ("streams", "small_products", "threads", "sort_cap", "tile_rows",
                                                "trials")},

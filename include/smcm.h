@@ -188,8 +188,11 @@ typedef struct smcm_batch_stats {
                                               concurrent worker streams */
   int64_t nodes, tiles, products_est;
   double plan_s, exec_s, large_s, total_s; /* host wall-clock seconds */
-  int32_t alone_jobs, pad;                 /* large jobs that ran out of device memory next to
+  int32_t alone_jobs;                      /* large jobs that ran out of device memory next to
                                               others and ran again by themselves */
+  int32_t oom_retries;                     /* device allocations of the call (all worker
+                                              streams) that failed once and succeeded after the
+                                              allocator released its cached blocks */
 } smcm_batch_stats;
 /* Runs n_chains chains in one call.  Chain i has path_len[i] = L_i matrices; its vertex types
  * are the next L_i+1 entries of vtypes and its edge types the next L_i entries of etypes.

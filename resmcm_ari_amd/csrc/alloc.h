@@ -52,12 +52,14 @@ struct DevAlloc {
   // device allocation from now on takes the out-of-memory path once (as if hipMallocAsync had
   // failed) before its real attempt.
   int64_t oom_inject = 0;
+  int64_t oom_retries = 0;  // allocations that failed once and were tried again after a release
 
   // hipMallocAsync on this stream; on failure the cache is released and the pool trimmed once
   void* raw_alloc(size_t bytes) {
     void* p = nullptr;
     hipError_t e = (oom_inject > 0 && --oom_inject == 0) ? hipErrorOutOfMemory : hipMallocAsync(&p, bytes, stream);
     if (e == hipErrorOutOfMemory) {
+      oom_retries++;
       (void)hipGetLastError();
       if (on_oom) on_oom();
       release_cache();

@@ -373,6 +373,13 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
     const double t_all = now_s();
     smcm_batch_stats bs{};
     bs.chains = n_chains;
+    // allocator retries after an out-of-memory failure, counted over this call
+    auto retries = [ctx] {
+      int64_t r = ctx->al.oom_retries;
+      for (smcm_ctx* w : ctx->batch_workers) r += w->al.oom_retries;
+      return r;
+    };
+    const int64_t retries0 = retries();
     if (results)
       for (int32_t i = 0; i < n_chains; ++i) results[i] = nullptr;
 
@@ -717,6 +724,7 @@ int smcm_batch_run(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const int32_t
     bs.jobs = n_jobs;
     for (int32_t i = 0; i < n_jobs; ++i) bs.products_est += (int64_t)ch[i].est_products;
     bs.total_s = now_s() - t_all;
+    bs.oom_retries = (int32_t)std::min<int64_t>(retries() - retries0, INT32_MAX);
     if (stats) *stats = bs;
   });
 }

@@ -55,7 +55,7 @@ class BatchStats(C.Structure):
     _fields_ = [("chains", C.c_int32), ("jobs", C.c_int32), ("small_jobs", C.c_int32), ("large_jobs", C.c_int32),
                 ("rerun_jobs", C.c_int32), ("groups", C.c_int32), ("nodes", C.c_int64),
                 ("tiles", C.c_int64), ("products_est", C.c_int64), ("plan_s", C.c_double), ("exec_s", C.c_double),
-                ("large_s", C.c_double), ("total_s", C.c_double), ("alone_jobs", C.c_int32), ("pad", C.c_int32)]
+                ("large_s", C.c_double), ("total_s", C.c_double), ("alone_jobs", C.c_int32), ("oom_retries", C.c_int32)]
 
 
 class BatchTuning(C.Structure):
