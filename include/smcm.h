@@ -84,7 +84,10 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
  * symbolic pass, default 64 MiB, 0 = never), "window_u32" (1 = 32-bit window accumulators when
  * every count of a product is bounded below 2^32, the default; 0 = always 64-bit), "oom_inject"
  * (tests of the out-of-memory recovery: the n-th device allocation from now reports out of
- * memory once, then the allocator releases its cache and tries again), "budget" (device
+ * memory once, then the allocator releases its cache and tries again), "headroom_mb" (device
+ * memory kept free for the HIP runtime, default 2048: the scratch of a kernel with a private
+ * segment is allocated at its dispatch, and a failure there aborts the process, so an allocation
+ * that would leave less free fails with SMCM_ENOMEM instead), "budget" (device
  * bytes the product-aware plan may assume: products whose estimated operands and output exceed it
  * are not planned when another plan avoids them). */
 int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);

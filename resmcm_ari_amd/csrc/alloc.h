@@ -6,11 +6,13 @@
 // more than the products themselves.  All work of a context runs on its one stream, so a block
 // freed here can be reused by any later allocation in stream order.  When an allocation fails
 // the cache is released, this stream synchronised and the device pool trimmed before one more
-// try (no device-wide wait: other contexts' streams keep running).
+// try (no device-wide wait: other contexts' streams keep running).  An allocation also fails
+// when it would leave less than `headroom` bytes of the device free for the runtime.
 #pragma once
 
 #include <functional>
 #include <map>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -53,11 +55,50 @@ struct DevAlloc {
   // failed) before its real attempt.
   int64_t oom_inject = 0;
   int64_t oom_retries = 0;  // allocations that failed once and were tried again after a release
+  // Device memory kept free for the runtime: the scratch of kernels with a private segment is
+  // allocated by the runtime when such a kernel is dispatched, and a failure there aborts the
+  // queue (and the process).  An allocation that would grow the device pool into this reserve
+  // takes the out-of-memory path instead (release, trim, retry, then an error the caller sees).
+  size_t headroom = (size_t)2 << 30;
+
+  // can `bytes` be allocated without leaving less than `headroom` free on the device?  The
+  // pool's free part takes it when used + bytes fits its reservation; otherwise the pool grows
+  // by the difference, which must leave the reserve free.  Callers hold pool_mutex().
+  bool room_for(size_t bytes) const {
+    if (headroom == 0) return true;
+    int dev = 0;
+    hipMemPool_t pool;
+    uint64_t reserved = 0, used = 0;
+    size_t freeb = 0, total = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess ||
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) != hipSuccess ||
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) != hipSuccess) {
+      (void)hipGetLastError();
+      return true;  // no pool figures: the allocation itself decides
+    }
+    if (used + bytes <= reserved) return true;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+      (void)hipGetLastError();
+      return true;
+    }
+    return used + bytes - reserved + headroom <= freeb;
+  }
+  // one check-and-allocate at a time in the process: concurrent allocators (batch workers,
+  // server workers) cannot pass the check together and then share the reserve out
+  static std::mutex& pool_mutex() {
+    static std::mutex mu;
+    return mu;
+  }
+  hipError_t try_alloc(void** p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    if (!room_for(bytes)) return hipErrorOutOfMemory;
+    return hipMallocAsync(p, bytes, stream);
+  }
 
   // hipMallocAsync on this stream; on failure the cache is released and the pool trimmed once
   void* raw_alloc(size_t bytes) {
     void* p = nullptr;
-    hipError_t e = (oom_inject > 0 && --oom_inject == 0) ? hipErrorOutOfMemory : hipMallocAsync(&p, bytes, stream);
+    hipError_t e = (oom_inject > 0 && --oom_inject == 0) ? hipErrorOutOfMemory : try_alloc(&p, bytes);
     if (e == hipErrorOutOfMemory) {
       oom_retries++;
       (void)hipGetLastError();
@@ -65,12 +106,13 @@ struct DevAlloc {
       release_cache();
       (void)hipStreamSynchronize(stream);
       trim_pool();
-      e = hipMallocAsync(&p, bytes, stream);
+      e = try_alloc(&p, bytes);
     }
     if (e != hipSuccess) {
       (void)hipGetLastError();
       throw HipError((int)e, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
-                                 hipGetErrorString(e));
+                                 hipGetErrorString(e) + " (" + std::to_string(headroom >> 20) +
+                                 " MiB of the device are kept free for the runtime)");
     }
     return p;
   }

@@ -258,6 +258,7 @@ std::vector<smcm_ctx*>& batch_workers(smcm_ctx* ctx, int n) {
     w->opts = ctx->opts;
     w->exact_cost_sum = ctx->exact_cost_sum;
     w->budget = ctx->budget;
+    w->al.headroom = ctx->al.headroom;
   }
   return ctx->batch_workers;
 }

@@ -205,6 +205,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
       ctx->batch.streams = (int)v;
     } else if (k == "kernel_timing") {
       ctx->timer.enabled = v != 0;
+    } else if (k == "headroom_mb") {
+      if (v < 0) fail(SMCM_EINVAL, "headroom_mb must be >= 0");
+      ctx->al.headroom = (size_t)v << 20;
     } else if (k == "oom_inject") {
       if (v < 0) fail(SMCM_EINVAL, "oom_inject must be >= 0");
       ctx->al.oom_inject = v;
@@ -262,6 +265,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "batch_streams") *v = ctx->batch.streams;
     else if (k == "kernel_timing") *v = ctx->timer.enabled;
     else if (k == "oom_inject") *v = ctx->al.oom_inject;
+    else if (k == "headroom_mb") *v = (int64_t)(ctx->al.headroom >> 20);
     else if (k == "lds_bytes") *v = o.lds_bytes;
     else if (k == "cus") *v = o.cus;
     else if (k == "budget") *v = (int64_t)ctx->budget;

@@ -216,6 +216,29 @@ def test_out_of_memory_recovery(ctx, route):
     assert injected >= 4, injected
 
 
+def test_device_reserve_reports_enomem(ctx):
+    """An allocation that would leave less device memory free than the reserve kept for the HIP
+    runtime fails with SMCM_ENOMEM (an error the caller sees, where a failed scratch allocation of
+    the runtime would abort the process); with the default reserve the same product runs."""
+    from resmcm_ari_amd import smcm
+    rng = np.random.default_rng(17)
+    A = random_csr(rng, 300, 200, row_len=lambda r: r.integers(0, 6), with_vals=True)
+    B = random_csr(rng, 200, 150, row_len=lambda r: r.integers(0, 8), with_vals=True)
+    ref = O.spgemm(A, B)
+    assert ctx.get_option("headroom_mb") == 2048
+    with options(ctx, headroom_mb=1 << 40):  # more than any device holds
+        # the first allocation takes the failure path: the pool is trimmed, so the retry has to
+        # grow it and meets the reserve
+        ctx.set_option("oom_inject", 1)
+        with pytest.raises(smcm.SmcmError) as ei:
+            _run(ctx, A, B)
+        ctx.set_option("oom_inject", 0)
+    assert ei.value.code == smcm.SMCM_ENOMEM
+    assert "kept free for the runtime" in str(ei.value)
+    ptr, col, val = _run(ctx, A, B)
+    assert_same_csr(ptr, col, val, ref, what="after the reserve error")
+
+
 @pytest.mark.parametrize("cap", [64, 1024, 2048, 4096])
 @pytest.mark.parametrize("n", [1, 31, 33, 4096, 32768])
 @pytest.mark.parametrize("shape", ["long", "short"])
