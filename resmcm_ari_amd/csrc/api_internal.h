@@ -101,6 +101,8 @@ struct smcm_ctx {
     int64_t arena_entries = (int64_t)1 << 31;  // executor output arena (12 bytes per entry)
     int streams = 2;                       // jobs of the tiered path run concurrently on this many
                                            // worker contexts (own stream and allocator each)
+    size_t worker_cache = SIZE_MAX;        // freed blocks a worker keeps for its next jobs; the rest
+                                           // go back to the device pool, shared by all streams
   } batch;
   std::vector<smcm_ctx*> batch_workers;  // created on first use, destroyed with the context
   // set while the batch workers run jobs: their allocators are then in use by their own threads

@@ -259,6 +259,7 @@ std::vector<smcm_ctx*>& batch_workers(smcm_ctx* ctx, int n) {
     w->exact_cost_sum = ctx->exact_cost_sum;
     w->budget = ctx->budget;
     w->al.headroom = ctx->al.headroom;
+    w->al.cache_cap = ctx->batch.worker_cache;
   }
   return ctx->batch_workers;
 }
