@@ -104,6 +104,10 @@ struct smcm_ctx {
     size_t worker_cache = SIZE_MAX;        // freed blocks a worker keeps for its next jobs; the rest
                                            // go back to the device pool, shared by all streams
   } batch;
+  struct {
+    size_t worker_cache = (size_t)4 << 30; // the same for each worker of a server created from this
+                                           // context
+  } server;
   std::vector<smcm_ctx*> batch_workers;  // created on first use, destroyed with the context
   // set while the batch workers run jobs: their allocators are then in use by their own threads
   // and an allocation failing on this context must not release their caches

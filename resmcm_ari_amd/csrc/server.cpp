@@ -486,7 +486,7 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
       w->opts = ctx->opts;
       w->exact_cost_sum = ctx->exact_cost_sum;
       w->plan_cost = o.plan_cost;
-      w->al.cache_cap = (size_t)4 << 30;  // keep most freed memory available to the other slots
+      w->al.cache_cap = ctx->server.worker_cache;  // the rest of the freed memory goes to the pool
       w->al.headroom = ctx->al.headroom;
       s->wctx.push_back(w);
       const bool large_lane = o.large_slots > 0 && o.large_slots < o.slots && i >= o.slots - o.large_slots;

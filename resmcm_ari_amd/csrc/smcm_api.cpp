@@ -208,6 +208,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "batch_cache_mb") {
       if (v < 0) fail(SMCM_EINVAL, "batch_cache_mb must be >= 0");
       ctx->batch.worker_cache = (size_t)v << 20;
+    } else if (k == "server_cache_mb") {
+      if (v < 0) fail(SMCM_EINVAL, "server_cache_mb must be >= 0");
+      ctx->server.worker_cache = (size_t)v << 20;
     } else if (k == "headroom_mb") {
       if (v < 0) fail(SMCM_EINVAL, "headroom_mb must be >= 0");
       ctx->al.headroom = (size_t)v << 20;
@@ -269,6 +272,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "kernel_timing") *v = ctx->timer.enabled;
     else if (k == "oom_inject") *v = ctx->al.oom_inject;
     else if (k == "headroom_mb") *v = (int64_t)(ctx->al.headroom >> 20);
+    else if (k == "server_cache_mb") *v = (int64_t)(ctx->server.worker_cache >> 20);
     else if (k == "batch_cache_mb")
       *v = ctx->batch.worker_cache == SIZE_MAX ? INT64_MAX : (int64_t)(ctx->batch.worker_cache >> 20);
     else if (k == "lds_bytes") *v = o.lds_bytes;
