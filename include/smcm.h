@@ -36,6 +36,7 @@ extern "C" {
 #define SMCM_EBUDGET (-28)
 #define SMCM_EDEVICE (-5)
 #define SMCM_EAGAIN (-11) /* smcm_server_poll: the query has not finished */
+#define SMCM_ECANCELED (-125) /* a query's status after smcm_server_cancel stopped it */
 
 typedef struct smcm_ctx smcm_ctx;
 typedef struct smcm_graph smcm_graph;
@@ -279,6 +280,12 @@ int smcm_server_submit(smcm_server* s, int32_t graph, const int32_t* vtypes, con
 /* wait != 0 blocks until the query is done; otherwise SMCM_EAGAIN while it runs.  A ticket is
  * released once its result has been returned. */
 int smcm_server_poll(smcm_server* s, int64_t ticket, int32_t wait, smcm_query_out* out);
+/* Cancels a submitted query (no counterpart in the reference, whose program runs to completion).
+ * A query no worker has started finishes at once with status SMCM_ECANCELED; a running one stops
+ * at its next check (before admission, before each product, between row blocks), its device
+ * memory freed, with that status; a query past its last check finishes normally.  The ticket
+ * stays valid: poll it as usual.  SMCM_EINVAL for an unknown or already polled ticket. */
+int smcm_server_cancel(smcm_server* s, int64_t ticket);
 int smcm_server_stats(smcm_server* s, int64_t* cache_hits, int64_t* cache_inserts, uint64_t* cache_bytes);
 /* Submits n queries at their arrival times (seconds from the call) and waits for all; times in
  * out[] are relative to the call.  Arrays as smcm_batch_run, plus graph[i] and arrival[i]. */

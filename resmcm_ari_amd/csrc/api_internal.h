@@ -512,6 +512,8 @@ struct Chain {
     int32_t i, j;
     bool owned;
   };
+  // set by a server's cancel: checked before every product
+  const std::atomic<bool>* cancel = nullptr;
 
   // resmcm.cpp:709-760 with device products; returns the product (owned by the caller).
   // With a consumer, the root product is materialised in row blocks of at most block_bytes
@@ -539,10 +541,19 @@ struct Chain {
       // the estimator's row densities guide row binning (absent when a cost provider planned)
       const double* est = i < (int32_t)R.size() && R[i].p ? R[i].p + (j - i) : nullptr;
       const bool root = ++done_products == nprod;
-      if (root && consumer)
-        smcm::spgemm_blocked(a.m, b.m, est, L - i, block_bytes, *consumer, ctx->opts, &ss, al, tm);
-      else
-        smcm::spgemm(a.m, b.m, est, L - i, C, ctx->opts, &ss, al, false, tm);
+      // the operands have left the stack: on an error they and the partial product are freed here
+      try {
+        if (cancel && cancel->load()) fail(SMCM_ECANCELED, "query cancelled");
+        if (root && consumer)
+          smcm::spgemm_blocked(a.m, b.m, est, L - i, block_bytes, *consumer, ctx->opts, &ss, al, tm);
+        else
+          smcm::spgemm(a.m, b.m, est, L - i, C, ctx->opts, &ss, al, false, tm);
+      } catch (...) {
+        free_csr(al, C);  // stream-ordered: blocks are reused only after the queued kernels
+        if (a.owned) free_csr(al, a.m);
+        if (b.owned) free_csr(al, b.m);
+        throw;
+      }
       if (on_product && !(root && consumer)) on_product(i, j, C);
       if (stats) {
         stats->products += ss.products;

@@ -223,6 +223,7 @@ int64_t execute_digest(Chain& ch, bool blocked, uint64_t block_bytes, DevAlloc& 
     DevBuf<uint64_t> dig(al, std::max(nrows, 1));
     int64_t total = 0;
     smcm::BlockConsumer consume = [&](const Csr& blk, int32_t b, int32_t e) {
+      if (ch.cancel && ch.cancel->load()) fail(SMCM_ECANCELED, "query cancelled");
       total += blk.nnz;
       if (dig_out) smcm::row_digest(blk, dig.p + b, st);
     };
@@ -252,6 +253,10 @@ int64_t execute_digest(Chain& ch, bool blocked, uint64_t block_bytes, DevAlloc& 
 // Plans and runs one query.  Returns false (query not finished) when a small-lane worker finds
 // it large: the query is then queued for the large-lane workers.
 bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
+  auto check_cancel = [&q] {
+    if (q.cancelled.load()) fail(SMCM_ECANCELED, "query cancelled");
+  };
+  check_cancel();
   smcm_graph* g = s->graphs[q.graph];
   DevAlloc& al = w->al;
   hipStream_t st = al.stream;
@@ -273,14 +278,14 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
     }
     SMCM_HIP(hipStreamSynchronize(st));
   };
-  // admission: held from admit() until the query's device work is done (RAII, also on errors)
+  // admission: held from admit() until the query's device work is done (RAII, also on errors);
+  // a query cancelled while it waits is not admitted
   struct Admission {
     smcm_sched::Scheduler& sc;
     Query* q = nullptr;
     void admit(uint64_t b, Query& qq) {
+      if (!(qq.alone ? sc.admit_alone(qq) : sc.admit(qq, b))) fail(SMCM_ECANCELED, "query cancelled");
       q = &qq;
-      if (qq.alone) sc.admit_alone(qq);
-      else sc.admit(qq, b);
       qq.out.predicted_bytes = b;
     }
     ~Admission() {
@@ -299,6 +304,7 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
     ch.prepare(q.vts.data(), q.ets.data(), q.L, q.whole ? nullptr : q.src.data(), q.whole ? 0 : q.src.size());
   }
   ch.want_products = s->opts.plan_cost == 1 && ch.L > 1;
+  ch.cancel = &q.cancelled;
   if (ch.L == 1) {
     // a single relation: merge duplicates through the per-chain path
     adm.admit(12ull * (uint64_t)ch.mats[0].nnz, q);
@@ -333,9 +339,11 @@ bool run_query(smcm_server* s, smcm_ctx* w, Query& q, bool small_lane) {
     for (int64_t b = 0; b < K; ++b) {
       const int64_t r0 = b * nrows / K, r1 = (b + 1) * nrows / K;
       if (r1 == r0) continue;
+      check_cancel();
       Chain cb{w, g};
       cb.prepare(q.vts.data(), q.ets.data(), q.L, rows.data() + r0, r1 - r0);
       cb.want_products = s->opts.plan_cost == 1;
+      cb.cancel = &q.cancelled;
       cb.estimate();
       cb.dp();
       double rb = 0;
@@ -374,7 +382,7 @@ bool run_job(smcm_server* s, smcm_sched::Job& job, int wi, bool small_lane) {
       // device.  A query that ran out of memory next to others runs once more by itself, after
       // its worker's cached blocks went back to the device pool (its buffers were freed while
       // the exception unwound).
-      if (e.code != (int)hipErrorOutOfMemory || q.alone) throw;
+      if (e.code != (int)hipErrorOutOfMemory || q.alone || q.cancelled.load()) throw;
       (void)hipGetLastError();
       q.alone = true;
       q.out.reruns++;
@@ -550,6 +558,13 @@ int smcm_server_poll(smcm_server* s, int64_t ticket, int32_t wait, smcm_query_ou
   out->t_done = q.t_done;
   std::memset(out->error, 0, sizeof(out->error));
   if (q.status != SMCM_OK) std::strncpy(out->error, q.error.c_str(), sizeof(out->error) - 1);
+  return SMCM_OK;
+}
+
+int smcm_server_cancel(smcm_server* s, int64_t ticket) {
+  if (!s) return set_err(SMCM_EINVAL, "null server");
+  const int rc = s->sched->cancel(ticket, SMCM_ECANCELED, "query cancelled before it started");
+  if (rc < 0) return set_err(SMCM_EINVAL, "unknown ticket");
   return SMCM_OK;
 }
 

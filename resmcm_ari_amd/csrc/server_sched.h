@@ -14,8 +14,14 @@
 // Shutdown (stop) lets the workers finish every submitted job.  A small-lane worker exits once
 // no job is waiting; a large-lane worker only once, in addition, no small-lane worker is busy,
 // because a busy small-lane worker may still hand a job over.
+//
+// Cancellation (cancel): a job no worker has taken yet (queued, or handed over and waiting for the
+// large lane) finishes at once with the given status; a taken job is flagged (Job::cancelled),
+// stops waiting for admission, and its executor ends it at its next check.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
@@ -39,6 +45,7 @@ struct Job {
   double t_submit = 0, t_start = 0, t_done = 0;  // seconds since the scheduler's creation
   uint64_t admitted_bytes = 0;
   bool admitted = false;
+  std::atomic<bool> cancelled{false};  // set by Scheduler::cancel; read by the executor
   virtual ~Job() = default;
 };
 
@@ -107,24 +114,69 @@ class Scheduler {
 
   // blocks until `bytes` fit the budget next to the admitted jobs (or nothing else is admitted);
   // every job counts at least one byte, so a budget of b bytes admits at most b jobs at once.
-  // No job is admitted while a job waits for admit_alone().
-  void admit(Job& j, uint64_t bytes) {
+  // No job is admitted while a job waits for admit_alone().  Returns false, admitting nothing,
+  // when the job is cancelled before or while it waits.
+  bool admit(Job& j, uint64_t bytes) {
     bytes = bytes > 0 ? bytes : 1;
     std::unique_lock<std::mutex> lk(mu_);
-    cv_budget_.wait(lk, [&] { return alone_waiting_ == 0 && (admitted_ == 0 || inflight_ + bytes <= budget_); });
+    cv_budget_.wait(lk, [&] {
+      return j.cancelled.load() || (alone_waiting_ == 0 && (admitted_ == 0 || inflight_ + bytes <= budget_));
+    });
+    if (j.cancelled.load()) return false;
     take(j, bytes);
+    return true;
   }
 
   // blocks until no other job is admitted, then admits j with the whole budget, so nothing else
   // is admitted until j is released: a job that ran out of device memory next to others runs
   // again by itself.  Jobs waiting here are admitted one at a time, before any admit() waiter.
-  void admit_alone(Job& j) {
+  // Returns false, admitting nothing, when the job is cancelled.
+  bool admit_alone(Job& j) {
     std::unique_lock<std::mutex> lk(mu_);
     alone_waiting_++;
-    cv_budget_.wait(lk, [&] { return admitted_ == 0; });
+    cv_budget_.wait(lk, [&] { return j.cancelled.load() || admitted_ == 0; });
     alone_waiting_--;
+    if (j.cancelled.load()) {
+      lk.unlock();
+      cv_budget_.notify_all();  // admit() waiters held back by this one may go on
+      return false;
+    }
     take(j, budget_ > 0 ? budget_ : 1);
     alone_runs_++;
+    return true;
+  }
+
+  // 0: cancelled (a job no worker has taken finishes now with `status` and `msg`; a taken one is
+  // flagged for its executor, which may still finish it normally if it is past its last check);
+  // 1: the job had already finished; -1: unknown ticket (or already polled)
+  int cancel(int64_t ticket, int status, const std::string& msg) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto it = all_.find(ticket);
+      if (it == all_.end()) return -1;
+      Job* j = it->second.get();
+      if (j->done) return 1;
+      j->cancelled = true;
+      auto drop = [&](std::deque<Job*>& dq) {
+        auto q = std::find(dq.begin(), dq.end(), j);
+        if (q == dq.end()) return false;
+        dq.erase(q);
+        return true;
+      };
+      if (drop(pending_) || drop(large_ready_)) {
+        j->status = status;
+        j->error = msg;
+        j->t_done = now();
+        if (j->t_start == 0) j->t_start = j->t_done;
+        j->done = true;
+        completed_++;
+        cancelled_++;
+      }
+    }
+    cv_done_.notify_all();
+    cv_budget_.notify_all();  // a flagged job waiting for admission gives up
+    cv_work_.notify_all();    // large-lane workers waiting to exit re-check the queues
+    return 0;
   }
 
   // the job's device work is done: its budget is freed
@@ -157,13 +209,13 @@ class Scheduler {
   struct Counters {
     uint64_t inflight;
     int admitted, running, small_busy, alone_waiting;
-    int64_t completed, over_budget, alone_runs;
+    int64_t completed, over_budget, alone_runs, cancelled_queued;
     size_t pending, large_ready, tickets;
   };
   Counters counters() {
     std::lock_guard<std::mutex> lk(mu_);
     return Counters{inflight_, admitted_, running_, small_busy_, alone_waiting_, completed_, over_budget_,
-                    alone_runs_, pending_.size(), large_ready_.size(), all_.size()};
+                    alone_runs_, cancelled_, pending_.size(), large_ready_.size(), all_.size()};
   }
 
  private:
@@ -237,6 +289,7 @@ class Scheduler {
   int small_busy_ = 0;
   int alone_waiting_ = 0;  // jobs blocked in admit_alone()
   int64_t completed_ = 0, over_budget_ = 0, alone_runs_ = 0;
+  int64_t cancelled_ = 0;  // jobs cancelled before a worker took them
   bool stop_ = false;
 };
 

@@ -21,6 +21,8 @@ SMCM_EINVAL = -22
 SMCM_ENOMEM = -12
 SMCM_EBUDGET = -28
 SMCM_EDEVICE = -5
+SMCM_EAGAIN = -11
+SMCM_ECANCELED = -125
 
 LIB_PATH = os.path.join(_build.LIBDIR, "libsmcm.so")
 
@@ -34,7 +36,7 @@ EXPORTS = [
     "smcm_result_row_digest", "smcm_result_free", "smcm_spgemm_csr", "smcm_chain_run_blocked",
     "smcm_kernel_stats", "smcm_kernel_stats_reset", "smcm_copy_bandwidth", "smcm_chain_products_estimate",
     "smcm_batch_run", "smcm_server_create", "smcm_server_destroy", "smcm_server_submit", "smcm_server_poll",
-    "smcm_server_stats", "smcm_server_replay", "smcm_batch_autotune", "smcm_cost_estimate",
+    "smcm_server_stats", "smcm_server_replay", "smcm_batch_autotune", "smcm_cost_estimate", "smcm_server_cancel",
 ]
 
 # DP cost providers (option plan_cost and Graph.cost_estimate)
@@ -116,6 +118,7 @@ def load() -> C.CDLL:
             "smcm_server_destroy": [vp],
             "smcm_server_submit": [vp, i32, vp, vp, i32, vp, i64, vp, vp],
             "smcm_server_poll": [vp, i64, i32, vp],
+            "smcm_server_cancel": [vp, i64],
             "smcm_server_stats": [vp, vp, vp, vp],
             "smcm_server_replay": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "smcm_estimator_rows": [vp, vp, vp, vp, i32, vp, i64, i32, vp, i64],
@@ -490,6 +493,7 @@ class Server:
         h = C.c_void_p()
         _check(load().smcm_server_create(ctx._h, arr, len(graphs), C.byref(o), C.byref(h)))
         self._h = h
+        self._pending = {}
         ctx._children.add(self)
 
     def close(self) -> None:
@@ -502,6 +506,37 @@ class Server:
             self.close()
         except Exception:
             pass
+
+    def submit(self, graph: int, path, src_rows=None):
+        """Queues one query; returns (ticket, per-row digest buffer filled when it finishes)."""
+        vts = np.ascontiguousarray(path.vertexTypes, np.int32)
+        ets = np.ascontiguousarray(path.edgeTypes, np.int32)
+        if src_rows is None:
+            src, n = None, int(self.graphs[graph].hg.dimensions[path.vertexTypes[0]])
+        else:
+            src = np.ascontiguousarray(src_rows, np.int32)
+            n = len(src)
+        dig = np.zeros(max(n, 1), np.uint64)
+        t = C.c_int64()
+        _check(load().smcm_server_submit(self._h, graph, _ptr(vts), _ptr(ets), path.pathLen, _ptr(src),
+                                         0 if src is None else len(src), _ptr(dig), C.byref(t)))
+        self._pending[t.value] = (dig, src)  # both stay alive until the query is polled
+        return t.value, dig[:n]
+
+    def poll(self, ticket: int, wait: bool = True):
+        """The query's result dict, or None while it runs (wait=False)."""
+        q = QueryOut()
+        rc = load().smcm_server_poll(self._h, ticket, 1 if wait else 0, C.byref(q))
+        if rc == SMCM_EAGAIN:
+            return None
+        _check(rc)
+        self._pending.pop(ticket, None)
+        d = {k: getattr(q, k) for k, _ in QueryOut._fields_}
+        d["error"] = d["error"].decode(errors="replace")
+        return d
+
+    def cancel(self, ticket: int) -> None:
+        _check(load().smcm_server_cancel(self._h, ticket))
 
     def stats(self) -> dict:
         hits, ins, byts = C.c_int64(), C.c_int64(), C.c_uint64()

@@ -11,7 +11,10 @@
 //           small-lane worker can still hand a job over); the stop_slow_plan variant plans much
 //           longer than it executes, which makes that window wide;
 //   tight : a budget that admits about one job at a time;
-//   nolane: no lanes (every worker takes everything).
+//   nolane: no lanes (every worker takes everything);
+//   cancel: jobs cancelled at random moments (queued, handed over, planning, waiting for
+//           admission, running, done): a cancelled job finishes once, with the cancel status or,
+//           if its executor was past its last check, normally; it never holds budget afterwards.
 // In every scenario some jobs "run out of memory" once they are admitted and run again through
 // admit_alone(), as the server reruns a query that ran out of device memory next to others.
 // Checks: every job finishes exactly once with its status, no admission beyond the budget
@@ -38,6 +41,7 @@ struct StubJob : Job {
   int plan_us = 0, exec_us = 0;
   uint64_t bytes = 0;
   bool big = false, fail = false, oom = false;
+  bool alone_ran = false;
   std::atomic<int> runs{0};
   int ran_on = -1;
 };
@@ -52,6 +56,8 @@ void sleep_us(int us) {
 }
 
 int failures = 0;
+// cancellations over all scenarios: of jobs no worker had taken, and of jobs a worker was running
+std::atomic<int64_t> g_cancel_queued{0}, g_cancel_taken{0};
 #define CHECK(cond, ...)                                 \
   do {                                                   \
     if (!(cond)) {                                       \
@@ -62,8 +68,11 @@ int failures = 0;
     }                                                    \
   } while (0)
 
+constexpr int kCancelStatus = -125;
+
 void run_scenario(const char* name, int slots, int large_slots, uint64_t budget, int njobs, double p_big,
-                  bool stop_early, uint64_t seed, int plan_max_us = 300, int exec_max_us = 600) {
+                  bool stop_early, uint64_t seed, int plan_max_us = 300, int exec_max_us = 600,
+                  double p_cancel = 0.0) {
   {
     std::lock_guard<std::mutex> lk(g_mu);
     g_scenario = std::string(name) + " slots=" + std::to_string(slots) + " large=" + std::to_string(large_slots) +
@@ -82,6 +91,13 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     // planning; on the large lane a handed-over job's plan is shorter, so large-lane workers can
     // run dry while small-lane workers still plan
     sleep_us(small ? q.plan_us : q.plan_us / 20);
+    auto cancelled = [&] {
+      if (!q.cancelled.load()) return false;
+      q.status = kCancelStatus;
+      q.error = "cancelled";
+      return true;
+    };
+    if (cancelled()) return true;
     if (small && q.big) {
       sc.hand_over(q);
       return false;
@@ -92,14 +108,26 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
       q.error = "stub failure";
       return true;
     }
-    sc.admit(q, q.bytes);
+    if (!sc.admit(q, q.bytes)) {
+      cancelled();
+      return true;
+    }
     const auto c = sc.counters();
     if (c.admitted > 1 && c.inflight > budget) over++;
     if (budget == 1 && c.admitted > 1) over++;  // a one-byte budget admits one job at a time
-    sleep_us(q.exec_us);  // device work
+    sleep_us(q.exec_us / 2);  // device work, with a check half way
+    if (cancelled()) {
+      sc.release(q);
+      return true;
+    }
+    sleep_us(q.exec_us - q.exec_us / 2);
     sc.release(q);
     if (q.oom) {  // ran out of memory: once more, by itself
-      sc.admit_alone(q);
+      if (!sc.admit_alone(q)) {
+        cancelled();
+        return true;
+      }
+      q.alone_ran = true;
       if (sc.counters().admitted != 1) not_alone++;
       sleep_us(q.exec_us);
       if (sc.counters().admitted != 1) not_alone++;
@@ -112,6 +140,7 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
   std::uniform_real_distribution<double> u(0.0, 1.0);
   std::vector<int64_t> tickets;
   std::vector<bool> want_fail;
+  std::map<int64_t, int> cancel_rc;  // ticket -> cancel()'s return
   int64_t n_oom = 0;
   std::map<int64_t, std::unique_ptr<Job>> early;  // finished jobs taken by a non-blocking poll
   for (int i = 0; i < njobs; ++i) {
@@ -125,6 +154,14 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     n_oom += j->oom ? 1 : 0;
     want_fail.push_back(j->fail);
     tickets.push_back(sc.submit(std::move(j)));
+    if (u(rng) < p_cancel) {  // cancel a random earlier ticket (it may be done, or polled)
+      const int64_t t = tickets[(size_t)(u(rng) * (double)tickets.size())];
+      if (!cancel_rc.count(t)) {
+        const int rc = sc.cancel(t, kCancelStatus, "cancelled");
+        CHECK(rc == 0 || rc == 1 || (rc == -1 && early.count(t)), "cancel of %lld: rc %d", (long long)t, rc);
+        cancel_rc[t] = rc;
+      }
+    }
     if (!stop_early) {
       sleep_us((int)(u(rng) * 150));
       if (i % 7 == 3) {  // non-blocking poll of an earlier ticket
@@ -154,11 +191,23 @@ void run_scenario(const char* name, int slots, int large_slots, uint64_t budget,
     auto& q = static_cast<StubJob&>(*out);
     ++done;
     CHECK(q.done, "ticket %lld polled before done", (long long)q.ticket);
-    CHECK(q.status == (want_fail[i] ? -5 : 0), "ticket %lld status %d", (long long)q.ticket, q.status);
-    CHECK(q.runs == (q.big && small_lanes > 0 && q.large ? 2 : 1) || (q.runs == 1 && !q.large),
-          "ticket %lld ran %d times (large %d)", (long long)q.ticket, q.runs.load(), (int)q.large);
-    if (q.large) CHECK(q.ran_on >= small_lanes, "handed-over ticket %lld ran on small-lane worker %d",
-                       (long long)q.ticket, q.ran_on);
+    const bool was_cancelled = cancel_rc.count(tickets[i]) && cancel_rc[tickets[i]] == 0;
+    if (was_cancelled) {
+      CHECK(q.status == kCancelStatus || q.status == (want_fail[i] ? -5 : 0), "cancelled ticket %lld status %d",
+            (long long)q.ticket, q.status);
+      if (q.status == kCancelStatus) {
+        (q.runs == 0 || (q.large && q.ran_on < 0) ? g_cancel_queued : g_cancel_taken)++;
+        if (q.oom && !q.alone_ran) n_oom--;  // cancelled before its alone run
+        CHECK(q.runs <= 2, "cancelled ticket %lld ran %d times", (long long)q.ticket, q.runs.load());
+      }
+    } else {
+      CHECK(q.status == (want_fail[i] ? -5 : 0), "ticket %lld status %d", (long long)q.ticket, q.status);
+      CHECK(q.runs == (q.big && small_lanes > 0 && q.large ? 2 : 1) || (q.runs == 1 && !q.large),
+            "ticket %lld ran %d times (large %d)", (long long)q.ticket, q.runs.load(), (int)q.large);
+    }
+    if (q.large && q.ran_on >= 0)
+      CHECK(q.ran_on >= small_lanes, "handed-over ticket %lld ran on small-lane worker %d", (long long)q.ticket,
+            q.ran_on);
     CHECK(q.t_done >= q.t_start && q.t_start >= q.t_submit, "ticket %lld times", (long long)q.ticket);
     CHECK(!q.admitted, "ticket %lld still holds budget", (long long)q.ticket);
     CHECK(sc.poll(tickets[i], false, nullptr) == -1, "ticket %lld polled twice", (long long)q.ticket);
@@ -215,12 +264,18 @@ int main(int argc, char** argv) {
     run_scenario("tight", 4, 2, 10, 60, 0.3, false, seed + 19);
     run_scenario("one_byte", 4, 2, 1, 40, 0.3, false, seed + 37);  // one job at a time
     run_scenario("nolane", 3, 0, 200, 60, 0.3, false, seed + 23);
+    run_scenario("cancel", 8, 3, 200, 120, 0.3, false, seed + 41, 300, 600, 0.4);
+    run_scenario("cancel_tight", 4, 2, 10, 80, 0.3, false, seed + 43, 300, 600, 0.4);
+    run_scenario("cancel_stop", 4, 2, 200, 60, 1.0, true, seed + 47, 3000, 50, 0.5);
   }
   g_finished = true;
+  CHECK(g_cancel_queued > 0 && g_cancel_taken > 0, "cancellations: %lld queued, %lld taken",
+        (long long)g_cancel_queued.load(), (long long)g_cancel_taken.load());
   if (failures) {
     std::fprintf(stderr, "%d check(s) failed\n", failures);
     return 1;
   }
-  std::printf("sched stress ok: %d rounds\n", reps);
+  std::printf("sched stress ok: %d rounds, %lld jobs cancelled before a worker took them, %lld while taken\n", reps,
+              (long long)g_cancel_queued.load(), (long long)g_cancel_taken.load());
   return 0;
 }

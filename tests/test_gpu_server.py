@@ -110,3 +110,40 @@ def test_bad_query_rejected(ctx, served):
             srv.replay([(0, mp, np.array([0], np.int32))], [0.0])  # global id 0 is a paper, not an author
     finally:
         srv.close()
+
+
+def test_cancel(ctx, served):
+    """Cancelled queries finish with SMCM_ECANCELED, whether queued, waiting for admission (a
+    one-byte budget admits one query at a time) or running; a query past its last check finishes
+    normally with its usual result.  The other queries' results are unchanged, and the server
+    keeps serving afterwards."""
+    import time
+    G, queries, arrivals, alone = served
+    srv = smcm.Server(ctx, G, slots=3, budget=1, cache=0)
+    try:
+        tickets = [srv.submit(gi, mp, src) for gi, mp, src in queries[:40]]
+        time.sleep(0.002)  # some queries have started
+        cancelled = set(range(0, 40, 3))
+        for i in sorted(cancelled, reverse=True):
+            srv.cancel(tickets[i][0])
+        n_cancelled = 0
+        for i, (t, dig) in enumerate(tickets):
+            o = srv.poll(t)
+            nnz, ref = alone[i]
+            if i in cancelled and o["status"] == smcm.SMCM_ECANCELED:
+                n_cancelled += 1
+                assert "cancelled" in o["error"], o
+                continue
+            assert o["status"] == 0, (i, o)
+            assert o["nnz"] == nnz, i
+            np.testing.assert_array_equal(dig, ref, err_msg=f"query {i}")
+        assert n_cancelled >= len(cancelled) // 2, n_cancelled
+        with pytest.raises(smcm.SmcmError) as ei:
+            srv.cancel(tickets[0][0])  # polled: the ticket is released
+        assert ei.value.code == smcm.SMCM_EINVAL
+        with pytest.raises(smcm.SmcmError):
+            srv.cancel(1 << 40)
+        outs, digs = srv.replay(queries[:20], np.zeros(20))
+        compare(outs, digs, alone[:20])
+    finally:
+        srv.close()
