@@ -665,22 +665,23 @@ int smcm_spgemm_csr(smcm_ctx* ctx, int32_t m, int32_t k, int32_t n, const int64_
     A.nrows = m;
     A.ncols = k;
     A.nnz = annz;
-    A.ptr = (int64_t*)up(a_ptr, sizeof(int64_t) * (m + 1));
-    A.col = (int32_t*)up(a_col, sizeof(int32_t) * annz);
-    A.val = a_val ? (uint64_t*)up(a_val, sizeof(uint64_t) * annz) : nullptr;
     B.nrows = k;
     B.ncols = n;
     B.nnz = bnnz;
-    B.ptr = (int64_t*)up(b_ptr, sizeof(int64_t) * (k + 1));
-    B.col = (int32_t*)up(b_col, sizeof(int32_t) * bnnz);
-    B.val = b_val ? (uint64_t*)up(b_val, sizeof(uint64_t) * bnnz) : nullptr;
     B.distinct = b_distinct;
     smcm::SpgemmStats ss;
-    try {
+    try {  // operands uploaded so far are freed when an allocation fails
+      A.ptr = (int64_t*)up(a_ptr, sizeof(int64_t) * (m + 1));
+      A.col = (int32_t*)up(a_col, sizeof(int32_t) * annz);
+      A.val = a_val ? (uint64_t*)up(a_val, sizeof(uint64_t) * annz) : nullptr;
+      B.ptr = (int64_t*)up(b_ptr, sizeof(int64_t) * (k + 1));
+      B.col = (int32_t*)up(b_col, sizeof(int32_t) * bnnz);
+      B.val = b_val ? (uint64_t*)up(b_val, sizeof(uint64_t) * bnnz) : nullptr;
       smcm::spgemm(A, B, nullptr, 0, C, ctx->opts, &ss, al, true, ctx->timer.enabled ? &ctx->timer : nullptr);
     } catch (...) {
       free_csr(al, A);
       free_csr(al, B);
+      free_csr(al, C);  // what the product allocated before the failure
       throw;
     }
     free_csr(al, A);
