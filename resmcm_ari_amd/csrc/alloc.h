@@ -157,6 +157,7 @@ struct DevAlloc {
   T* alloc_n(int64_t n) {
     return (T*)alloc(sizeof(T) * (size_t)(n > 0 ? n : 1));
   }
+  // never throws (it runs in destructors, also while an error unwinds)
   void free(void* p) {
     if (!p) return;
     if (arena) {
@@ -175,7 +176,7 @@ struct DevAlloc {
       cached_bytes += sz;
       return;
     }
-    SMCM_HIP(hipFreeAsync(p, stream));
+    if (hipFreeAsync(p, stream) != hipSuccess) (void)hipGetLastError();
   }
   static void trim_pool() {
     int dev = 0;
