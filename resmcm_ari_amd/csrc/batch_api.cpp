@@ -275,6 +275,17 @@ struct LargeJob : smcm_sched::Job {
 int32_t run_large_jobs(smcm_ctx* ctx, smcm_graph* g, std::vector<BatchChain>& ch, std::vector<int32_t> ids,
                     bool keep) {
   const int nw = std::min<int>(ctx->batch.streams, (int)ids.size());
+  // workers beyond this call's count stay idle (an autotuner trial with more streams created
+  // them): their cached blocks go back to the device, where this call's allocations can map them
+  bool freed = false;
+  for (size_t i = (size_t)std::max(nw, 0); i < ctx->batch_workers.size(); ++i) {
+    smcm_ctx* w = ctx->batch_workers[i];
+    if (w->al.cached_bytes == 0 && !w->al.zero_block) continue;
+    w->al.release_cache();
+    SMCM_HIP(hipStreamSynchronize(w->stream));
+    freed = true;
+  }
+  if (freed) smcm::DevAlloc::trim_pool();
   if (nw <= 1) {
     for (int32_t i : ids) run_large(ctx, g, ch[i], keep);
     return 0;
