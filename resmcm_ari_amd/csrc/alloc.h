@@ -5,8 +5,9 @@
 // 1.5x the request), up to cache_cap bytes: remapping tens of GB of HBM for every call costs far
 // more than the products themselves.  All work of a context runs on its one stream, so a block
 // freed here can be reused by any later allocation in stream order.  When an allocation fails
-// the cache is released, this stream synchronised and the device pool trimmed before one more
-// try (no device-wide wait: other contexts' streams keep running).  An allocation also fails
+// the oldest cached blocks, then the whole cache, are released, this stream synchronised and the
+// device pool trimmed before each further try (no device-wide wait: other contexts' streams keep
+// running).  An allocation also fails
 // when it would leave less than `headroom` bytes of the device free for the runtime.
 #pragma once
 
@@ -36,7 +37,8 @@ struct DevAlloc {
   size_t cached_bytes = 0;
   size_t cache_cap = SIZE_MAX;
   std::unordered_map<void*, size_t> sizes;
-  std::multimap<size_t, void*> cache;
+  std::multimap<size_t, std::pair<void*, uint64_t>> cache;  // size -> (block, insertion order)
+  uint64_t cache_seq = 0;
   // called when an allocation fails, before this allocator's own cache is released: frees what
   // idle allocators tied to this one (a context's batch workers) keep cached
   std::function<void()> on_oom;
@@ -95,12 +97,38 @@ struct DevAlloc {
     return hipMallocAsync(p, bytes, stream);
   }
 
-  // hipMallocAsync on this stream; on failure the cache is released and the pool trimmed once
+  // frees the oldest cached blocks until at least `want` bytes are out; returns the bytes freed
+  size_t evict_oldest(size_t want) {
+    size_t got = 0;
+    while (got < want && !cache.empty()) {
+      auto old = cache.begin();
+      for (auto it = cache.begin(); it != cache.end(); ++it)
+        if (it->second.second < old->second.second) old = it;
+      (void)hipFreeAsync(old->second.first, stream);
+      got += old->first;
+      cached_bytes -= old->first;
+      cache.erase(old);
+    }
+    return got;
+  }
+
+  // hipMallocAsync on this stream.  On failure the oldest cached blocks (as many bytes as the
+  // request) go back to the device first, then the whole cache; the pool is trimmed before
+  // each further try.  Blocks kept for reuse are what makes repeated products fast: mapping
+  // them again costs more than the products.
   void* raw_alloc(size_t bytes) {
     void* p = nullptr;
     hipError_t e = (oom_inject > 0 && --oom_inject == 0) ? hipErrorOutOfMemory : try_alloc(&p, bytes);
     if (e == hipErrorOutOfMemory) {
       oom_retries++;
+      (void)hipGetLastError();
+      if (evict_oldest(bytes) > 0) {
+        (void)hipStreamSynchronize(stream);
+        trim_pool();
+        e = try_alloc(&p, bytes);
+      }
+    }
+    if (e == hipErrorOutOfMemory) {
       (void)hipGetLastError();
       if (on_oom) on_oom();
       release_cache();
@@ -141,7 +169,7 @@ struct DevAlloc {
     if (bytes >= kCacheMin) {
       auto it = cache.lower_bound(bytes);
       if (it != cache.end() && it->first <= bytes + bytes / 2) {
-        p = it->second;
+        p = it->second.first;
         got = it->first;
         cached_bytes -= got;
         cache.erase(it);
@@ -172,7 +200,7 @@ struct DevAlloc {
     if (it != sizes.end()) sizes.erase(it);
     live_bytes = live_bytes > sz ? live_bytes - sz : 0;
     if (sz >= kCacheMin && cached_bytes + sz <= cache_cap) {
-      cache.emplace(sz, p);
+      cache.emplace(sz, std::make_pair(p, cache_seq++));
       cached_bytes += sz;
       return;
     }
@@ -186,7 +214,7 @@ struct DevAlloc {
     (void)hipGetLastError();
   }
   void release_cache() {
-    for (auto& kv : cache) (void)hipFreeAsync(kv.second, stream);
+    for (auto& kv : cache) (void)hipFreeAsync(kv.second.first, stream);
     cache.clear();
     cached_bytes = 0;
     if (zero_pins == 0) free_zero_block();
