@@ -2743,13 +2743,15 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
   C.sorted = true;
   C.distinct = true;
   C.ptr = al.alloc_n<int64_t>((int64_t)m + 1);
-  if (spgemm_dense(A, B, C, o, stats, al, tm)) {
+  SpgemmStats dstats;
+  if (spgemm_dense(A, B, C, o, stats ? stats : &dstats, al, tm)) {
     if (timing || tm || trace) SMCM_HIP(hipStreamSynchronize(st));
     if (tm) tm->flush();
     if (stats) stats->ms_numeric = ms_since(t0);
     if (trace)
-      fprintf(stderr, "[smcm] spgemm %dx%d * %dx%d: dense tier %.3f ms, nnz %lld\n", A.nrows, A.ncols, B.nrows,
-              B.ncols, ms_since(t0), (long long)C.nnz);
+      fprintf(stderr, "[smcm] spgemm %dx%d * %dx%d: dense tier %.3f ms, entries %lld, products %lld, nnz %lld\n",
+              A.nrows, A.ncols, B.nrows, B.ncols, ms_since(t0), (long long)A.nnz,
+              (long long)(stats ? stats : &dstats)->products, (long long)C.nnz);
     return;
   }
   SpgemmCore core(A, B, est_r, est_stride, o, al, tm);
@@ -2789,11 +2791,11 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
   }
   if (trace)
     fprintf(stderr,
-            "[smcm] spgemm %dx%d * %dx%d: analysis %.3f symbolic %.3f numeric(+alloc) %.3f ms, nnz %lld, "
-            "windows of %lld columns with %d-byte accumulators\n",
+            "[smcm] spgemm %dx%d * %dx%d: analysis %.3f symbolic %.3f numeric(+alloc) %.3f ms, products %lld, "
+            "nnz %lld, windows of %lld columns with %d-byte accumulators\n",
             A.nrows, A.ncols, B.nrows, B.ncols, std::chrono::duration<double, std::milli>(t1 - t0).count(),
-            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), (long long)nnz,
-            (long long)core.split.wl.w, core.split.wl.vbytes);
+            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), (long long)core.products,
+            (long long)nnz, (long long)core.split.wl.w, core.split.wl.vbytes);
 }
 
 void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_stride, int64_t block_bytes,
