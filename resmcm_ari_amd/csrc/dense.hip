@@ -369,6 +369,19 @@ void launch_scatter(const Csr& A, const Csr& B, int32_t nw, unsigned long long* 
   SMCM_LAUNCH_CHECK();
 }
 
+// products of A * B (the sum over A's entries of their B row lengths) into *out (zeroed by the
+// caller; one atomic per block)
+__global__ void __launch_bounds__(kDThreads) dense_products_kernel(const int32_t* __restrict__ acol, int64_t nnz_a,
+                                                                   const int64_t* __restrict__ bptr,
+                                                                   unsigned long long* __restrict__ out) {
+  unsigned long long s = 0;
+  for (int64_t e = (int64_t)blockIdx.x * kDThreads + threadIdx.x; e < nnz_a; e += (int64_t)gridDim.x * kDThreads) {
+    const int32_t k = acol[e];
+    s += (unsigned long long)(bptr[k + 1] - bptr[k]);
+  }
+  block_add<kDWaves>(s, out);
+}
+
 }  // namespace
 
 bool spgemm_dense(const Csr& A, const Csr& B, Csr& C, const SpgemmOpts& o, SpgemmStats* stats, DevAlloc& al,
@@ -378,6 +391,21 @@ bool spgemm_dense(const Csr& A, const Csr& B, Csr& C, const SpgemmOpts& o, Spgem
   if (o.dense_cells <= 0 || cells > o.dense_cells || o.force_tier >= 0 || al.arena || m == 0 || ncols == 0)
     return false;
   hipStream_t st = al.stream;
+  if (ncols > kLcombCols && o.dense_row_products > 0 && A.nnz > 0) {
+    // no LDS combining over this many columns: hub rows would send their products to the same
+    // columns by device atomics, which the row tiers' LDS windows do far faster
+    DevBuf<unsigned long long> cnt(al, 1);
+    fill_i64((int64_t*)cnt.p, 1, 0, st);
+    dense_products_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nnz, kDThreads), 1024), kDThreads, 0, st>>>(
+        A.col, A.nnz, B.ptr, cnt.p);
+    SMCM_LAUNCH_CHECK();
+    unsigned char* hs = host_stage() + kHostStageDense;
+    SMCM_HIP(hipMemcpyAsync(hs, cnt.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipStreamSynchronize(st));
+    unsigned long long products = 0;
+    memcpy(&products, hs, sizeof(products));
+    if ((double)products >= (double)o.dense_row_products * (double)m) return false;
+  }
   const int32_t nw = (int32_t)ceil_div(ncols, kWave);
   const int32_t nch = (int32_t)ceil_div(nw, kDChunk);
   const int64_t tasks = m * nch;

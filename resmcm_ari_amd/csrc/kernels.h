@@ -166,6 +166,10 @@ struct SpgemmOpts {
                                 // to the window kernel (LDS value slots are bypassed)
   int64_t dense_cells = (int64_t)1 << 23;  // products with rows(A) x cols(B) up to this many cells
                                            // accumulate whole in a dense device array (0 = never)
+  int64_t dense_row_products = 10000;  // ... unless B has more columns than a workgroup combines in
+                                       // LDS and the rows average at least this many products
+                                       // (hub rows: device atomics on their columns serialise;
+                                       // 0 = no such limit)
   int window_u32 = 1;       // window accumulators of 32 bits (twice the columns per window) when the
                             // operands' largest values bound every count below 2^32
   int64_t bbits_bytes = (int64_t)64 << 20;  // right operands whose column bitmap (rows x words) fits
@@ -191,7 +195,8 @@ void spgemm(const Csr& A, const Csr& B, const double* est_r, int est_stride, Csr
             KernelTimer* tm = nullptr);
 
 // Dense tier (dense.hip): C = A * B accumulated whole in a dense all-zero device array when
-// rows(A) x cols(B) <= o.dense_cells (and no tier is forced).  Returns false, having done
+// rows(A) x cols(B) <= o.dense_cells (and no tier is forced; products over wide B whose rows
+// average o.dense_row_products or more stay with the row tiers).  Returns false, having done
 // nothing, when the product does not qualify; C.ptr must be allocated (nrows + 1).
 bool spgemm_dense(const Csr& A, const Csr& B, Csr& C, const SpgemmOpts& o, SpgemmStats* st, DevAlloc& al,
                   KernelTimer* tm);

@@ -170,6 +170,28 @@ def test_dense_tier(ctx, case, vals):
             assert_same_csr(ptr, col, val, ref, what=f"dense {name} vals={vals} rep={rep}")
 
 
+@pytest.mark.parametrize("limit", [0, 10000])
+def test_dense_tier_wide_hub_rows(ctx, capfd, monkeypatch, limit):
+    """Over more columns than a workgroup combines in LDS, rows averaging at least
+    dense_row_products products stay with the row tiers (0 = no such limit: the dense tier takes
+    them); both routes give the oracle's result, and narrow products keep the dense tier."""
+    rng = np.random.default_rng(zlib.crc32(f"dense_wide/{limit}".encode()))
+    A = random_csr(rng, 50, 400, row_len=lambda r: r.integers(150, 250), with_vals=True)
+    B = random_csr(rng, 400, 20000, row_len=lambda r: r.integers(60, 140), with_vals=True)
+    A2 = random_csr(rng, 50, 400, row_len=lambda r: r.integers(150, 250), with_vals=True)
+    B2 = random_csr(rng, 400, 4000, row_len=lambda r: r.integers(60, 140), with_vals=True)
+    monkeypatch.setenv("SMCM_TRACE", "1")
+    with options(ctx, dense_cells=1 << 23, dense_row_products=limit):
+        assert ctx.get_option("dense_row_products") == limit
+        for a, b, wide in ((A, B, True), (A2, B2, False)):
+            ref = O.spgemm(a, b)
+            capfd.readouterr()
+            ptr, col, val = _run(ctx, a, b)
+            err = capfd.readouterr().err
+            assert_same_csr(ptr, col, val, ref, what=f"dense wide={wide} limit={limit}")
+            assert ("dense tier" in err) == (not wide or limit == 0), err[-400:]
+
+
 def test_dense_tier_wraps_mod_2_64(ctx):
     """Counts of the dense tier wrap mod 2^64; an entry whose sum wraps to 0 stays in the
     structure."""
