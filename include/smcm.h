@@ -81,7 +81,9 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
  * smcm_cost_estimate), "lgraph_seed", "window_major", "window_min_products", "window_cols",
  * "w_threads", "window_all_nwin", "window_presence", "sym_window_cols", "sym_merge", "dense_cells"
  * (products with rows x columns up to this many cells accumulate whole in a dense device array,
- * default 2^23, 0 = never), "bbits_bytes" (largest right operand kept as a column bitmap for the
+ * default 2^23, 0 = never), "dense_row_products" (products over more than 4096 columns whose rows
+ * average this many products or more stay with the row tiers; default 10000, 0 = no limit),
+ * "bbits_bytes" (largest right operand kept as a column bitmap for the
  * symbolic pass, default 64 MiB, 0 = never), "window_u32" (1 = 32-bit window accumulators when
  * every count of a product is bounded below 2^32, the default; 0 = always 64-bit), "oom_inject"
  * (tests of the out-of-memory recovery: the n-th device allocation from now reports out of
