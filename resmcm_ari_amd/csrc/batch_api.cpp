@@ -820,6 +820,16 @@ int smcm_batch_autotune(smcm_ctx* ctx, smcm_graph* g, int32_t n_chains, const in
       ctx->batch = bestb;
     }
     (void)saved;
+    // the trials' cached blocks are sized for the sample and its other settings: they go back to
+    // the device, so the tuned batch does not meet full caches (each out-of-memory retry on the
+    // way costs a synchronisation, a pool trim and new mappings)
+    for (smcm_ctx* w : ctx->batch_workers) {
+      w->al.release_cache();
+      SMCM_HIP(hipStreamSynchronize(w->stream));
+    }
+    ctx->al.release_cache();
+    SMCM_HIP(hipStreamSynchronize(ctx->stream));
+    smcm::DevAlloc::trim_pool();
     t.best_s = best;
     t.small_products = ctx->batch.small_products;
     t.threads = ctx->batch.threads;
