@@ -2598,6 +2598,10 @@ struct SpgemmCore {
       const long double arow = A.val ? (scan_a ? (long double)sh[5 * kNumBins] : 0x1p64L) : (long double)widest;
       const long double bmax = B.val ? (scan_b ? (long double)sh[5 * kNumBins + 1] : 0x1p64L) : 1.0L;
       const bool window_rows = tc.rows[kBitmapTier] + tc.rows[kHeavyTier] > 0;
+      if (window_rows && std::getenv("SMCM_TRACE"))
+        fprintf(stderr, "[smcm] count bound %dx%d * %dx%d: row sum of A %.4Lg (%s) x largest B %.4Lg (%s)\n", A.nrows,
+                A.ncols, B.nrows, B.ncols, arow, A.val ? (scan_a ? "scanned" : "not scanned") : "row length", bmax,
+                B.val ? (scan_b ? "scanned" : "not scanned") : "no values");
       if (o.window_u32 && split.wcnt && window_rows && arow * bmax < 4294967296.0L) {
         const WindowLayout w4 = choose_window(B.ncols, o, 4);
         if (w4.w > split.wl.w) {
@@ -2850,9 +2854,12 @@ void spgemm_blocked(const Csr& A, const Csr& B, const double* est_r, int est_str
   DevBuf<int64_t> bptr(al, (int64_t)m + 1);
   if (trace) {
     SMCM_HIP(hipStreamSynchronize(st));
-    fprintf(stderr, "[smcm] blocked %dx%d * %dx%d: analysis %.3f symbolic %.3f alloc %.3f ms, %zu blocks\n", A.nrows,
-            A.ncols, B.nrows, B.ncols, std::chrono::duration<double, std::milli>(t1 - t0).count(),
-            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), blocks.size());
+    fprintf(stderr,
+            "[smcm] blocked %dx%d * %dx%d: analysis %.3f symbolic %.3f alloc %.3f ms, %zu blocks, products %lld, "
+            "windows of %lld columns with %d-byte accumulators\n",
+            A.nrows, A.ncols, B.nrows, B.ncols, std::chrono::duration<double, std::milli>(t1 - t0).count(),
+            std::chrono::duration<double, std::milli>(t2 - t1).count(), ms_since(t2), blocks.size(),
+            (long long)core.products, (long long)core.split.wl.w, core.split.wl.vbytes);
   }
   for (auto [b, e] : blocks) {
     auto tb = std::chrono::steady_clock::now();
