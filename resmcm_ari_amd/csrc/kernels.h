@@ -172,6 +172,10 @@ struct SpgemmOpts {
                                        // 0 = no such limit)
   int window_u32 = 1;       // window accumulators of 32 bits (twice the columns per window) when the
                             // operands' largest values bound every count below 2^32
+  int64_t window_mixed_min = (int64_t)1 << 26;  // products with at least this many products, whose
+                                                // whole bound does not fit 32 bits, bound each row:
+                                                // rows that fit take 32 bits, the rest 64 (same
+                                                // windows, a second launch)
   int64_t bbits_bytes = (int64_t)64 << 20;  // right operands whose column bitmap (rows x words) fits
                                             // this take the symbolic pass as ORs of bitmap rows,
                                             // when that reads fewer words than 4x the products (0 = never)

@@ -156,6 +156,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "bbits_bytes") {
       if (v < 0) fail(SMCM_EINVAL, "bbits_bytes must be >= 0");
       o.bbits_bytes = v;
+    } else if (k == "window_mixed_min") {
+      if (v < 0) fail(SMCM_EINVAL, "window_mixed_min must be >= 0");
+      o.window_mixed_min = v;
     } else if (k == "dense_row_products") {
       if (v < 0) fail(SMCM_EINVAL, "dense_row_products must be >= 0");
       o.dense_row_products = v;
@@ -253,6 +256,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "window_all_nwin") *v = o.window_all_nwin;
     else if (k == "dense_cells") *v = o.dense_cells;
     else if (k == "dense_row_products") *v = o.dense_row_products;
+    else if (k == "window_mixed_min") *v = o.window_mixed_min;
     else if (k == "bbits_bytes") *v = o.bbits_bytes;
     else if (k == "window_u32") *v = o.window_u32;
     else if (k == "sym_window_cols") *v = o.sym_window_cols;

@@ -400,6 +400,104 @@ __global__ void __launch_bounds__(kThreads) row_sum_max_kernel(const int64_t* __
   }
 }
 
+// largest value of every row of B into rmax[k] (1 for every row when B has no values; 0 for an
+// empty row): a wave per row
+__global__ void __launch_bounds__(kThreads) row_max_kernel(const int64_t* __restrict__ bptr,
+                                                           const uint64_t* __restrict__ bval, int32_t nrows,
+                                                           unsigned long long* __restrict__ rmax) {
+  const int lane = lane_id();
+  for (int64_t k = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); k < nrows;
+       k += (int64_t)gridDim.x * (kThreads / kWave)) {
+    const int64_t rs = bptr[k], re = bptr[k + 1];
+    unsigned long long mx = 0;
+    if (!bval) {
+      mx = re > rs ? 1ull : 0ull;
+    } else {
+      for (int64_t e = rs + lane; e < re; e += kWave) mx = bval[e] > mx ? bval[e] : mx;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long y = __shfl_xor(mx, o);
+        mx = y > mx ? y : mx;
+      }
+    }
+    if (lane == 0) rmax[k] = mx;
+  }
+}
+
+// Bound of every count in row u of A * B: sum over the row's entries of A[u,e] (1 without values)
+// times the largest value of B's row k_e (saturating).  big[u] = 1 when the bound reaches 2^32;
+// *nbig counts such rows (one atomic per block).  A wave per row.
+__global__ void __launch_bounds__(kThreads) row_bound_kernel(const int64_t* __restrict__ aptr,
+                                                             const int32_t* __restrict__ acol,
+                                                             const uint64_t* __restrict__ aval, int32_t m,
+                                                             const unsigned long long* __restrict__ rmax,
+                                                             uint8_t* __restrict__ big,
+                                                             unsigned long long* __restrict__ nbig) {
+  __shared__ unsigned long long part[kThreads / kWave];
+  const int lane = lane_id();
+  unsigned long long mine = 0;
+  for (int64_t u = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); u < m;
+       u += (int64_t)gridDim.x * (kThreads / kWave)) {
+    const int64_t rs = aptr[u], re = aptr[u + 1];
+    unsigned long long sum = 0;
+    bool sat = false;
+    for (int64_t e = rs + lane; e < re; e += kWave) {
+      const unsigned long long a = aval ? aval[e] : 1ull, r = rmax[acol[e]];
+      const unsigned long long t = a * r;
+      sat |= __umul64hi(a, r) != 0;
+      const unsigned long long y = sum + t;
+      sat |= y < sum;
+      sum = y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long y = __shfl_xor(sum, o);
+      const unsigned long long z = sum + y;
+      sat |= z < sum;
+      sum = z;
+    }
+    const bool b = __ballot(sat) != 0 || sum >= (1ull << 32);
+    if (lane == 0) {
+      big[u] = b ? 1 : 0;
+      mine += b ? 1 : 0;
+    }
+  }
+  if (lane == 0) part[wave_id()] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kThreads / kWave; ++w) t += part[w];
+    if (t) atomicAdd(nbig, t);
+  }
+}
+
+// splits a row list by the big[] flags: rows with small bounds to small_out, the others to
+// big_out (counts[0], counts[1], zeroed by the caller; order within each list not kept)
+__global__ void __launch_bounds__(kThreads) split_big_kernel(const int32_t* __restrict__ rows, int32_t n,
+                                                             const uint8_t* __restrict__ big,
+                                                             int32_t* __restrict__ small_out,
+                                                             int32_t* __restrict__ big_out,
+                                                             int32_t* __restrict__ counts) {
+  const int lane = lane_id();
+  for (int64_t i0 = ((int64_t)blockIdx.x * (kThreads / kWave) + wave_id()) * kWave; i0 < n;
+       i0 += (int64_t)gridDim.x * kThreads) {
+    const int64_t i = i0 + lane;
+    const bool valid = i < n;
+    const int32_t u = valid ? rows[i] : 0;
+    const bool b = valid && big[u];
+    const uint64_t mb = __ballot(b), ms = __ballot(valid && !b);
+    int32_t bb = 0, bs = 0;
+    if (lane == 0) {
+      if (mb) bb = atomicAdd(&counts[1], (int32_t)__popcll(mb));
+      if (ms) bs = atomicAdd(&counts[0], (int32_t)__popcll(ms));
+    }
+    bb = __shfl(bb, 0);
+    bs = __shfl(bs, 0);
+    if (b) big_out[bb + lanes_below(mb)] = u;
+    else if (valid) small_out[bs + lanes_below(ms)] = u;
+  }
+}
+
 // sum of symbolic row widths per tier; the empty tier has no row list (its rows are only
 // counted), so blockIdx.y covers the tiers after it
 __global__ void __launch_bounds__(kThreads) tier_nnz_kernel(const int32_t* __restrict__ bins, int32_t m,
@@ -2108,6 +2206,9 @@ struct SplitScratch {
   int32_t* wcnt = nullptr;            // columns per (row, window) from the symbolic pass, or null
                                       // (then the window tier runs one row at a time)
   WindowLayout wl;
+  // with 4-byte windows: per row of A, 1 when its counts may reach 2^32 (those rows run in a
+  // second launch with 8-byte accumulators over the same windows), or null when every row fits
+  const uint8_t* big = nullptr;
   bool staged = false;  // split counts already copied to the host stage and synchronised
   struct SpgemmCore* owner = nullptr;
   DevAlloc* al = nullptr;             // the owner's allocator (per-launch task metadata)
@@ -2326,11 +2427,6 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
       const int tok = T.start("spgemm.numeric.window", st);
       if (split->wcnt) {
         const int32_t nwin = (int32_t)split->wl.nwin;
-        DevBuf<WinRow> info(*split->al, cnt[1]);
-        DevBuf<int32_t> pre(*split->al, (int64_t)cnt[1] * (nwin + 1));
-        window_prep_kernel<<<(unsigned)std::min<int64_t>(ceil_div(cnt[1], kThreads / kWave), 65536), kThreads, 0,
-                             st>>>(split->wide, cnt[1], A.ptr, cptr, split->wcnt, nwin, info.p, pre.p);
-        SMCM_LAUNCH_CHECK();
         // presence bits pay off when windows are sparsely filled (the compaction then touches few
         // slots); dense windows are cheaper with values alone (no per-product bit update)
         const double fill = (double)cls.c_nnz[1] / ((double)cnt[1] * nwin * (double)split->wl.w);
@@ -2338,17 +2434,48 @@ void launch_tiers(const Csr& A, const Csr& B, const int64_t* S, const int32_t* b
         // the compaction's presence pass reads a wave's run of words with one lane each
         if ((split->wl.w / 64 + o.w_threads / kWave - 1) / (o.w_threads / kWave) > kWave)
           throw HipError(-22, "window tier: more than 64 words per wave in a window");
-        auto kern = split->wl.vbytes == 4
-                        ? (pres ? window_major_kernel<AV, BV, true, uint32_t> : window_major_kernel<AV, BV, false, uint32_t>)
-                        : (pres ? window_major_kernel<AV, BV, true> : window_major_kernel<AV, BV, false>);
-        const unsigned gm = persistent_grid(o, occupancy((const void*)kern, o.w_threads, lds));
         if (!split->nseg_tried) {
           split->nseg = split->build_segs(B, 1, st);
           split->nseg_tried = true;
         }
-        kern<<<gm, o.w_threads, lds, st>>>(A.col, A.val, B.ptr, B.col, B.val, info.p, pre.p, cnt[1], B.ncols,
-                                           split->wl.log2w, nwin, split->bnd, split->nseg, B.nrows, queue, ccol,
-                                           cval);
+        // one persistent launch over `rows` with accumulators of vbytes bytes
+        auto run_rows = [&](const int32_t* rows, int32_t nrows, int vbytes) {
+          DevBuf<WinRow> info(*split->al, nrows);
+          DevBuf<int32_t> pre(*split->al, (int64_t)nrows * (nwin + 1));
+          window_prep_kernel<<<(unsigned)std::min<int64_t>(ceil_div(nrows, kThreads / kWave), 65536), kThreads, 0,
+                               st>>>(rows, nrows, A.ptr, cptr, split->wcnt, nwin, info.p, pre.p);
+          SMCM_LAUNCH_CHECK();
+          WindowLayout wl = split->wl;
+          wl.vbytes = vbytes;
+          const size_t lb = wl.bytes();
+          auto kern = vbytes == 4
+                          ? (pres ? window_major_kernel<AV, BV, true, uint32_t> : window_major_kernel<AV, BV, false, uint32_t>)
+                          : (pres ? window_major_kernel<AV, BV, true> : window_major_kernel<AV, BV, false>);
+          const unsigned gm = persistent_grid(o, occupancy((const void*)kern, o.w_threads, lb));
+          fill_i32(queue, 1, 0, st);
+          kern<<<gm, o.w_threads, lb, st>>>(A.col, A.val, B.ptr, B.col, B.val, info.p, pre.p, nrows, B.ncols,
+                                            split->wl.log2w, nwin, split->bnd, split->nseg, B.nrows, queue, ccol,
+                                            cval);
+          SMCM_LAUNCH_CHECK();
+        };
+        if (split->wl.vbytes == 4 && split->big) {
+          // rows whose counts may reach 2^32 take 8-byte accumulators over the same windows
+          DevBuf<int32_t> lists(*split->al, 2 * (int64_t)cnt[1] + 16);
+          int32_t* c2 = lists.p + 2 * (int64_t)cnt[1];
+          fill_i32(c2, 2, 0, st);
+          split_big_kernel<<<(unsigned)std::min<int64_t>(ceil_div(cnt[1], kThreads), 65536), kThreads, 0, st>>>(
+              split->wide, cnt[1], split->big, lists.p, lists.p + cnt[1], c2);
+          SMCM_LAUNCH_CHECK();
+          int32_t hc[2] = {0, 0};
+          SMCM_HIP(hipMemcpyAsync(hc, c2, sizeof(hc), hipMemcpyDeviceToHost, st));
+          SMCM_HIP(hipStreamSynchronize(st));
+          if (hc[0] < 0 || hc[1] < 0 || hc[0] + hc[1] != cnt[1])
+            throw HipError(-22, "window tier: row split lost rows");
+          if (hc[0] > 0) run_rows(lists.p, hc[0], 4);
+          if (hc[1] > 0) run_rows(lists.p + cnt[1], hc[1], 8);
+        } else {
+          run_rows(split->wide, cnt[1], split->wl.vbytes);
+        }
       } else {
         window_tier_kernel<AV, BV><<<g, o.w_threads, lds, st>>>(
             A.ptr, A.col, A.val, B.ptr, B.col, B.val, split->wide, cnt[1], B.ncols, split->wl.log2w,
@@ -2499,6 +2626,8 @@ struct SpgemmCore {
   unsigned long long* sums = nullptr;
   DevBuf<int32_t> split_rows, bnd, wcnt;
   DevBuf<longlong2> segbuf[2];
+  DevBuf<unsigned long long> rmax;  // largest value per row of B (+ a counter): mixed widths
+  DevBuf<uint8_t> big_rows;         // per row of A: counts may reach 2^32
   SplitScratch split;
   TierCounts tc;
   BitmapLayout lay;
@@ -2608,8 +2737,45 @@ struct SpgemmCore {
           split.wl = w4;
           set_all_lds_attrs(split.wl.bytes());
         }
+      } else if (o.window_u32 && split.wcnt && window_rows && products >= o.window_mixed_min) {
+        mixed_widths();
       }
     }
+  }
+
+  // Per-row count bounds (sum over the row of A[u,e] x the largest value of B's row k_e): when
+  // few rows can reach 2^32, the product takes 4-byte windows and those rows run in a second
+  // window launch with 8-byte accumulators over the same windows (which must fit the LDS).
+  void mixed_widths() {
+    hipStream_t st = al.stream;
+    const WindowLayout w4 = choose_window(B.ncols, o, 4);
+    WindowLayout w8 = w4;
+    w8.vbytes = 8;
+    if (w4.w <= split.wl.w || (int64_t)w8.bytes() > o.lds_bytes - kStaticLds) return;
+    rmax = DevBuf<unsigned long long>(al, (int64_t)B.nrows + 1);
+    unsigned long long* nbig = rmax.p + B.nrows;
+    fill_i64((int64_t*)nbig, 1, 0, st);
+    if (B.nrows > 0) {
+      row_max_kernel<<<(unsigned)std::min<int64_t>(ceil_div(B.nrows, kThreads / kWave), 65536), kThreads, 0, st>>>(
+          B.ptr, B.val, B.nrows, rmax.p);
+      SMCM_LAUNCH_CHECK();
+    }
+    big_rows = DevBuf<uint8_t>(al, A.nrows);
+    row_bound_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nrows, kThreads / kWave), 65536), kThreads, 0, st>>>(
+        A.ptr, A.col, A.val, A.nrows, rmax.p, big_rows.p, nbig);
+    SMCM_LAUNCH_CHECK();
+    unsigned long long hb = 0;
+    SMCM_HIP(hipMemcpyAsync(host_stage() + kStageNnz, nbig, sizeof(hb), hipMemcpyDeviceToHost, st));
+    SMCM_HIP(hipStreamSynchronize(st));
+    memcpy(&hb, host_stage() + kStageNnz, sizeof(hb));
+    if (std::getenv("SMCM_TRACE"))
+      fprintf(stderr, "[smcm] count bounds per row %dx%d * %dx%d: %llu of %d rows may reach 2^32\n", A.nrows, A.ncols,
+              B.nrows, B.ncols, hb, A.nrows);
+    // the second launch pays off while it takes a small share of the rows
+    if (hb * 8 > (unsigned long long)A.nrows) return;
+    split.wl = w4;
+    split.big = hb ? big_rows.p : nullptr;
+    set_all_lds_attrs(std::max(w4.bytes(), w8.bytes()));
   }
 
   // widths of the binned rows into row_nnz (other entries untouched).  The per-tier widths are

@@ -206,6 +206,44 @@ def test_dense_tier_wraps_mod_2_64(ctx):
     assert 0 in set(val.tolist())
 
 
+@pytest.mark.parametrize("nbig", [0, 5, 60])
+@pytest.mark.parametrize("mode", ["values", "presence"])
+def test_window_mixed_widths(ctx, capfd, monkeypatch, nbig, mode):
+    """Rows whose own count bound (sum of A[u,e] x the largest value of B's row) stays below 2^32
+    take 32-bit window accumulators even when the whole product's bound does not; the other rows
+    run in a second launch with 64-bit accumulators over the same windows.  When more than an
+    eighth of the rows need 64 bits, the product keeps 64-bit windows throughout.  Big rows carry
+    values near 2^63, so their counts wrap mod 2^64."""
+    rng = np.random.default_rng(zlib.crc32(f"mixed/{nbig}/{mode}".encode()))
+    window = 4096
+    A = random_csr(rng, 200, 300, row_len=lambda r: r.integers(2, 60))
+    B = random_csr(rng, 300, 20000, row_len=lambda r: r.integers(50, 900))
+    A.val = rng.integers(1, 50, size=A.nnz, dtype=np.uint64)
+    B.val = rng.integers(1, 60, size=B.nnz, dtype=np.uint64)
+    big = rng.choice(A.nrows, size=nbig, replace=False)
+    for u in big:
+        A.val[A.ptr[u]:A.ptr[u + 1]] = rng.integers(1 << 62, 1 << 63, size=A.ptr[u + 1] - A.ptr[u], dtype=np.uint64)
+    if nbig == 0:
+        # the whole-product bound (largest row sum of A x largest value of B) exceeds 2^32, but
+        # row 0's large value meets a row of B of ones: no row's own bound reaches 2^32
+        k0 = A.col[A.ptr[0]]
+        B.val[B.ptr[k0]:B.ptr[k0 + 1]] = 1
+        A.val[A.ptr[0]] = np.uint64(1 << 31)
+        assert int(A.val[A.ptr[0]:A.ptr[1]].astype(object).sum()) * int(B.val.max()) >= 1 << 32
+    ref = O.spgemm(A, B)
+    monkeypatch.setenv("SMCM_TRACE", "1")
+    with options(ctx, force_tier=2, bitmap_vcap=0, window_cols=window, window_min_products=0,
+                 window_presence=1 if mode == "presence" else 0, window_mixed_min=0):
+        capfd.readouterr()
+        ptr, col, val = _run(ctx, A, B)
+        err = capfd.readouterr().err
+    assert f"{nbig} of 200 rows may reach 2^32" in err, err[-600:]
+    mixed = nbig * 8 <= 200
+    want = f"windows of {window * 2 if mixed else window} columns with {4 if mixed else 8}-byte accumulators"
+    assert want in err, err[-600:]
+    assert_same_csr(ptr, col, val, ref, what=f"mixed widths nbig={nbig} {mode}")
+
+
 @pytest.mark.parametrize("route", ["dense", "window_major"])
 def test_out_of_memory_recovery(ctx, route):
     """Every device allocation of a product in turn reports out of memory once (injected on the
