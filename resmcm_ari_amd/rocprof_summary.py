@@ -183,18 +183,26 @@ def main(argv=None) -> int:
         out.append(f"| `{n}` | {c} | {t / 1e6:.3f} | {avg / 1e6:.3f} | {pct:.2f} |")
     out += ["", f"All kernels: {total / 1e6:.3f} ms.", ""]
 
+    # a timed launch may span several dispatches (the window tier's second, 64-bit launch for rows
+    # whose counts may reach 2^32): with the contract line's steps and warm-up, the trace's timed
+    # launches are counted and its figures are per timed launch, else per dispatch
+    per_launch = {}  # timed kernel -> dispatches per timed launch in the traced run
     if bench:
+        steps, warm = bench.get("steps"), bench.get("warmup")
         out += ["## HIP-event timing in the same run (bench.py `kernels`) vs the trace", "",
-                "| timed kernel | launches | HIP-event avg ms | trace avg ms | algorithmic GB/launch |",
-                "|---|---:|---:|---:|---:|"]
+                "| timed kernel | launches | HIP-event avg ms | trace avg ms | dispatches per launch | algorithmic GB/launch |",
+                "|---|---:|---:|---:|---:|---:|"]
         for tname, v in bench.get("kernels", {}).items():
             pat = TIMED.get(tname)
             names = match(pat, [r[0] for r in ks]) if pat else []
             calls = sum(r[1] for r in ks if r[0] in names)
-            tr = sum(r[2] for r in ks if r[0] in names) / calls / 1e6 if calls else float("nan")
+            traced = v["launches"] * (steps + warm) / steps if steps and warm is not None and v["launches"] else 0
+            per_launch[tname] = calls / traced if traced else 1.0
+            div = traced if traced else calls
+            tr = sum(r[2] for r in ks if r[0] in names) / div / 1e6 if div else float("nan")
             ev = v["ms"] / v["launches"] if v["launches"] else float("nan")  # per launch
             gb = v.get("bytes_per_launch", 0) / 1e9
-            out.append(f"| {tname} | {v['launches']} | {ev:.3f} | {tr:.3f} | {gb:.3f} |")
+            out.append(f"| {tname} | {v['launches']} | {ev:.3f} | {tr:.3f} | {per_launch[tname]:.2f} | {gb:.3f} |")
         out += ["", "The trace also covers the warm-up step; estimator.pass spans several kernels (light and heavy "
                 "rows, partial and final sums), so its trace column is a per-kernel average.", ""]
 
@@ -237,7 +245,7 @@ def main(argv=None) -> int:
         pat = TIMED.get(kernel)
         names = match(pat, fe.keys()) if pat else []
         if names:
-            d = sum(fe[n][0] for n in names)
+            d = sum(fe[n][0] for n in names) / per_launch.get(kernel, 1.0)  # timed launches
             fb = sum(fe[n][1] for n in names) * 1024 * f_scale
             wb = sum(wr[n][1] for n in names if n in wr) * 1024 * w_scale
             per = (fb + wb) / d
@@ -245,7 +253,7 @@ def main(argv=None) -> int:
             if bench and kernel in bench.get("kernels", {}):
                 v = bench["kernels"][kernel]
                 alg = v.get("bytes_per_launch") or None
-            out += [f"{kernel} ({', '.join('`' + n + '`' for n in names)}): {d} dispatches, "
+            out += [f"{kernel} ({', '.join('`' + n + '`' for n in names)}): {d:.0f} launches, "
                     f"{fb / d / 1e9:.3f} GB fetched + {wb / d / 1e9:.3f} GB written = {per / 1e9:.3f} GB per launch"
                     + (f"; algorithmic {alg / 1e9:.3f} GB per launch (ratio {per / alg:.2f})." if alg else "."), ""]
             traffic_line = f"traffic_bytes_per_launch: {kernel} {per:.0f}"
