@@ -85,7 +85,10 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
  * average this many products or more stay with the row tiers; default 10000, 0 = no limit),
  * "bbits_bytes" (largest right operand kept as a column bitmap for the
  * symbolic pass, default 64 MiB, 0 = never), "window_u32" (1 = 32-bit window accumulators when
- * every count of a product is bounded below 2^32, the default; 0 = always 64-bit), "oom_inject"
+ * every count of a product is bounded below 2^32, the default; 0 = always 64-bit),
+ * "window_mixed_min" (products with at least this many products, default 2^26, whose whole bound
+ * does not fit 32 bits bound each row: rows that fit take 32-bit windows, the rest a 64-bit
+ * launch over the same windows), "oom_inject"
  * (tests of the out-of-memory recovery: the n-th device allocation from now reports out of
  * memory once, then the allocator releases its cache and tries again), "headroom_mb" (device
  * memory kept free for the HIP runtime, default 2048: the scratch of a kernel with a private
