@@ -443,7 +443,7 @@ def main() -> int:
                     help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by source rows: each GPU runs the chain on a slice of start vertices)", file=sys.stderr)
         return 2
     if args.only:
