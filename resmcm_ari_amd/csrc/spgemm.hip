@@ -425,29 +425,33 @@ __global__ void __launch_bounds__(kThreads) row_max_kernel(const int64_t* __rest
 }
 
 // Bound of every count in row u of A * B: sum over the row's entries of A[u,e] (1 without values)
-// times the largest value of B's row k_e (saturating).  big[u] = 1 when the bound reaches 2^32;
-// *nbig counts such rows (one atomic per block).  A wave per row.
+// times the largest value of B's row k_e (saturating).  big[u] = 1 when the bound reaches 2^32.
+// stats[0] counts such rows, stats[1] is the largest product count among them and stats[2] their
+// product total (one atomic each per block).  A wave per row.
 __global__ void __launch_bounds__(kThreads) row_bound_kernel(const int64_t* __restrict__ aptr,
                                                              const int32_t* __restrict__ acol,
                                                              const uint64_t* __restrict__ aval, int32_t m,
+                                                             const int64_t* __restrict__ bptr,
                                                              const unsigned long long* __restrict__ rmax,
                                                              uint8_t* __restrict__ big,
-                                                             unsigned long long* __restrict__ nbig) {
-  __shared__ unsigned long long part[kThreads / kWave];
+                                                             unsigned long long* __restrict__ stats) {
+  __shared__ unsigned long long part[3][kThreads / kWave];
   const int lane = lane_id();
-  unsigned long long mine = 0;
+  unsigned long long mine = 0, pmax = 0, psum = 0;
   for (int64_t u = (int64_t)blockIdx.x * (kThreads / kWave) + wave_id(); u < m;
        u += (int64_t)gridDim.x * (kThreads / kWave)) {
     const int64_t rs = aptr[u], re = aptr[u + 1];
-    unsigned long long sum = 0;
+    unsigned long long sum = 0, prods = 0;
     bool sat = false;
     for (int64_t e = rs + lane; e < re; e += kWave) {
-      const unsigned long long a = aval ? aval[e] : 1ull, r = rmax[acol[e]];
+      const int32_t k = acol[e];
+      const unsigned long long a = aval ? aval[e] : 1ull, r = rmax[k];
       const unsigned long long t = a * r;
       sat |= __umul64hi(a, r) != 0;
       const unsigned long long y = sum + t;
       sat |= y < sum;
       sum = y;
+      prods += (unsigned long long)(bptr[k + 1] - bptr[k]);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -455,19 +459,36 @@ __global__ void __launch_bounds__(kThreads) row_bound_kernel(const int64_t* __re
       const unsigned long long z = sum + y;
       sat |= z < sum;
       sum = z;
+      prods += __shfl_xor(prods, o);
     }
     const bool b = __ballot(sat) != 0 || sum >= (1ull << 32);
     if (lane == 0) {
       big[u] = b ? 1 : 0;
-      mine += b ? 1 : 0;
+      if (b) {
+        mine += 1;
+        pmax = prods > pmax ? prods : pmax;
+        psum += prods;
+      }
     }
   }
-  if (lane == 0) part[wave_id()] = mine;
+  if (lane == 0) {
+    part[0][wave_id()] = mine;
+    part[1][wave_id()] = pmax;
+    part[2][wave_id()] = psum;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < kThreads / kWave; ++w) t += part[w];
-    if (t) atomicAdd(nbig, t);
+    unsigned long long t = 0, mx = 0, ps = 0;
+    for (int w = 0; w < kThreads / kWave; ++w) {
+      t += part[0][w];
+      mx = part[1][w] > mx ? part[1][w] : mx;
+      ps += part[2][w];
+    }
+    if (t) {
+      atomicAdd(&stats[0], t);
+      atomicMax(&stats[1], mx);
+      atomicAdd(&stats[2], ps);
+    }
   }
 }
 
@@ -2752,9 +2773,9 @@ struct SpgemmCore {
     WindowLayout w8 = w4;
     w8.vbytes = 8;
     if (w4.w <= split.wl.w || (int64_t)w8.bytes() > o.lds_bytes - kStaticLds) return;
-    rmax = DevBuf<unsigned long long>(al, (int64_t)B.nrows + 1);
-    unsigned long long* nbig = rmax.p + B.nrows;
-    fill_i64((int64_t*)nbig, 1, 0, st);
+    rmax = DevBuf<unsigned long long>(al, (int64_t)B.nrows + 3);
+    unsigned long long* nbig = rmax.p + B.nrows;  // big rows, their largest product count, their total
+    fill_i64((int64_t*)nbig, 3, 0, st);
     if (B.nrows > 0) {
       row_max_kernel<<<(unsigned)std::min<int64_t>(ceil_div(B.nrows, kThreads / kWave), 65536), kThreads, 0, st>>>(
           B.ptr, B.val, B.nrows, rmax.p);
@@ -2762,17 +2783,28 @@ struct SpgemmCore {
     }
     big_rows = DevBuf<uint8_t>(al, A.nrows);
     row_bound_kernel<<<(unsigned)std::min<int64_t>(ceil_div(A.nrows, kThreads / kWave), 65536), kThreads, 0, st>>>(
-        A.ptr, A.col, A.val, A.nrows, rmax.p, big_rows.p, nbig);
+        A.ptr, A.col, A.val, A.nrows, B.ptr, rmax.p, big_rows.p, nbig);
     SMCM_LAUNCH_CHECK();
-    unsigned long long hb = 0;
-    SMCM_HIP(hipMemcpyAsync(host_stage() + kStageNnz, nbig, sizeof(hb), hipMemcpyDeviceToHost, st));
+    unsigned long long hs[3] = {0, 0, 0};
+    SMCM_HIP(hipMemcpyAsync(host_stage() + kStageNnz, nbig, sizeof(hs), hipMemcpyDeviceToHost, st));
     SMCM_HIP(hipStreamSynchronize(st));
-    memcpy(&hb, host_stage() + kStageNnz, sizeof(hb));
+    memcpy(hs, host_stage() + kStageNnz, sizeof(hs));
+    const unsigned long long hb = hs[0];
+    // The second launch runs the big rows alone: it pays off while they are a small share of the
+    // rows and of the products, and while none of its (row, window) tasks, spread over the row's
+    // windows, outweighs a small part of what each CU does in the first launch (a hub row alone in
+    // a launch of few tasks would leave most CUs idle).
+    const double task_max = (double)hs[1] / (double)std::max<int64_t>(w4.nwin, 1);
+    const double cu_work = (double)products / (double)(std::max(o.cus, 1) * 2);
+    const bool take = hb * 8 <= (unsigned long long)A.nrows && (double)hs[2] * 8.0 <= (double)products &&
+                      task_max * 4.0 <= cu_work;
     if (std::getenv("SMCM_TRACE"))
-      fprintf(stderr, "[smcm] count bounds per row %dx%d * %dx%d: %llu of %d rows may reach 2^32\n", A.nrows, A.ncols,
-              B.nrows, B.ncols, hb, A.nrows);
-    // the second launch pays off while it takes a small share of the rows
-    if (hb * 8 > (unsigned long long)A.nrows) return;
+      fprintf(stderr,
+              "[smcm] count bounds per row %dx%d * %dx%d: %llu of %d rows may reach 2^32 (%llu of %lld products, "
+              "largest row %llu): %s\n",
+              A.nrows, A.ncols, B.nrows, B.ncols, hb, A.nrows, hs[2], (long long)products, hs[1],
+              take ? "mixed widths" : "8-byte windows");
+    if (!take) return;
     split.wl = w4;
     split.big = hb ? big_rows.p : nullptr;
     set_all_lds_attrs(std::max(w4.bytes(), w8.bytes()));

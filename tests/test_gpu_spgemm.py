@@ -206,7 +206,7 @@ def test_dense_tier_wraps_mod_2_64(ctx):
     assert 0 in set(val.tolist())
 
 
-@pytest.mark.parametrize("nbig", [0, 5, 60])
+@pytest.mark.parametrize("nbig", [0, 5, 600])
 @pytest.mark.parametrize("mode", ["values", "presence"])
 def test_window_mixed_widths(ctx, capfd, monkeypatch, nbig, mode):
     """Rows whose own count bound (sum of A[u,e] x the largest value of B's row) stays below 2^32
@@ -216,7 +216,7 @@ def test_window_mixed_widths(ctx, capfd, monkeypatch, nbig, mode):
     values near 2^63, so their counts wrap mod 2^64."""
     rng = np.random.default_rng(zlib.crc32(f"mixed/{nbig}/{mode}".encode()))
     window = 4096
-    A = random_csr(rng, 200, 300, row_len=lambda r: r.integers(2, 60))
+    A = random_csr(rng, 4000, 300, row_len=lambda r: r.integers(2, 60))
     B = random_csr(rng, 300, 20000, row_len=lambda r: r.integers(50, 900))
     A.val = rng.integers(1, 50, size=A.nnz, dtype=np.uint64)
     B.val = rng.integers(1, 60, size=B.nnz, dtype=np.uint64)
@@ -237,8 +237,8 @@ def test_window_mixed_widths(ctx, capfd, monkeypatch, nbig, mode):
         capfd.readouterr()
         ptr, col, val = _run(ctx, A, B)
         err = capfd.readouterr().err
-    assert f"{nbig} of 200 rows may reach 2^32" in err, err[-600:]
-    mixed = nbig * 8 <= 200
+    assert f"{nbig} of 4000 rows may reach 2^32" in err, err[-600:]
+    mixed = nbig * 8 <= 4000
     want = f"windows of {window * 2 if mixed else window} columns with {4 if mixed else 8}-byte accumulators"
     assert want in err, err[-600:]
     assert_same_csr(ptr, col, val, ref, what=f"mixed widths nbig={nbig} {mode}")
