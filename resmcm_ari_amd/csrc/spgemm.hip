@@ -671,7 +671,7 @@ __device__ __forceinline__ int64_t block_exscan_i64_once(int64_t x, int64_t* red
 // segment is found once per piece (wave-uniform LDS reads).
 constexpr int kChunk = 512;  // A entries per traversal round
 constexpr int64_t kSegMaxBytes = (int64_t)1 << 30;  // largest segment table of B per product
-constexpr int kTiny = 2;
+constexpr int kTiny = 4;  // segments of up to this many elements are walked by their own thread
 constexpr int kCompactBatch = 4;  // 64-column words per step of the window kernel's compaction
 constexpr int kQuarter = 16;
 
