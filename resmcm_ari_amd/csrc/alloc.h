@@ -63,9 +63,8 @@ struct DevAlloc {
   // takes the out-of-memory path instead (release, trim, retry, then an error the caller sees).
   size_t headroom = (size_t)2 << 30;
 
-  // can `bytes` be allocated without leaving less than `headroom` free on the device?  The
-  // pool's free part takes it when used + bytes fits its reservation; otherwise the pool grows
-  // by the difference, which must leave the reserve free.  Callers hold pool_mutex().
+  // can `bytes` be allocated without leaving less than `headroom` free on the device?  Callers
+  // hold pool_mutex().
   bool room_for(size_t bytes) const {
     if (headroom == 0) return true;
     int dev = 0;
@@ -78,12 +77,17 @@ struct DevAlloc {
       (void)hipGetLastError();
       return true;  // no pool figures: the allocation itself decides
     }
-    if (used + bytes <= reserved) return true;
+    // A small block fits in the pool's free part when used + bytes is within its reservation.  A
+    // large one is checked as if it were mapped anew: the pool's free part may be held in
+    // pieces smaller than the request (large freed blocks stay in this allocator's cache), and a
+    // pool that grows past the reserve leaves the runtime nothing for kernel scratch.
+    if (bytes < kCacheMin && used + bytes <= reserved) return true;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
       (void)hipGetLastError();
       return true;
     }
-    return used + bytes - reserved + headroom <= freeb;
+    const uint64_t grow = bytes >= kCacheMin ? bytes : used + bytes - reserved;
+    return grow + headroom <= freeb;
   }
   // one check-and-allocate at a time in the process: concurrent allocators (batch workers,
   // server workers) cannot pass the check together and then share the reserve out
