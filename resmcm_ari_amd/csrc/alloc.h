@@ -9,10 +9,18 @@
 // device pool trimmed before each further try (no device-wide wait: other contexts' streams keep
 // running).  An allocation also fails
 // when it would leave less than `headroom` bytes of the device free for the runtime.
+// With small_cache_cap > 0 freed small blocks are kept too (up to that many bytes), so that a
+// worker's steady state makes no device-pool calls: the runtime serialises them, and one that maps
+// a new large block for another thread holds every other allocation, small ones included, for
+// as long as the mapping takes (seconds for tens of GB).
 #pragma once
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -34,8 +42,10 @@ struct DevAlloc {
   std::vector<size_t> arena_stack;
   size_t live_bytes = 0;
   size_t peak_bytes = 0;
-  size_t cached_bytes = 0;
+  size_t cached_bytes = 0;  // large blocks (>= kCacheMin) in the cache
   size_t cache_cap = SIZE_MAX;
+  size_t cached_small = 0;  // small blocks in the cache
+  size_t small_cache_cap = 0;
   std::unordered_map<void*, size_t> sizes;
   std::multimap<size_t, std::pair<void*, uint64_t>> cache;  // size -> (block, insertion order)
   uint64_t cache_seq = 0;
@@ -101,6 +111,8 @@ struct DevAlloc {
     return hipMallocAsync(p, bytes, stream);
   }
 
+  void uncount(size_t sz) { (sz >= kCacheMin ? cached_bytes : cached_small) -= sz; }
+
   // frees the oldest cached blocks until at least `want` bytes are out; returns the bytes freed
   size_t evict_oldest(size_t want) {
     size_t got = 0;
@@ -110,7 +122,7 @@ struct DevAlloc {
         if (it->second.second < old->second.second) old = it;
       (void)hipFreeAsync(old->second.first, stream);
       got += old->first;
-      cached_bytes -= old->first;
+      uncount(old->first);
       cache.erase(old);
     }
     return got;
@@ -120,7 +132,25 @@ struct DevAlloc {
   // request) go back to the device first, then the whole cache; the pool is trimmed before
   // each further try.  Blocks kept for reuse are what makes repeated products fast: mapping
   // them again costs more than the products.
+  // With SMCM_TRACE set, an allocation that took the out-of-memory path or 50 ms or more is
+  // reported on stderr.
   void* raw_alloc(size_t bytes) {
+    static const bool trace = std::getenv("SMCM_TRACE") != nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t retries0 = oom_retries;
+    struct Report {
+      const DevAlloc& al;
+      size_t bytes;
+      std::chrono::steady_clock::time_point t0;
+      int64_t retries0;
+      ~Report() {
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms >= 50.0 || al.oom_retries != retries0)
+          fprintf(stderr, "[smcm] device allocation of %zu MiB: %.1f ms%s\n", bytes >> 20, ms,
+                  al.oom_retries != retries0 ? " (out-of-memory path: cached blocks released, pool trimmed)" : "");
+      }
+    };
+    std::unique_ptr<Report> report(trace ? new Report{*this, bytes, t0, retries0} : nullptr);
     void* p = nullptr;
     hipError_t e = (oom_inject > 0 && --oom_inject == 0) ? hipErrorOutOfMemory : try_alloc(&p, bytes);
     if (e == hipErrorOutOfMemory) {
@@ -170,12 +200,12 @@ struct DevAlloc {
     }
     void* p = nullptr;
     size_t got = bytes;
-    if (bytes >= kCacheMin) {
+    if (bytes >= kCacheMin || small_cache_cap > 0) {
       auto it = cache.lower_bound(bytes);
       if (it != cache.end() && it->first <= bytes + bytes / 2) {
         p = it->second.first;
         got = it->first;
-        cached_bytes -= got;
+        uncount(got);
         cache.erase(it);
       }
     }
@@ -203,9 +233,9 @@ struct DevAlloc {
     const size_t sz = it != sizes.end() ? it->second : 0;
     if (it != sizes.end()) sizes.erase(it);
     live_bytes = live_bytes > sz ? live_bytes - sz : 0;
-    if (sz >= kCacheMin && cached_bytes + sz <= cache_cap) {
+    if (sz >= kCacheMin ? cached_bytes + sz <= cache_cap : sz > 0 && cached_small + sz <= small_cache_cap) {
       cache.emplace(sz, std::make_pair(p, cache_seq++));
-      cached_bytes += sz;
+      (sz >= kCacheMin ? cached_bytes : cached_small) += sz;
       return;
     }
     if (hipFreeAsync(p, stream) != hipSuccess) (void)hipGetLastError();
@@ -221,6 +251,7 @@ struct DevAlloc {
     for (auto& kv : cache) (void)hipFreeAsync(kv.second.first, stream);
     cache.clear();
     cached_bytes = 0;
+    cached_small = 0;
     if (zero_pins == 0) free_zero_block();
   }
 };

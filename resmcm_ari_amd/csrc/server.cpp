@@ -495,6 +495,9 @@ int smcm_server_create(smcm_ctx* ctx, smcm_graph* const* graphs, int32_t n_graph
       w->exact_cost_sum = ctx->exact_cost_sum;
       w->plan_cost = o.plan_cost;
       w->al.cache_cap = ctx->server.worker_cache;  // the rest of the freed memory goes to the pool
+      // small blocks too (at most 1 GiB of them): a short query's allocations then wait for no
+      // device-pool call, which another worker's mapping of a large block can hold for seconds
+      w->al.small_cache_cap = std::min<size_t>(ctx->server.worker_cache, (size_t)1 << 30);
       w->al.headroom = ctx->al.headroom;
       s->wctx.push_back(w);
       const bool large_lane = o.large_slots > 0 && o.large_slots < o.slots && i >= o.slots - o.large_slots;
