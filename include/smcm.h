@@ -93,7 +93,9 @@ int smcm_ctx_destroy(smcm_ctx* ctx);
  * memory once, then the allocator releases its cache and tries again), "headroom_mb" (device
  * memory kept free for the HIP runtime, default 2048: the scratch of a kernel with a private
  * segment is allocated at its dispatch, and a failure there aborts the process, so an allocation
- * that would leave less free fails with SMCM_ENOMEM instead), "budget" (device
+ * that would leave less free fails with SMCM_ENOMEM instead), "small_cache_mb" (freed device
+ * blocks below 64 MiB the context keeps for reuse, default 256; blocks of 64 MiB and more are always
+ * kept), "budget" (device
  * bytes the product-aware plan may assume: products whose estimated operands and output exceed it
  * are not planned when another plan avoids them). */
 int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t value);

@@ -9,10 +9,11 @@
 // device pool trimmed before each further try (no device-wide wait: other contexts' streams keep
 // running).  An allocation also fails
 // when it would leave less than `headroom` bytes of the device free for the runtime.
-// With small_cache_cap > 0 freed small blocks are kept too (up to that many bytes), so that a
-// worker's steady state makes no device-pool calls: the runtime serialises them, and one that maps
-// a new large block for another thread holds every other allocation, small ones included, for
-// as long as the mapping takes (seconds for tens of GB).
+// Freed small blocks are kept too, up to small_cache_cap bytes (0 = none), so that repeated calls
+// make no device-pool calls in their steady state: each costs microseconds (a third of a 1.2 ms
+// chain of three products went to them), and the runtime serialises them, so that one mapping a
+// new large block for another thread holds every other allocation, small ones included, for as
+// long as the mapping takes (seconds for tens of GB).
 #pragma once
 
 #include <chrono>
@@ -45,7 +46,7 @@ struct DevAlloc {
   size_t cached_bytes = 0;  // large blocks (>= kCacheMin) in the cache
   size_t cache_cap = SIZE_MAX;
   size_t cached_small = 0;  // small blocks in the cache
-  size_t small_cache_cap = 0;
+  size_t small_cache_cap = (size_t)256 << 20;
   std::unordered_map<void*, size_t> sizes;
   std::multimap<size_t, std::pair<void*, uint64_t>> cache;  // size -> (block, insertion order)
   uint64_t cache_seq = 0;

@@ -220,6 +220,9 @@ int smcm_set_option(smcm_ctx* ctx, const char* key, int64_t v) {
     } else if (k == "headroom_mb") {
       if (v < 0) fail(SMCM_EINVAL, "headroom_mb must be >= 0");
       ctx->al.headroom = (size_t)v << 20;
+    } else if (k == "small_cache_mb") {
+      if (v < 0) fail(SMCM_EINVAL, "small_cache_mb must be >= 0");
+      ctx->al.small_cache_cap = (size_t)v << 20;
     } else if (k == "oom_inject") {
       if (v < 0) fail(SMCM_EINVAL, "oom_inject must be >= 0");
       ctx->al.oom_inject = v;
@@ -280,6 +283,7 @@ int smcm_get_option(smcm_ctx* ctx, const char* key, int64_t* v) {
     else if (k == "kernel_timing") *v = ctx->timer.enabled;
     else if (k == "oom_inject") *v = ctx->al.oom_inject;
     else if (k == "headroom_mb") *v = (int64_t)(ctx->al.headroom >> 20);
+    else if (k == "small_cache_mb") *v = (int64_t)(ctx->al.small_cache_cap >> 20);
     else if (k == "server_cache_mb") *v = (int64_t)(ctx->server.worker_cache >> 20);
     else if (k == "batch_cache_mb")
       *v = ctx->batch.worker_cache == SIZE_MAX ? INT64_MAX : (int64_t)(ctx->batch.worker_cache >> 20);

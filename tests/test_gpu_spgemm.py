@@ -262,7 +262,8 @@ def test_out_of_memory_recovery(ctx, route):
         opts = dict(WIDE_MODES["window_major"], dense_cells=0)
     ref = O.spgemm(A, B)
     injected = 0
-    with options(ctx, **opts):
+    # small blocks are not kept either, so that every allocation of the call is a device allocation
+    with options(ctx, small_cache_mb=0, **opts):
         _run(ctx, A, B)  # the all-zero block and code objects exist before the injections
         for k in range(1, 64):
             ctx.set_option("oom_inject", k)
@@ -286,7 +287,8 @@ def test_device_reserve_reports_enomem(ctx):
     B = random_csr(rng, 200, 150, row_len=lambda r: r.integers(0, 8), with_vals=True)
     ref = O.spgemm(A, B)
     assert ctx.get_option("headroom_mb") == 2048
-    with options(ctx, headroom_mb=1 << 40):  # more than any device holds
+    # more than any device holds; no kept small blocks, so the call allocates on the device
+    with options(ctx, headroom_mb=1 << 40, small_cache_mb=0):
         # the first allocation takes the failure path: the pool is trimmed, so the retry has to
         # grow it and meets the reserve
         ctx.set_option("oom_inject", 1)
