@@ -1525,7 +1525,7 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
     const int64_t width = cptr[u + 1] - rowbase;
     // rows with at most `chunk` entries keep their entry (B row, A value, position) in registers
     const bool fast = re - rs <= chunk;
-    int32_t my_k = 0, my_len = 0;
+    int32_t my_k = 0;
     int64_t my_b0 = 0;
     uint64_t my_a = 1ull;
     // window range of the row: first / last window holding one of its products
@@ -1540,7 +1540,6 @@ __global__ void window_tier_kernel(const int64_t* __restrict__ aptr, const int32
       if (fast) {
         my_k = k;
         my_b0 = b0;
-        my_len = len;
         my_a = AV ? aval[e] : 1ull;
       }
     }
